@@ -1,0 +1,126 @@
+/*
+ * lbx_oracle.h -- CPU restatement of LinBox's FFT polynomial-matrix product.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in linbox_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * Each function restates one piece of the reference (LinBox 1.7.1); the
+ * file:line it follows is given next to it.  Every function is pinned by an
+ * independent check in tests/test_oracle.py (Horner known answers, brute-force
+ * big-integer convolution, committed golden fixtures, closed forms):
+ *
+ *   or_compute_primitive_root  fft-utils.h:83-104        pinned: order check + brute force
+ *   or_init_powers             fft-integral.inl:1614-1650 pinned: closed form w^i
+ *   or_fft_dif / or_fft_dit    fft-integral.inl:100-124,204-317,366-558  pinned: Horner KAT
+ *   or_polymat_mul_fft_prime   matpoly-mult-fft-wordsize-fast.inl:62-190 pinned: naive + golden
+ *   or_polymat_mul_naive       matpoly-mult-naive.h:56-82  pinned: Python big-int golden
+ *   or_generate_fft_primes     random-fftprime.h:107-166   pinned: sympy-free MR + golden list
+ *   or_polymat_mul_crt         matpoly-mult-fft-multiprecision.inl:141-277,611-630 pinned: golden
+ *
+ * Arithmetic is exact integer arithmetic in uint64_t / unsigned __int128.
+ * The reference's lazy Harvey butterflies keep representatives in [0,2p) or
+ * [0,4p) and reduce at the end (fft-integral.inl:140-190); every intermediate
+ * is congruent mod p to the exact value used here and the final outputs are
+ * the canonical representatives in [0,p), so outputs are identical.
+ */
+#ifndef LBX_ORACLE_H
+#define LBX_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes, same numbering as include/lbx/lbx.h */
+#define OR_OK 0
+#define OR_ERR_BAD_FFT_PRIME 1
+#define OR_ERR_K_TOO_LARGE 2
+#define OR_ERR_K_NOT_POSITIVE 3
+#define OR_ERR_NOT_PRIMITIVE_ROOT 4
+#define OR_ERR_SIZE_MISMATCH 5
+#define OR_ERR_NOT_ENOUGH_PRIMES 6
+#define OR_ERR_INVALID_ARG 7
+
+/* ---- deterministic input generator (shared with the bench input spec) ---- */
+uint64_t or_splitmix64(uint64_t *state);
+/* Fill out[0..n) with values uniform in [0,p) from xoshiro256** seeded by
+ * splitmix64(seed); Lemire's multiply-shift rejection for uniformity. */
+void or_fill_uniform(uint64_t seed, uint32_t p, uint32_t *out, size_t n);
+
+/* ---- prime-field helpers ---- */
+uint32_t or_powmod(uint32_t a, uint64_t e, uint32_t p);
+uint32_t or_invmod(uint32_t a, uint32_t p);
+int or_is_prime_u64(uint64_t n); /* deterministic Miller-Rabin, n < 2^64 */
+
+/* fft-utils.h:83-104: first t=2,3,... with w=t^(m*2^(v-k)) of exact order 2^k */
+int or_compute_primitive_root(uint32_t p, unsigned k, uint32_t *w);
+/* fft-utils.h:65-78 */
+int or_is_primitive_root(uint32_t p, uint32_t w, unsigned k);
+
+/* fft-integral.inl:1614-1650: pow_w (n-1 entries) and Shoup constants
+ * pow_wp[i] = floor(pow_w[i] * 2^32 / p). */
+void or_init_powers(uint32_t p, unsigned L, uint32_t w, uint32_t *pow_w, uint32_t *pow_wp);
+
+/* FFT_base::DIF (fft-integral.inl:100-106, NoSimd DIF_core :204-217):
+ * natural order in, bit-reversed order out, root w. */
+void or_fft_dif(uint32_t p, unsigned L, const uint32_t *pow_w, uint32_t *coeffs);
+/* FFT_base::DIT (fft-integral.inl:117-124, NoSimd DIT_core :284-299):
+ * bit-reversed order in, natural order out, root w. */
+void or_fft_dit(uint32_t p, unsigned L, const uint32_t *pow_w, uint32_t *coeffs);
+
+/* FFT<Field>(F,L,w) + FFT_direct / FFT_inverse (fft.h:57-89,133-144).
+ * root==0 selects compute_primitive_root.  *root_used receives the root. */
+int or_fft_direct(uint32_t p, unsigned L, uint32_t root, uint32_t *coeffs, uint32_t *root_used);
+int or_fft_inverse(uint32_t p, unsigned L, uint32_t root, uint32_t *coeffs, uint32_t *root_used);
+/* fft.h:119-130 */
+uint32_t or_fft_invroot(uint32_t p, unsigned L, uint32_t w);
+
+/* PolynomialMatrixFFTPrimeMulDomain::mul (matpoly-mult-fft-wordsize-fast.inl:62-82)
+ * + mul_fft (:86-190).  All operands polfirst: entry e, coefficient t at
+ * [e*size+t].  deg = max_rowdeg ? max_rowdeg : sa+sb-2; c must hold
+ * m*n*(deg+1) words; entries written at [e*(deg+1)+t]. */
+int or_polymat_mul_fft_prime(uint32_t p, size_t m, size_t k, size_t n,
+                             const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
+                             uint32_t *c, size_t max_rowdeg);
+
+/* Stage pieces of mul_fft, used to time a bounded sample on the host:
+ * forward DIF on `count` polynomials of length 2^L (in place),
+ * pointwise m x k . k x n product mod p at `npts` points of polfirst data
+ * (pointwise over point indices [pt0, pt0+npts)), inverse DIT on `count`. */
+void or_stage_forward(uint32_t p, unsigned L, const uint32_t *pow_w, uint32_t *polys, size_t count);
+void or_stage_pointwise(uint32_t p, size_t m, size_t k, size_t n, size_t pts,
+                        const uint32_t *ahat, const uint32_t *bhat, uint32_t *chat,
+                        size_t pt0, size_t npts);
+void or_stage_inverse(uint32_t p, unsigned L, const uint32_t *pow_winv, uint32_t *polys, size_t count);
+
+/* PolynomialMatrixNaiveMulDomain::mul (matpoly-mult-naive.h:56-82):
+ * c[t] = sum_{i+j=t} a[i] b[j] mod p, t < sa+sb-1, polfirst, c stride sa+sb-1. */
+int or_polymat_mul_naive(uint32_t p, size_t m, size_t k, size_t n,
+                         const uint32_t *a, size_t sa, const uint32_t *b, size_t sb, uint32_t *c);
+
+/* RandomFFTPrime::generatePrimes (random-fftprime.h:107-133) with
+ * compute_max_m (:161-166): the largest m*2^k+1 < pbound, m odd, for
+ * k = bitsize(pbound-2)-1 down to kmin+1, until the product exceeds
+ * prodbound (given as little-endian u32 limbs).  Returns the count written
+ * or -1 if not enough primes fit in max_primes / the k range. */
+int or_generate_fft_primes(uint64_t pbound, const uint32_t *prodbound, int nlimbs,
+                           unsigned kmin, uint32_t *primes, int max_primes);
+
+/* Multiprecision modulus P (little-endian u32 limbs, nlimbs <= 16).
+ * Product over Z/PZ following PolynomialMatrixFFTMulDomain<Modular<integer>>::mul
+ * (matpoly-mult-fft-multiprecision.inl:611-630) and ZRing<integer>::mul_crtla
+ * (:141-277): bound 2 P^2 k min(sa,sb), FFT primes from or_generate_fft_primes
+ * with pbound 2^31 and kmin = L (so 2^L | p_i - 1), per-prime product, CRT,
+ * reduction mod P.  a,b,c hold nlimbs words per coefficient, polfirst,
+ * c stride sa+sb-1.  primes_used (optional, >=32 entries) / nprimes report the basis. */
+int or_polymat_mul_crt(const uint32_t *P, int nlimbs, size_t m, size_t k, size_t n,
+                       const uint32_t *a, size_t sa, const uint32_t *b, size_t sb, uint32_t *c,
+                       uint32_t *primes_used, int *nprimes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,155 @@
+"""ctypes wrapper over oracle/_build/liblbx_oracle.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liblbx_oracle.so")
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        sz, u32, u64, i32 = ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        L.or_fill_uniform.argtypes = [u64, u32, _u32p, sz]
+        L.or_powmod.argtypes = [u32, u64, u32]
+        L.or_powmod.restype = u32
+        L.or_invmod.argtypes = [u32, u32]
+        L.or_invmod.restype = u32
+        L.or_is_prime_u64.argtypes = [u64]
+        L.or_compute_primitive_root.argtypes = [u32, ctypes.c_uint, _u32p]
+        L.or_is_primitive_root.argtypes = [u32, u32, ctypes.c_uint]
+        L.or_init_powers.argtypes = [u32, ctypes.c_uint, u32, _u32p, _u32p]
+        L.or_fft_dif.argtypes = [u32, ctypes.c_uint, _u32p, _u32p]
+        L.or_fft_dit.argtypes = [u32, ctypes.c_uint, _u32p, _u32p]
+        L.or_fft_direct.argtypes = [u32, ctypes.c_uint, u32, _u32p, _u32p]
+        L.or_fft_inverse.argtypes = [u32, ctypes.c_uint, u32, _u32p, _u32p]
+        L.or_fft_invroot.argtypes = [u32, ctypes.c_uint, u32]
+        L.or_fft_invroot.restype = u32
+        L.or_polymat_mul_fft_prime.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz]
+        L.or_polymat_mul_naive.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p]
+        L.or_generate_fft_primes.argtypes = [u64, _u32p, i32, ctypes.c_uint, _u32p, i32]
+        L.or_polymat_mul_crt.argtypes = [_u32p, i32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, _u32p,
+                                         ctypes.POINTER(i32)]
+        L.or_stage_forward.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
+        L.or_stage_inverse.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
+        L.or_stage_pointwise.argtypes = [u32, sz, sz, sz, sz, _u32p, _u32p, _u32p, sz, sz]
+        _lib = L
+    return _lib
+
+
+def _p(arr):
+    assert arr.dtype == np.uint32 and arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(_u32p)
+
+
+def fill_uniform(seed, p, n):
+    out = np.empty(n, dtype=np.uint32)
+    lib().or_fill_uniform(seed, p, _p(out), n)
+    return out
+
+
+def primitive_root(p, k):
+    w = ctypes.c_uint32(0)
+    st = lib().or_compute_primitive_root(p, k, ctypes.byref(w))
+    if st:
+        raise ValueError("oracle status %d" % st)
+    return w.value
+
+
+def init_powers(p, L, w):
+    n = 1 << L
+    pw = np.empty(n - 1, dtype=np.uint32)
+    pwp = np.empty(n - 1, dtype=np.uint32)
+    lib().or_init_powers(p, L, w, _p(pw), _p(pwp))
+    return pw, pwp
+
+
+def fft_direct(p, L, coeffs, root=0):
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32).copy()
+    w = ctypes.c_uint32(0)
+    st = lib().or_fft_direct(p, L, root, _p(c), ctypes.byref(w))
+    if st:
+        raise ValueError("oracle status %d" % st)
+    return c, w.value
+
+
+def fft_inverse(p, L, coeffs, root=0):
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32).copy()
+    w = ctypes.c_uint32(0)
+    st = lib().or_fft_inverse(p, L, root, _p(c), ctypes.byref(w))
+    if st:
+        raise ValueError("oracle status %d" % st)
+    return c, w.value
+
+
+def invroot(p, L, w):
+    return lib().or_fft_invroot(p, L, w)
+
+
+def mul_fft_prime(p, m, k, n, a, sa, b, sb, max_rowdeg=0):
+    """Polfirst product; returns (status, c) with c of shape (m*n, deg+1)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    deg = max_rowdeg if max_rowdeg else sa + sb - 2
+    c = np.zeros(m * n * (deg + 1), dtype=np.uint32)
+    st = lib().or_polymat_mul_fft_prime(p, m, k, n, _p(a), sa, _p(b), sb, _p(c), max_rowdeg)
+    return st, c
+
+
+def mul_naive(p, m, k, n, a, sa, b, sb):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    c = np.zeros(m * n * (sa + sb - 1), dtype=np.uint32)
+    st = lib().or_polymat_mul_naive(p, m, k, n, _p(a), sa, _p(b), sb, _p(c))
+    assert st == 0
+    return c
+
+
+def generate_fft_primes(pbound, prodbound, kmin, max_primes=64):
+    nl = max(1, (prodbound.bit_length() + 31) // 32)
+    limbs = np.array([(prodbound >> (32 * i)) & 0xFFFFFFFF for i in range(nl)], dtype=np.uint32)
+    out = np.zeros(max_primes, dtype=np.uint32)
+    r = lib().or_generate_fft_primes(pbound, _p(limbs), nl, kmin, _p(out), max_primes)
+    return None if r < 0 else [int(x) for x in out[:r]]
+
+
+def mul_crt(P_limbs, m, k, n, a, sa, b, sb):
+    P_limbs = np.ascontiguousarray(P_limbs, dtype=np.uint32)
+    nl = len(P_limbs)
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    c = np.zeros(m * n * (sa + sb - 1) * nl, dtype=np.uint32)
+    primes = np.zeros(64, dtype=np.uint32)
+    npr = ctypes.c_int(0)
+    st = lib().or_polymat_mul_crt(_p(P_limbs), nl, m, k, n, _p(a), sa, _p(b), sb, _p(c), _p(primes),
+                                  ctypes.byref(npr))
+    return st, c, [int(x) for x in primes[:npr.value]]
+
+
+def stage_forward(p, L, pow_w, polys, count):
+    lib().or_stage_forward(p, L, _p(pow_w), _p(polys), count)
+
+
+def stage_inverse(p, L, pow_w, polys, count):
+    lib().or_stage_inverse(p, L, _p(pow_w), _p(polys), count)
+
+
+def stage_pointwise(p, m, k, n, pts, ahat, bhat, chat, pt0, npts):
+    lib().or_stage_pointwise(p, m, k, n, pts, _p(ahat), _p(bhat), _p(chat), pt0, npts)
