@@ -1,0 +1,199 @@
+"""Benchmark: bit-exact FFT polynomial-matrix products per second on one MI355X.
+
+Workload (default): 128x128 . 128x128 polynomial matrices over p = 2013265921
+(15*2^27+1), entries of degree 8191 with coefficients uniform in [0,p),
+product of degree 16382 through length-16384 NTTs.  A "step" is one whole
+product c = a*b (forward NTTs, pointwise stage, inverse NTTs) with a, b and c
+resident in HBM.  `value` is products/s of the INT8-MFMA form; the plain
+uint64 vector-ALU form is timed the same way for the ratio.
+
+The JSON line also carries:
+  roofline      the dominant stage's algorithmic HBM bytes per launch divided by
+                its average duration, measured live with HIP events recorded on
+                the launch stream during the timed steps;
+  cpu_baseline  the oracle's restatement of the reference's single-prime
+                mul_fft timed on this host on a bounded sample of the same
+                workload and extrapolated to one product.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P = 2013265921
+M = K = N = 128
+SA = SB = 8192
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(m, k, n, sa, sb, pts):
+    """HBM bytes each stage must move at least once (4-byte words)."""
+    deg = sa + sb - 2
+    return {
+        "direct FFT_DIF": 4 * (m * k * (sa + pts) + k * n * (sb + pts)),
+        "Polfirst to Matfirst": 4 * 2 * (m * k + k * n) * pts,
+        "Pointwise mult": 4 * (m * k + k * n + m * n) * pts,
+        "Matfirst to Polfirst": 4 * 2 * m * n * pts,
+        "inverse FFT_DIT": 4 * m * n * (pts + deg + 1),
+    }
+
+
+def run_gpu(args):
+    import linbox_amd as lbx
+    from linbox_amd import hip, PolynomialMatrix, PolynomialMatrixMulDomain
+
+    ctx = lbx.Context(0)
+    rng = np.random.default_rng(0x4C494E424F58 + 2)
+    a = rng.integers(0, P, M * K * SA, dtype=np.uint32)
+    b = rng.integers(0, P, K * N * SB, dtype=np.uint32)
+    sc = SA + SB - 1
+    da, db = hip.DeviceBuffer.from_numpy(a), hip.DeviceBuffer.from_numpy(b)
+    dc = hip.DeviceBuffer(M * N * sc * 4)
+    del a, b
+    stream = hip.Stream()
+    A, B = PolynomialMatrix(M, K, SA, data=da), PolynomialMatrix(K, N, SB, data=db)
+    C = PolynomialMatrix(M, N, sc, data=dc)
+
+    def timed(method, steps, warmup):
+        dom = PolynomialMatrixMulDomain(P, ctx, method, stream=stream.value)
+        for _ in range(warmup):
+            dom.mul(C, A, B)
+        stream.synchronize()
+        hip.synchronize()
+        ctx.profile(True)
+        ctx.stage_times()  # drop anything recorded before the timed region
+        e0, e1 = hip.Event(), hip.Event()
+        t0 = time.perf_counter()
+        e0.record(stream.value)
+        for _ in range(steps):
+            dom.mul(C, A, B)
+        e1.record(stream.value)
+        e1.synchronize()
+        wall = time.perf_counter() - t0
+        hip.synchronize()
+        ctx.profile(False)
+        stages = ctx.stage_times()
+        return e0.elapsed_ms(e1) / steps, wall, stages
+
+    ms_int8, wall_int8, stages_int8 = timed(lbx.METHOD_INT8_MFMA, args.steps, args.warmup)
+    print("int8 form: %.3f ms/product" % ms_int8, file=sys.stderr, flush=True)
+    plain_steps = max(2, args.steps // 4)
+    ms_plain, _, stages_plain = timed(lbx.METHOD_PLAIN_U64, plain_steps, 1)
+    print("plain form: %.3f ms/product" % ms_plain, file=sys.stderr, flush=True)
+    ctx.close()
+    return ms_int8, wall_int8, stages_int8, ms_plain, stages_plain
+
+
+def cpu_baseline(budget_s):
+    """Oracle (C restatement of mul_fft, one thread) on a bounded sample of the workload.
+
+    Times forward DIF transforms of 2^14 points, the pointwise 128x128x128
+    product at single points and inverse DIT transforms, each repeated until
+    budget_s/3 seconds pass, then extrapolates to one product:
+      t = (m*k + k*n) t_fwd + pts t_point + m*n t_inv.
+    """
+    from oracle import oracle as orc
+    L = 14
+    pts = 1 << L
+    w = orc.primitive_root(P, L)
+    pw, _ = orc.init_powers(P, L, w)
+    wi = orc.invroot(P, L, w)
+    pwi, _ = orc.init_powers(P, L, wi)
+    rng = np.random.default_rng(7)
+
+    def per_item(fn, make, chunk):
+        done, t = 0, 0.0
+        while t < budget_s / 3:
+            data = make(chunk)
+            t0 = time.perf_counter()
+            fn(data, chunk)
+            t += time.perf_counter() - t0
+            done += chunk
+        return t / done, done
+
+    polys = lambda c: rng.integers(0, P, c * pts, dtype=np.uint32)
+    t_fwd, n_fwd = per_item(lambda d, c: orc.stage_forward(P, L, pw, d, c), polys, 64)
+    t_inv, n_inv = per_item(lambda d, c: orc.stage_inverse(P, L, pwi, d, c), polys, 64)
+    npt = 8  # points per timing chunk; operands hold all 2^14 points, only npt are multiplied
+    ahat = rng.integers(0, P, M * K * npt, dtype=np.uint32)
+    bhat = rng.integers(0, P, K * N * npt, dtype=np.uint32)
+    chat = np.zeros(M * N * npt, dtype=np.uint32)
+    t_pt, n_pt = per_item(lambda d, c: orc.stage_pointwise(P, M, K, N, npt, ahat, bhat, chat, 0, c),
+                          lambda c: None, npt)
+    t_prod = (M * K + K * N) * t_fwd + pts * t_pt + M * N * t_inv
+    sample = ("oracle mul_fft stages on one thread: %d forward and %d inverse 2^14-point NTTs, "
+              "%d pointwise 128x128x128 products mod p; extrapolated to 32768 forward NTTs, "
+              "16384 points and 16384 inverse NTTs (one product)" % (n_fwd, n_inv, n_pt))
+    return 1.0 / t_prod, sample, t_prod
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs as a single process on one GPU", file=sys.stderr)
+        sys.exit(2)
+
+    ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
+    pts = 1 << 14
+    algo = algorithmic_bytes(M, K, N, SA, SB, pts)
+    st = dict(stages)
+    dominant = max((l for l in st if l in algo), key=lambda l: st[l])
+    achieved = algo[dominant] / (st[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "stage": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": algo[dominant]}
+    if args.skip_cpu:
+        cpu = None
+    else:
+        v, sample, t_prod = cpu_baseline(args.cpu_budget)
+        cpu = {"value": v, "unit": "products/s", "cores": 1, "kind": "port", "sample": sample,
+               "s_per_product": round(t_prod, 2)}
+    value = 1000.0 / ms_int8
+    out = {
+        "metric": "bit-exact polymatrix mul/s, 128^2 deg 8191: INT8-MFMA vs uint64-ALU form",
+        "value": round(value, 3),
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_int8, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "polymat_mul 128x128.128x128 deg 8191 p=2013265921 (length-16384 NTT)",
+                   "m": M, "k": K, "n": N, "degree": SA - 1, "p": P, "ntt_points": pts,
+                   "pointwise": "int8_mfma", "parallelism": "single_gpu"},
+        "plain_u64": {"value": round(1000.0 / ms_plain, 3), "ms_per_step": round(ms_plain, 4),
+                      "stages_ms": {k: round(v, 4) for k, v in stages_plain}},
+        "int8_vs_plain": round(ms_plain / ms_int8, 3),
+        "stages_ms": {k: round(v, 4) for k, v in stages},
+        "gmulmod_per_s": round(40265318400 / (ms_int8 * 1e-3) / 1e9, 2),
+        "wall_s_timed": round(wall, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * lbx.h -- C ABI of the MI355X (gfx950) FFT polynomial-matrix multiplication.
+ *
+ * The reference (LinBox 1.7.1) is header-only C++ and has no C ABI; the
+ * seam this library replaces is the class template
+ *   PolynomialMatrixFFTMulDomain<Givaro::Modular<uint32_t,uint64_t>>::mul(c,a,b,max_rowdeg)
+ *   (linbox/algorithms/polynomial-matrix/matpoly-mult-fft-wordsize.inl:57-88)
+ * reached from PolynomialMatrixMulDomain::mul (polynomial-matrix-domain.h:56-75).
+ * Each entry point below names the reference interface it stands for.
+ *
+ * Conventions
+ *  - Every polynomial matrix is described by an lbx_polmat: a base pointer,
+ *    one of the three PolynomialMatrix storage layouts (polynomial-matrix.h:54-77)
+ *    and the view window of a SubPolynomialMatrix (polynomial-matrix.h:940-1036).
+ *  - Pointers may be host or device memory (detected per call).  With device
+ *    pointers the call is stream-ordered on `stream` (a hipStream_t, NULL for
+ *    the default stream) and returns once the work is enqueued; with any host
+ *    pointer the call copies through device buffers and returns when done.
+ *  - Coefficients are uint32_t in [0,p) (Modular<uint32_t,uint64_t>::Element).
+ *    The multiprecision entry point uses nlimbs little-endian uint32_t words
+ *    per coefficient.
+ *  - Errors are returned as lbx_status; the codes map one-to-one onto the
+ *    reference's throw sites (see lbx_status).  lbx_strerror gives the text.
+ *  - A context serialises the calls made on it; use one context per host
+ *    thread for concurrency.
+ */
+#ifndef LBX_LBX_H
+#define LBX_LBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lbx_status {
+    LBX_OK = 0,
+    LBX_ERR_BAD_FFT_PRIME = 1,      /* "LinBox ERROR: bad FFT Prime" (matpoly-mult-fft-wordsize-fast.inl:107-112) */
+    LBX_ERR_K_TOO_LARGE = 2,        /* "FFT: k is larger that log2(p-1)" (fft-utils.h:92-93) */
+    LBX_ERR_K_NOT_POSITIVE = 3,     /* "FFT: k must be positive" (fft.h:135-136) */
+    LBX_ERR_NOT_PRIMITIVE_ROOT = 4, /* "w is not a primitive k-root of unity" (fft.h:140-143) */
+    LBX_ERR_SIZE_MISMATCH = 5,      /* a.coldim()!=b.rowdim() (wordsize-fast.inl:65), or c too small: a view
+                                       cannot be resized larger (polynomial-matrix.h:983-989) */
+    LBX_ERR_NOT_ENOUGH_PRIMES = 6,  /* "not enough FFT primes" (three-primes.inl:122-126, matpoly-mult-fft.h:321-324) */
+    LBX_ERR_INVALID_ARG = 7,        /* null pointer, zero dimension, unknown layout or method */
+    LBX_ERR_PRIME_TOO_LARGE = 8,    /* modulus outside the supported range (three-primes.inl:57-64 analogue) */
+    LBX_ERR_HIP = 9,                /* a HIP runtime call failed */
+    LBX_ERR_OUT_OF_MEMORY = 10,     /* device allocation failed */
+    LBX_ERR_UNSUPPORTED = 11,       /* shape beyond what this build implements */
+    LBX_ERR_NO_DEVICE = 12          /* no gfx950 device at the requested index */
+} lbx_status;
+
+typedef enum lbx_layout {
+    LBX_POLFIRST = 0,   /* element (e,t) at data[e*storage_size + t], e = i*cols+j (polynomial-matrix.h:131-137) */
+    LBX_MATFIRST = 1,   /* element (e,t) at data[e + t*rows*cols] (polynomial-matrix.h:422-428) */
+    LBX_MATROWFIRST = 2 /* element (i,j,t) at data[i*cols*storage_size + j + t*cols] (polynomial-matrix.h:660-683) */
+} lbx_layout;
+
+typedef enum lbx_method {
+    LBX_METHOD_AUTO = 0,      /* INT8 MFMA pointwise when it applies, else PLAIN_U64 */
+    LBX_METHOD_PLAIN_U64 = 1, /* pointwise products on vector ALUs, 64-bit accumulation, delayed reduction */
+    LBX_METHOD_INT8_MFMA = 2  /* pointwise products on INT8 MFMA over balanced signed 8-bit limbs */
+} lbx_method;
+
+/* A polynomial matrix or a SubPolynomialMatrix view of one.  Coefficient t
+ * (0 <= t < size) of entry e lives at storage coefficient shift+t. */
+typedef struct lbx_polmat {
+    void *data;          /* storage base, host or device */
+    int layout;          /* lbx_layout */
+    size_t rows, cols;   /* rowdim(), coldim() */
+    size_t size;         /* size() of the view: degree+1 */
+    size_t storage_size; /* size() of the underlying PolynomialMatrix (>= shift+size) */
+    size_t shift;        /* SubPolynomialMatrix::_shift (0 for a whole matrix) */
+} lbx_polmat;
+
+typedef struct lbx_ctx lbx_ctx;
+
+/* Context on one device: stream-ordered workspace and cached twiddle tables. */
+lbx_status lbx_init(lbx_ctx **ctx, int device);
+lbx_status lbx_destroy(lbx_ctx *ctx);
+const char *lbx_strerror(lbx_status status);
+/* Release cached device workspace (kept between calls otherwise). */
+lbx_status lbx_trim(lbx_ctx *ctx);
+
+/* PolynomialMatrixFFTMulDomain<Modular<uint32_t,uint64_t>>::mul(c,a,b,max_rowdeg)
+ * (matpoly-mult-fft-wordsize.inl:57-88), together with the d<=2 naive route of
+ * PolynomialMatrixMulDomain::mul (polynomial-matrix-domain.h:56-75).
+ * deg = max_rowdeg ? max_rowdeg : a.size+b.size-2; coefficients 0..deg of c are
+ * written and c->size is set to deg+1 (the reference's c.resize(deg+1));
+ * c->size must be >= deg+1 on entry.  Any prime p < 2^32 is accepted: FFT
+ * primes with 2^ceil(log2(deg+1)) | p-1 and p < 2^31 take the single-prime
+ * NTT product; every other p goes through lbx_polymat_mul_mp with one limb. */
+lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
+                               const lbx_polmat *b, size_t max_rowdeg, int method, void *stream);
+
+/* PolynomialMatrixFFTPrimeMulDomain<F>::mul_fft(lpts, c, a, b)
+ * (matpoly-mult-fft-wordsize-fast.inl:86-190): the narrower seam on padded
+ * polfirst buffers of 2^lpts coefficients per entry; a and b are left intact
+ * (the reference transforms them in place).  Returns LBX_ERR_BAD_FFT_PRIME
+ * when 2^lpts does not divide p-1. */
+lbx_status lbx_polymat_mul_fft_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, size_t m, size_t k, size_t n,
+                                   const uint32_t *a, const uint32_t *b, uint32_t *c, int method,
+                                   void *stream);
+
+/* FFT<Modular<uint32_t,uint64_t>>(F, lpts, root) then FFT_direct (DIF: natural
+ * order in, bit-reversed out) / FFT_inverse (DIT: bit-reversed in, natural out,
+ * no scaling) on `count` contiguous polynomials of 2^lpts words, in place
+ * (fft.h:57-89,133-144).  root==0 selects compute_primitive_root
+ * (fft-utils.h:83-104); *root_used (may be NULL) receives the root. */
+lbx_status lbx_ntt_forward_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data,
+                               size_t count, uint32_t *root_used, void *stream);
+lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data,
+                               size_t count, uint32_t *root_used, void *stream);
+
+/* PolynomialMatrixFFTMulDomain<Modular<integer>>::mul (matpoly-mult-fft-multiprecision.inl:611-630,
+ * CRT in ZRing<integer>::mul_crtla :141-277) for a multiprecision modulus P
+ * given as nlimbs (1..16) little-endian uint32 words; coefficients hold
+ * nlimbs words each, in [0,P).  The integer product is computed modulo FFT
+ * primes below 2^31 (RandomFFTPrime::generatePrimes order, random-fftprime.h:107-133)
+ * covering 2 P^2 k min(sa,sb), recombined by CRT on the device, and reduced mod P.
+ * c->size must be >= a.size+b.size-1 and is set to that value. */
+lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nlimbs, lbx_polmat *c,
+                              const lbx_polmat *a, const lbx_polmat *b, int method, void *stream);
+
+/* One product of a batch (device pointers, polfirst storage, whole matrices). */
+typedef struct lbx_job {
+    uint32_t p;          /* prime of this product */
+    size_t m, k, n;      /* a is m x k, b is k x n */
+    size_t sa, sb;       /* a.size(), b.size(); c receives sa+sb-1 coefficients */
+    const uint32_t *a;   /* polfirst, stride sa */
+    const uint32_t *b;   /* polfirst, stride sb */
+    uint32_t *c;         /* polfirst, stride sa+sb-1 */
+} lbx_job;
+
+/* Independent products of uneven shapes, degrees and primes, the mix that
+ * OrderBasis::PM_Basis issues through PolynomialMatrixMulDomain::mul
+ * (order-basis.h:219-326 -> :297).  Every job must use an FFT prime below 2^31
+ * with 2^ceil(log2(sa+sb-1)) | p-1.  All forward transforms, all pointwise
+ * products and all inverse transforms run as three persistent work-queue
+ * launches replayed from a cached hipGraph; results equal one
+ * lbx_polymat_mul_u32 per job. */
+lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream);
+
+/* FFT_PROFILER analogue (matpoly-mult-fft.h:43-77): while enabled, calls
+ * record device events between their stages, labelled as in mul_fft
+ * ("direct FFT_DIF", "Polfirst to Matfirst", "Pointwise mult",
+ * "Matfirst to Polfirst", "inverse FFT_DIT"; wordsize-fast.inl:116-189),
+ * without any host synchronisation.  lbx_profile_get waits for the recorded
+ * events, returns the per-stage average time (ms) over the calls made since
+ * the previous get, copies up to `max` (label, ms) pairs and returns their
+ * count. */
+lbx_status lbx_profile_enable(lbx_ctx *ctx, int enable);
+int lbx_profile_get(lbx_ctx *ctx, const char **labels, float *ms, int max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBX_LBX_H */

@@ -1,0 +1,593 @@
+// C ABI (include/lbx/lbx.h): argument checking, dispatch between the
+// single-prime, constant-matrix and multiprecision routes, device workspace,
+// twiddle-table cache, host/device pointer handling and stage timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "lbx/lbx.h"
+#include "lbx_internal.h"
+#include "lbx_mp.h"
+
+namespace lbx {
+
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace lbx
+
+struct lbx_ctx {
+    int device = 0;
+    std::mutex mu;
+    std::map<std::string, lbx::Buffer> bufs;
+    std::map<std::tuple<uint32_t, unsigned, uint32_t>, uint2 *> twiddles;
+    bool profiling = false;
+    // recorded (label, start, end) event triples, resolved by lbx_profile_get
+    std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;
+    std::vector<std::string> prof_labels;
+    std::vector<float> prof_ms;
+    std::vector<const char *> prof_label_ptrs;
+};
+
+namespace lbx {
+namespace {
+
+lbx_status from_hip(hipError_t e)
+{
+    if (e == hipSuccess) return LBX_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return LBX_ERR_OUT_OF_MEMORY;
+    return LBX_ERR_HIP;
+}
+
+#define LBX_TRY_HIP(expr)                           \
+    do {                                            \
+        hipError_t e_ = (expr);                     \
+        if (e_ != hipSuccess) return from_hip(e_);  \
+    } while (0)
+#define LBX_TRY(expr)                   \
+    do {                                \
+        lbx_status s_ = (expr);         \
+        if (s_ != LBX_OK) return s_;    \
+    } while (0)
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int dev)
+    {
+        (void)hipGetDevice(&prev_);
+        if (prev_ != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev_); }
+
+private:
+    int prev_ = 0;
+};
+
+lbx_status ensure(lbx_ctx *ctx, const char *name, size_t bytes, void **out)
+{
+    Buffer &b = ctx->bufs[name];
+    if (b.bytes < bytes) {
+        if (b.ptr) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(b.ptr);
+            b.ptr = nullptr;
+            b.bytes = 0;
+        }
+        LBX_TRY_HIP(hipMalloc(&b.ptr, bytes ? bytes : 16));
+        b.bytes = bytes;
+    }
+    *out = b.ptr;
+    return LBX_OK;
+}
+
+bool is_device_ptr(const void *p)
+{
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+unsigned v2(uint32_t x)
+{
+    unsigned v = 0;
+    while (x && !(x & 1)) {
+        x >>= 1;
+        ++v;
+    }
+    return v;
+}
+
+// fft-utils.h:65-78
+bool is_primitive_root(uint32_t p, uint32_t w, unsigned k)
+{
+    uint64_t x = w;
+    for (unsigned i = 0; i < k; ++i) {
+        if (x == 1) return false;
+        x = x * x % p;
+    }
+    return true;
+}
+
+// fft-utils.h:83-104
+lbx_status primitive_root(uint32_t p, unsigned k, uint32_t *w_out)
+{
+    const unsigned val2p = v2(p - 1);
+    const uint32_t m = (p - 1) >> val2p;
+    if (k > val2p) return LBX_ERR_K_TOO_LARGE;
+    for (uint32_t t = 2;; ++t) {
+        uint64_t w = powmod_h(t, m, p);
+        for (unsigned i = 0; i < val2p - k; ++i) w = w * w % p;
+        if (is_primitive_root(p, (uint32_t)w, k)) {
+            *w_out = (uint32_t)w;
+            return LBX_OK;
+        }
+    }
+}
+
+// fft.h:119-130
+uint32_t invroot(uint32_t p, unsigned L, uint32_t w)
+{
+    if (L == 1) return p - 1;
+    const uint32_t x = powmod_h(w, ((uint64_t)1 << (L - 1)) - 1, p);
+    return x ? p - x : 0;
+}
+
+// Twiddle table in the pow_w layout of init_powers (fft-integral.inl:1614-1650),
+// cached per (p, L, root).
+lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint2 **out)
+{
+    auto key = std::make_tuple(p, L, w);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end()) {
+        *out = it->second;
+        return LBX_OK;
+    }
+    const size_t n = (size_t)1 << L;
+    std::vector<uint32_t> pw(n > 1 ? n - 1 : 1);
+    pw[0] = 1 % p;
+    for (size_t i = 1; i < n / 2; ++i) pw[i] = (uint32_t)((uint64_t)pw[i - 1] * w % p);
+    size_t idx = n / 2;
+    for (size_t kk = 2; kk <= n / 2; kk <<= 1)
+        for (size_t i = 0; i < n / 2; i += kk, idx++) pw[idx] = pw[i];
+    std::vector<uint2> tw(pw.size());
+    for (size_t i = 0; i < pw.size(); ++i) tw[i] = make_uint2(pw[i], shoup_pre(pw[i], p));
+    uint2 *d = nullptr;
+    LBX_TRY_HIP(hipMalloc(&d, sizeof(uint2) * tw.size()));
+    LBX_TRY_HIP(hipMemcpy(d, tw.data(), sizeof(uint2) * tw.size(), hipMemcpyHostToDevice));
+    ctx->twiddles[key] = d;
+    *out = d;
+    return LBX_OK;
+}
+
+size_t storage_words(const lbx_polmat *v) { return v->rows * v->cols * v->storage_size; }
+
+lbx_status check_view(const lbx_polmat *v)
+{
+    if (!v || !v->data) return LBX_ERR_INVALID_ARG;
+    if (v->layout < 0 || v->layout > 2) return LBX_ERR_INVALID_ARG;
+    if (!v->rows || !v->cols) return LBX_ERR_INVALID_ARG;
+    if (v->shift + v->size > v->storage_size) return LBX_ERR_SIZE_MISMATCH;
+    return LBX_OK;
+}
+
+DevView devview(const lbx_polmat *v, void *dev_base)
+{
+    DevView d;
+    d.base = (uint32_t *)dev_base;
+    d.layout = v->layout;
+    d.rows = v->rows;
+    d.cols = v->cols;
+    d.stor = v->storage_size;
+    d.shift = v->shift;
+    d.size = v->size;
+    return d;
+}
+
+// Stage timer for the FFT_PROFILER analogue: events are recorded on the
+// stream between stages and resolved later, so timing adds no host sync.
+class Stages {
+public:
+    Stages(lbx_ctx *ctx, hipStream_t s) : ctx_(ctx), s_(s)
+    {
+        if (ctx_->profiling) last_ = record();
+    }
+    void mark(const char *label_of_finished_stage)
+    {
+        if (!ctx_->profiling) return;
+        hipEvent_t ev = record();
+        ctx_->prof_pending.emplace_back(label_of_finished_stage, last_, ev);
+        last_ = ev;
+    }
+
+private:
+    hipEvent_t record()
+    {
+        hipEvent_t ev = nullptr;
+        (void)hipEventCreate(&ev);
+        (void)hipEventRecord(ev, s_);
+        return ev;
+    }
+    lbx_ctx *ctx_;
+    hipStream_t s_;
+    hipEvent_t last_ = nullptr;
+};
+
+int choose_method(int method, size_t m, size_t k, size_t n)
+{
+    if (method == LBX_METHOD_PLAIN_U64 || method == LBX_METHOD_INT8_MFMA) return method;
+    // MFMA tiles are 32x32 with a 64-deep inner step; small products waste them.
+    return (m >= 16 && n >= 16 && k >= 16) ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
+}
+
+}  // namespace
+
+// Forward transforms, pointwise products and inverse transform of one
+// single-prime product on device views.  deg+1 coefficients of c are written.
+lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
+                             const DevView &c, size_t m, size_t k, size_t n, size_t count, int method,
+                             hipStream_t s)
+{
+    const size_t N = (size_t)1 << L;
+    if (L > (unsigned)kMaxLdsLog) return LBX_ERR_UNSUPPORTED;
+    uint32_t w;
+    LBX_TRY(primitive_root(p, L, &w));
+    const uint32_t wi = invroot(p, L, w);
+    const uint2 *twf, *twi;
+    LBX_TRY(twiddles(ctx, p, L, w, &twf));
+    LBX_TRY(twiddles(ctx, p, L, wi, &twi));
+    void *ahat, *bhat, *chat;
+    LBX_TRY(ensure(ctx, "ahat", m * k * N * 4, &ahat));
+    LBX_TRY(ensure(ctx, "bhat", k * n * N * 4, &bhat));
+    LBX_TRY(ensure(ctx, "chat", m * n * N * 4, &chat));
+    const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
+    method = choose_method(method, m, k, n);
+
+    Stages st(ctx, s);
+    LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twf, p, (uint32_t *)ahat, s));
+    LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twf, p, (uint32_t *)bhat, s));
+    st.mark("direct FFT_DIF");
+    if (method == LBX_METHOD_INT8_MFMA) {
+        void *al, *bl;
+        LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
+        LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
+        LBX_TRY_HIP(launch_to_limbs((const uint32_t *)ahat, (uint8_t *)al, m, k, true, N, p, s));
+        LBX_TRY_HIP(launch_to_limbs((const uint32_t *)bhat, (uint8_t *)bl, n, k, false, N, p, s));
+        st.mark("Polfirst to Matfirst");
+        // the point-major result reuses the forward buffer of a
+        void *cmat;
+        LBX_TRY(ensure(ctx, "cmat", m * n * N * 4, &cmat));
+        LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N, p,
+                                          inv_n, s));
+        st.mark("Pointwise mult");
+        LBX_TRY_HIP(launch_point_to_entry((const uint32_t *)cmat, (uint32_t *)chat, m * n, N, s));
+        st.mark("Matfirst to Polfirst");
+    } else {
+        LBX_TRY_HIP(launch_pointwise_plain((const uint32_t *)ahat, (const uint32_t *)bhat, (uint32_t *)chat, m, k, n,
+                                           N, p, inv_n, s));
+        st.mark("Pointwise mult");
+    }
+    LBX_TRY_HIP(launch_ntt_inverse((const uint32_t *)chat, m * n, L, twi, p, c, count, 1, s));
+    st.mark("inverse FFT_DIT");
+    return LBX_OK;
+}
+
+// Constant matrices (a.size()+b.size() <= 2): one pointwise product.
+static lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c,
+                                   size_t m, size_t k, size_t n, hipStream_t s)
+{
+    void *ahat, *bhat, *chat;
+    LBX_TRY(ensure(ctx, "ahat", m * k * 4, &ahat));
+    LBX_TRY(ensure(ctx, "bhat", k * n * 4, &bhat));
+    LBX_TRY(ensure(ctx, "chat", m * n * 4, &chat));
+    LBX_TRY_HIP(launch_gather(a, m * k, 1, (uint32_t *)ahat, 1, s));
+    LBX_TRY_HIP(launch_gather(b, k * n, 1, (uint32_t *)bhat, 1, s));
+    LBX_TRY_HIP(launch_pointwise_plain((const uint32_t *)ahat, (const uint32_t *)bhat, (uint32_t *)chat, m, k, n, 1,
+                                       p, 1, s));
+    LBX_TRY_HIP(launch_scatter((const uint32_t *)chat, 1, m * n, 1, c, s));
+    return LBX_OK;
+}
+
+// Host operands are staged through device copies of their whole storage.
+struct Staged {
+    lbx_polmat v;
+    void *dev = nullptr;
+    bool owned = false;
+};
+
+static lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out)
+{
+    out->v = *src;
+    if (is_device_ptr(src->data)) {
+        out->dev = src->data;
+        return LBX_OK;
+    }
+    const size_t bytes = storage_words(src) * word_bytes;
+    LBX_TRY_HIP(hipMalloc(&out->dev, bytes ? bytes : 16));
+    out->owned = true;
+    if (copy_in) LBX_TRY_HIP(hipMemcpy(out->dev, src->data, bytes, hipMemcpyHostToDevice));
+    return LBX_OK;
+}
+
+static void release(Staged &s)
+{
+    if (s.owned && s.dev) (void)hipFree(s.dev);
+    s.dev = nullptr;
+}
+
+}  // namespace lbx
+
+using namespace lbx;
+
+extern "C" {
+
+lbx_status lbx_init(lbx_ctx **out, int device)
+{
+    if (!out) return LBX_ERR_INVALID_ARG;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return LBX_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return LBX_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return LBX_ERR_NO_DEVICE;
+    lbx_ctx *ctx = new lbx_ctx();
+    ctx->device = device;
+    *out = ctx;
+    return LBX_OK;
+}
+
+lbx_status lbx_trim(lbx_ctx *ctx)
+{
+    if (!ctx) return LBX_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard g(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : ctx->bufs)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    ctx->bufs.clear();
+    return LBX_OK;
+}
+
+lbx_status lbx_destroy(lbx_ctx *ctx)
+{
+    if (!ctx) return LBX_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        DeviceGuard g(ctx->device);
+        (void)hipDeviceSynchronize();
+        for (auto &kv : ctx->bufs)
+            if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+        for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
+        lbx_mp_release(ctx);
+    }
+    delete ctx;
+    return LBX_OK;
+}
+
+const char *lbx_strerror(lbx_status st)
+{
+    switch (st) {
+    case LBX_OK: return "success";
+    case LBX_ERR_BAD_FFT_PRIME: return "LinBox ERROR: bad FFT Prime";
+    case LBX_ERR_K_TOO_LARGE: return "FFT: k is larger that log2(p-1)";
+    case LBX_ERR_K_NOT_POSITIVE: return "FFT: k must be positive";
+    case LBX_ERR_NOT_PRIMITIVE_ROOT: return "w is not a primitive k-root of unity";
+    case LBX_ERR_SIZE_MISMATCH: return "size mismatch: incompatible dimensions or view resized larger than available";
+    case LBX_ERR_NOT_ENOUGH_PRIMES: return "not enough FFT primes";
+    case LBX_ERR_INVALID_ARG: return "invalid argument";
+    case LBX_ERR_PRIME_TOO_LARGE: return "modulus out of the supported range";
+    case LBX_ERR_HIP: return "HIP runtime error";
+    case LBX_ERR_OUT_OF_MEMORY: return "device out of memory";
+    case LBX_ERR_UNSUPPORTED: return "unsupported size";
+    case LBX_ERR_NO_DEVICE: return "no gfx950 device";
+    }
+    return "unknown status";
+}
+
+lbx_status lbx_profile_enable(lbx_ctx *ctx, int enable)
+{
+    if (!ctx) return LBX_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->profiling = enable != 0;
+    return LBX_OK;
+}
+
+int lbx_profile_get(lbx_ctx *ctx, const char **labels, float *ms, int max)
+{
+    if (!ctx) return 0;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    // average per label over every call recorded since the previous get
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, int>> acc;
+    std::vector<hipEvent_t> used;
+    for (auto &rec : ctx->prof_pending) {
+        const std::string &label = std::get<0>(rec);
+        hipEvent_t a = std::get<1>(rec), b = std::get<2>(rec);
+        (void)hipEventSynchronize(b);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, a, b);
+        if (!acc.count(label)) order.push_back(label);
+        acc[label].first += t;
+        acc[label].second += 1;
+        used.push_back(a);
+        used.push_back(b);
+    }
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    for (auto ev : used) (void)hipEventDestroy(ev);
+    ctx->prof_pending.clear();
+    if (!order.empty()) {
+        ctx->prof_labels.clear();
+        ctx->prof_ms.clear();
+        for (auto &l : order) {
+            ctx->prof_labels.push_back(l);
+            ctx->prof_ms.push_back((float)(acc[l].first / acc[l].second));
+        }
+    }
+    int n = (int)ctx->prof_ms.size();
+    if (n > max) n = max;
+    ctx->prof_label_ptrs.clear();
+    for (auto &l : ctx->prof_labels) ctx->prof_label_ptrs.push_back(l.c_str());
+    for (int i = 0; i < n; ++i) {
+        if (labels) labels[i] = ctx->prof_label_ptrs[i];
+        if (ms) ms[i] = ctx->prof_ms[i];
+    }
+    return n;
+}
+
+lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
+                               size_t max_rowdeg, int method, void *stream)
+{
+    if (!ctx || !a || !b || !c) return LBX_ERR_INVALID_ARG;
+    LBX_TRY(check_view(a));
+    LBX_TRY(check_view(b));
+    LBX_TRY(check_view(c));
+    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+    if (p < 2) return LBX_ERR_INVALID_ARG;
+    if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
+    if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
+    const size_t m = a->rows, k = a->cols, n = b->cols, sa = a->size, sb = b->size;
+    const size_t deg = max_rowdeg ? max_rowdeg : sa + sb - 2;
+    if (c->size < deg + 1) return LBX_ERR_SIZE_MISMATCH;
+    unsigned L = 0;
+    size_t pts = 1;
+    while (pts <= deg) {
+        pts <<= 1;
+        ++L;
+    }
+    if (sa > pts || sb > pts) return LBX_ERR_SIZE_MISMATCH;
+    const bool constant = sa + sb <= 2;
+    const bool single_prime = !constant && p < (1u << 31) && (p - 1) % pts == 0;
+    if (!constant && !single_prime) {
+        if (max_rowdeg && max_rowdeg != sa + sb - 2) return LBX_ERR_UNSUPPORTED;
+        lbx_status st = lbx_polymat_mul_mp(ctx, &p, 1, c, a, b, method, stream);
+        return st;
+    }
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    Staged sa_, sb_, sc_;
+    lbx_status st = stage_in(a, 4, true, &sa_);
+    if (st == LBX_OK) st = stage_in(b, 4, true, &sb_);
+    if (st == LBX_OK) st = stage_in(c, 4, true, &sc_);
+    const bool host_io = sa_.owned || sb_.owned || sc_.owned;
+    if (st == LBX_OK) {
+        const DevView da = devview(a, sa_.dev), db = devview(b, sb_.dev), dc = devview(c, sc_.dev);
+        if (constant)
+            st = constant_product(ctx, p, da, db, dc, m, k, n, s);
+        else
+            st = fft_prime_product(ctx, p, L, da, db, dc, m, k, n, deg + 1, method, s);
+    }
+    if (st == LBX_OK && sc_.owned) {
+        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        st = from_hip(e);
+    } else if (st == LBX_OK && host_io) {
+        st = from_hip(hipStreamSynchronize(s));
+    }
+    if (host_io) (void)hipStreamSynchronize(s);
+    release(sa_);
+    release(sb_);
+    release(sc_);
+    if (st == LBX_OK) c->size = deg + 1;
+    return st;
+}
+
+lbx_status lbx_polymat_mul_fft_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, size_t m, size_t k, size_t n,
+                                   const uint32_t *a, const uint32_t *b, uint32_t *c, int method, void *stream)
+{
+    if (!ctx || !a || !b || !c || !m || !k || !n) return LBX_ERR_INVALID_ARG;
+    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+    const size_t pts = (size_t)1 << lpts;
+    if (p < 2 || p >= (1u << 31)) return LBX_ERR_PRIME_TOO_LARGE;
+    if ((p - 1) % pts != 0) return LBX_ERR_BAD_FFT_PRIME;
+    if (lpts == 0) return LBX_ERR_K_NOT_POSITIVE;
+    lbx_polmat va{(void *)a, LBX_POLFIRST, m, k, pts, pts, 0};
+    lbx_polmat vb{(void *)b, LBX_POLFIRST, k, n, pts, pts, 0};
+    lbx_polmat vc{(void *)c, LBX_POLFIRST, m, n, pts, pts, 0};
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    Staged sa_, sb_, sc_;
+    lbx_status st = stage_in(&va, 4, true, &sa_);
+    if (st == LBX_OK) st = stage_in(&vb, 4, true, &sb_);
+    if (st == LBX_OK) st = stage_in(&vc, 4, false, &sc_);
+    if (st == LBX_OK)
+        st = fft_prime_product(ctx, p, lpts, devview(&va, sa_.dev), devview(&vb, sb_.dev), devview(&vc, sc_.dev), m,
+                               k, n, pts, method, s);
+    if (st == LBX_OK && sc_.owned) {
+        hipError_t e = hipMemcpyAsync(c, sc_.dev, m * n * pts * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        st = from_hip(e);
+    }
+    if (sa_.owned || sb_.owned || sc_.owned) (void)hipStreamSynchronize(s);
+    release(sa_);
+    release(sb_);
+    release(sc_);
+    return st;
+}
+
+static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data, size_t count,
+                            uint32_t *root_used, void *stream, bool inverse)
+{
+    if (!ctx || !data) return LBX_ERR_INVALID_ARG;
+    if (p < 3 || p >= (1u << 31)) return LBX_ERR_PRIME_TOO_LARGE;
+    if (!lpts) return LBX_ERR_K_NOT_POSITIVE;  // fft.h:135-136
+    uint32_t w = root;
+    if (w == 0) {
+        LBX_TRY(primitive_root(p, lpts, &w));
+    } else if (!is_primitive_root(p, w, lpts)) {
+        return LBX_ERR_NOT_PRIMITIVE_ROOT;
+    }
+    if (lpts > (unsigned)kMaxLdsLog) return LBX_ERR_UNSUPPORTED;
+    if (root_used) *root_used = w;
+    if (!count) return LBX_OK;
+    const size_t N = (size_t)1 << lpts;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    const uint2 *tw;
+    LBX_TRY(twiddles(ctx, p, lpts, w, &tw));
+    lbx_polmat v{(void *)data, LBX_POLFIRST, count, 1, N, N, 0};
+    Staged sd;
+    LBX_TRY(stage_in(&v, 4, true, &sd));
+    const DevView dv = devview(&v, sd.dev);
+    hipError_t e = inverse ? launch_ntt_inverse((const uint32_t *)sd.dev, count, lpts, tw, p, dv, N, 1, s)
+                           : launch_ntt_forward(dv, count, lpts, tw, p, (uint32_t *)sd.dev, s);
+    lbx_status st = from_hip(e);
+    if (st == LBX_OK && sd.owned) {
+        e = hipMemcpyAsync(data, sd.dev, count * N * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        st = from_hip(e);
+    }
+    if (sd.owned) (void)hipStreamSynchronize(s);
+    release(sd);
+    return st;
+}
+
+lbx_status lbx_ntt_forward_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data, size_t count,
+                               uint32_t *root_used, void *stream)
+{
+    return ntt_entry(ctx, p, lpts, root, data, count, root_used, stream, false);
+}
+
+lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data, size_t count,
+                               uint32_t *root_used, void *stream)
+{
+    return ntt_entry(ctx, p, lpts, root, data, count, root_used, stream, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+// Pointwise stage of the FFT product: at every evaluation point t,
+// C_t = A_t . B_t (m x k . k x n) mod p, then the 1/2^L scaling of
+// mul_fft (matpoly-mult-fft-wordsize-fast.inl:160-164,183-189) folded in.
+//
+// The reference runs one fflas fgemm per point.  Here all points are one
+// launch, in one of two forms:
+//
+//  * plain: vector ALUs, one point per lane, a TIxTJ register tile of 64-bit
+//    accumulators per lane.  Each product a*b < p^2 < 2^62 is added with
+//    carry into (acc:u64, hi:u32), i.e. reduction is delayed to the end of the
+//    k loop, where (hi,acc) is reduced with three Shoup products.  Operands
+//    are read straight from the polfirst NTT output: consecutive lanes read
+//    consecutive points, so every load is coalesced.
+//
+//  * INT8 MFMA: every residue is centred (|c| < p/2 < 2^30) and split into
+//    four balanced signed 8-bit digits, c = sum_s d_s 2^(8s).  Then
+//    C = sum_{s,t} 2^(8(s+t)) (A_s . B_t), and the 16 digit products are
+//    accumulated on v_mfma_i32_16x16x64_i8 directly into the seven
+//    anti-diagonals D_d = sum_{s+t=d} A_s.B_t (int32, exact: |D_d| <= 4 k 2^14,
+//    k is processed in chunks of 4096 so |D_d| < 2^28).  The epilogue forms
+//    sum_d D_d (2^(8d) / 2^L mod p) in int64 (|.| < 2^62) and reduces it.
+//    The digit split happens once per element in launch_to_limbs, which also
+//    transposes the NTT output into per-point blocks laid out for direct
+//    16-byte operand loads.
+#include "lbx_internal.h"
+
+namespace lbx {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Constants of the plain epilogue: V = hi 2^64 + ah 2^32 + al,
+// V*scale mod p = hi*c2 + ah*c1 + al*c0 mod p.
+struct PlainConst {
+    uint32_t c0, c0p, c1, c1p, c2, c2p;
+};
+
+template <int TI, int TJ>
+__global__ void __launch_bounds__(256) pw_plain_kernel(const uint32_t *__restrict__ A, const uint32_t *__restrict__ B,
+                                                       uint32_t *__restrict__ C, int m, int k, int n, size_t npts,
+                                                       uint32_t p, PlainConst pc)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= npts) return;
+    const int tiles_j = (n + TJ - 1) / TJ;
+    const int i0 = (blockIdx.y / tiles_j) * TI, j0 = (blockIdx.y % tiles_j) * TJ;
+
+    uint64_t acc[TI][TJ];
+    uint32_t hi[TI][TJ];
+#pragma unroll
+    for (int ii = 0; ii < TI; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < TJ; ++jj) {
+            acc[ii][jj] = 0;
+            hi[ii][jj] = 0;
+        }
+    for (int q = 0; q < k; ++q) {
+        uint32_t a[TI], b[TJ];
+#pragma unroll
+        for (int ii = 0; ii < TI; ++ii)
+            a[ii] = (i0 + ii < m) ? A[((size_t)(i0 + ii) * k + q) * npts + t] : 0u;
+#pragma unroll
+        for (int jj = 0; jj < TJ; ++jj)
+            b[jj] = (j0 + jj < n) ? B[((size_t)q * n + j0 + jj) * npts + t] : 0u;
+#pragma unroll
+        for (int ii = 0; ii < TI; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < TJ; ++jj) {
+                const uint64_t prod = (uint64_t)a[ii] * b[jj];
+                const uint64_t s = acc[ii][jj] + prod;
+                hi[ii][jj] += (s < prod) ? 1u : 0u;
+                acc[ii][jj] = s;
+            }
+    }
+#pragma unroll
+    for (int ii = 0; ii < TI; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < TJ; ++jj) {
+            if (i0 + ii >= m || j0 + jj >= n) continue;
+            const uint32_t al = (uint32_t)acc[ii][jj], ah = (uint32_t)(acc[ii][jj] >> 32);
+            uint32_t r = shoup_mul(hi[ii][jj], pc.c2, pc.c2p, p);
+            r = add_mod(r, shoup_mul(ah, pc.c1, pc.c1p, p), p);
+            r = add_mod(r, shoup_mul(al, pc.c0, pc.c0p, p), p);
+            C[((size_t)(i0 + ii) * n + j0 + jj) * npts + t] = r;
+        }
+}
+
+// ---- limb blocks -------------------------------------------------------
+// Per point, an operand with `rows` rows and inner dimension kdim is stored as
+// [rpad][kpad/64][4 planes][64 bytes]: the 64 inner-dimension digits of plane s
+// of one row for one 64-wide k step are contiguous, so a lane's MFMA operand
+// (16 consecutive digits) is one 16-byte load.
+__host__ __device__ inline size_t rpad_of(size_t rows) { return (rows + 31) & ~(size_t)31; }
+__host__ __device__ inline size_t kpad_of(size_t kdim) { return (kdim + 63) & ~(size_t)63; }
+
+__global__ void __launch_bounds__(256) to_limbs_kernel(const uint32_t *__restrict__ X, uint8_t *__restrict__ Y,
+                                                       int rows, int kdim, int row_major, size_t npts, uint32_t p,
+                                                       size_t block_bytes, int kchunks)
+{
+    __shared__ uint32_t tile[64][65];
+    const int tid = threadIdx.x;
+    const size_t t0 = (size_t)blockIdx.x * 64;
+    const int c = blockIdx.y;   // 64-wide chunk of the inner dimension
+    const int r = blockIdx.z;   // row (padded range)
+    const int q0 = c * 64;
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int ql = it * 4 + tid / 64, tl = tid % 64;
+        const int q = q0 + ql;
+        const size_t t = t0 + tl;
+        uint32_t v = 0;
+        if (r < rows && q < kdim && t < npts) {
+            const size_t e = row_major ? (size_t)r * kdim + q : (size_t)q * rows + r;
+            v = to_limbs(X[e * npts + t], p);
+        }
+        tile[ql][tl] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int tl = it * 4 + tid / 64, w = tid % 64;
+        const size_t t = t0 + tl;
+        if (t >= npts) continue;
+        const int s = w >> 4, qb = (w & 15) * 4;
+        const uint32_t sh = 8 * s;
+        const uint32_t word = ((tile[qb][tl] >> sh) & 0xffu) | (((tile[qb + 1][tl] >> sh) & 0xffu) << 8) |
+                              (((tile[qb + 2][tl] >> sh) & 0xffu) << 16) | (((tile[qb + 3][tl] >> sh) & 0xffu) << 24);
+        uint8_t *dst = Y + t * block_bytes + ((size_t)r * kchunks + c) * 256 + s * 64 + (w & 15) * 4;
+        *reinterpret_cast<uint32_t *>(dst) = word;
+    }
+}
+
+// Constants of the MFMA epilogue.
+struct MfmaConst {
+    int64_t cd[7];    // 2^(8d) * scale mod p
+    uint64_t off;     // multiple of p >= 2^62
+    uint32_t r32, r32p;  // 2^32 mod p and its Shoup constant
+    uint32_t onep;       // floor(2^32/p): Shoup constant of 1
+};
+
+__device__ __forceinline__ uint32_t mfma_combine(const v4i (&acc)[7], int r, uint32_t p, const MfmaConst &mc)
+{
+    int64_t S = 0;
+#pragma unroll
+    for (int d = 0; d < 7; ++d) S += (int64_t)acc[d][r] * mc.cd[d];
+    const uint64_t u = (uint64_t)S + mc.off;
+    const uint32_t hi = (uint32_t)(u >> 32), lo = (uint32_t)u;
+    return add_mod(shoup_mul(hi, mc.r32, mc.r32p, p), shoup_mul(lo, 1u, mc.onep, p), p);
+}
+
+constexpr int kMaxKbPerChunk = 64;  // 64 steps of 64 = 4096 inner products per exact int32 chunk
+
+// One workgroup per point; each wave computes 32x32 output tiles as 2x2
+// 16x16 MFMA tiles, for each of the 16 digit pairs.
+__global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict__ Al, const uint8_t *__restrict__ Bl,
+                                                      uint32_t *__restrict__ Cmat, int m, int n, int kchunks,
+                                                      int mpad, int npad, size_t a_bytes, size_t b_bytes, uint32_t p,
+                                                      MfmaConst mc)
+{
+    const size_t t = blockIdx.x;
+    const uint8_t *A = Al + t * a_bytes;
+    const uint8_t *B = Bl + t * b_bytes;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int tiles_n = npad / 32, ntiles = (mpad / 32) * tiles_n;
+    const size_t row_bytes = (size_t)kchunks * 256;
+    for (int tile = wave; tile < ntiles; tile += 4) {
+        const int i0 = (tile / tiles_n) * 32, j0 = (tile % tiles_n) * 32;
+        uint32_t res[2][2][4];
+#pragma unroll
+        for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) res[ai][bj][r] = 0;
+        for (int kc0 = 0; kc0 < kchunks; kc0 += kMaxKbPerChunk) {
+            const int kc1 = min(kchunks, kc0 + kMaxKbPerChunk);
+            v4i acc[2][2][7];
+#pragma unroll
+            for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int d = 0; d < 7; ++d) acc[ai][bj][d] = v4i{0, 0, 0, 0};
+            for (int kb = kc0; kb < kc1; ++kb) {
+                v4i a[2][4], b[2][4];
+#pragma unroll
+                for (int sub = 0; sub < 2; ++sub)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        a[sub][s] = *reinterpret_cast<const v4i *>(A + (size_t)(i0 + sub * 16 + l16) * row_bytes +
+                                                                   (size_t)kb * 256 + s * 64 + lg * 16);
+                        b[sub][s] = *reinterpret_cast<const v4i *>(B + (size_t)(j0 + sub * 16 + l16) * row_bytes +
+                                                                   (size_t)kb * 256 + s * 64 + lg * 16);
+                    }
+#pragma unroll
+                for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                acc[ai][bj][s + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                                    a[ai][s], b[bj][u], acc[ai][bj][s + u], 0, 0, 0);
+            }
+#pragma unroll
+            for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        res[ai][bj][r] = add_mod(res[ai][bj][r], mfma_combine(acc[ai][bj], r, p, mc), p);
+        }
+        uint32_t *Ct = Cmat + t * (size_t)m * n;
+#pragma unroll
+        for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = i0 + ai * 16 + lg * 4 + r, j = j0 + bj * 16 + l16;
+                    if (i < m && j < n) Ct[(size_t)i * n + j] = res[ai][bj][r];
+                }
+    }
+}
+
+__global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__restrict__ X, uint32_t *__restrict__ Y,
+                                                             size_t entries, size_t npts)
+{
+    __shared__ uint32_t tile[64][65];
+    const int tid = threadIdx.x;
+    const size_t e0 = (size_t)blockIdx.x * 64, t0 = (size_t)blockIdx.y * 64;
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int tl = it * 4 + tid / 64, el = tid % 64;
+        const size_t t = t0 + tl, e = e0 + el;
+        tile[tl][el] = (t < npts && e < entries) ? X[t * entries + e] : 0u;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it) {
+        const int el = it * 4 + tid / 64, tl = tid % 64;
+        const size_t t = t0 + tl, e = e0 + el;
+        if (t < npts && e < entries) Y[e * npts + t] = tile[tl][el];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pointwise_plain(const uint32_t *A, const uint32_t *B, uint32_t *C, size_t m, size_t k, size_t n,
+                                  size_t npts, uint32_t p, uint32_t scale, hipStream_t s)
+{
+    constexpr int TI = 8, TJ = 8;
+    PlainConst pc;
+    const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
+    const uint32_t r64 = (uint32_t)((uint64_t)r32 * r32 % p);
+    pc.c0 = scale % p;
+    pc.c1 = (uint32_t)((uint64_t)r32 * pc.c0 % p);
+    pc.c2 = (uint32_t)((uint64_t)r64 * pc.c0 % p);
+    pc.c0p = shoup_pre(pc.c0, p);
+    pc.c1p = shoup_pre(pc.c1, p);
+    pc.c2p = shoup_pre(pc.c2, p);
+    const size_t tiles = ((m + TI - 1) / TI) * ((n + TJ - 1) / TJ);
+    if (tiles > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((npts + 255) / 256), (unsigned)tiles);
+    hipLaunchKernelGGL((pw_plain_kernel<TI, TJ>), grid, dim3(256), 0, s, A, B, C, (int)m, (int)k, (int)n, npts, p, pc);
+    return hipGetLastError();
+}
+
+size_t limb_block_bytes(size_t rows, size_t kdim) { return rpad_of(rows) * kpad_of(kdim) * 4; }
+
+hipError_t launch_to_limbs(const uint32_t *X, uint8_t *Y, size_t rows, size_t kdim, bool row_major, size_t npts,
+                           uint32_t p, hipStream_t s)
+{
+    const size_t rpad = rpad_of(rows), kchunks = kpad_of(kdim) / 64;
+    if (rpad > 65535 || kchunks > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((npts + 63) / 64), (unsigned)kchunks, (unsigned)rpad);
+    hipLaunchKernelGGL(to_limbs_kernel, grid, dim3(256), 0, s, X, Y, (int)rows, (int)kdim, row_major ? 1 : 0, npts, p,
+                       limb_block_bytes(rows, kdim), (int)kchunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t *Cmat, size_t m, size_t k, size_t n,
+                                 size_t npts, uint32_t p, uint32_t scale, hipStream_t s)
+{
+    MfmaConst mc;
+    uint64_t pw = scale % p;
+    for (int d = 0; d < 7; ++d) {
+        mc.cd[d] = (int64_t)pw;
+        pw = pw * 256 % p;
+    }
+    const uint64_t two62 = (uint64_t)1 << 62;
+    mc.off = ((two62 + p - 1) / p) * p;
+    mc.r32 = (uint32_t)(((uint64_t)1 << 32) % p);
+    mc.r32p = shoup_pre(mc.r32, p);
+    mc.onep = shoup_pre(1, p);
+    const size_t mpad = rpad_of(m), npad = rpad_of(n), kchunks = kpad_of(k) / 64;
+    if (npts > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pw_mfma_kernel, dim3((unsigned)npts), dim3(256), 0, s, Al, Bl, Cmat, (int)m, (int)n,
+                       (int)kchunks, (int)mpad, (int)npad, limb_block_bytes(m, k), limb_block_bytes(n, k), p, mc);
+    return hipGetLastError();
+}
+
+hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries, size_t npts, hipStream_t s)
+{
+    if (!entries || !npts) return hipSuccess;
+    dim3 grid((unsigned)((entries + 63) / 64), (unsigned)((npts + 63) / 64));
+    if (grid.y > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(point_to_entry_kernel, grid, dim3(256), 0, s, X, Y, entries, npts);
+    return hipGetLastError();
+}
+
+}  // namespace lbx

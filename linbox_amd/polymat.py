@@ -1,0 +1,253 @@
+"""Host-side mirror of LinBox's polynomial-matrix interface over the C ABI.
+
+* PolynomialMatrix            -- PolynomialMatrix<Modular<uint32_t,uint64_t>,PMType>
+                                 (linbox/matrix/polynomial-matrix.h:54-910): the three
+                                 storage layouts, get/ref, resize, views (at()).
+* PolynomialMatrixMulDomain   -- PolynomialMatrixMulDomain::mul (polynomial-matrix-domain.h:56-75);
+                                 like the reference it does not forward max_rowdeg (:61).
+* PolynomialMatrixFFTMulDomain / PolynomialMatrixFFTPrimeMulDomain
+                              -- the FFT domains (matpoly-mult-fft-wordsize.inl:57-88,
+                                 matpoly-mult-fft-wordsize-fast.inl:62-190).
+* FFT                         -- FFT<Field> with FFT_direct / FFT_inverse (fft.h:57-144).
+
+Storage is a numpy uint32 array (host) or any device tensor exposing
+data_ptr() (e.g. a torch int32 CUDA tensor holding the same bits).
+"""
+import ctypes
+
+import numpy as np
+
+from . import (LinboxError, Polmat, Job, check, default_context, lib, POLFIRST, MATFIRST, MATROWFIRST,
+               METHOD_AUTO)
+
+LAYOUT_NAMES = {"polfirst": POLFIRST, "matfirst": MATFIRST, "matrowfirst": MATROWFIRST}
+
+
+def _layout(x):
+    return LAYOUT_NAMES[x] if isinstance(x, str) else int(x)
+
+
+def _ptr(data):
+    if isinstance(data, np.ndarray):
+        return data.ctypes.data
+    return int(data.data_ptr())
+
+
+class PolynomialMatrix:
+    """rows x cols matrix of polynomials with `size` coefficients each."""
+
+    def __init__(self, rows, cols, size, layout=POLFIRST, data=None, nlimbs=1):
+        self.rows, self.cols = int(rows), int(cols)
+        self.layout = _layout(layout)
+        self.nlimbs = nlimbs
+        self._size = int(size)
+        self.storage_size = int(size)
+        self.shift = 0
+        self._parent = None
+        n = self.rows * self.cols * self.storage_size * nlimbs
+        if data is None:
+            data = np.zeros(n, dtype=np.uint32)
+        self.data = data
+
+    # --- reference accessors -------------------------------------------
+    def rowdim(self):
+        return self.rows
+
+    def coldim(self):
+        return self.cols
+
+    def size(self):
+        return self._size
+
+    def degree(self):
+        return self._size - 1
+
+    def index(self, e, t):
+        """Storage word of coefficient t of entry e = i*cols+j (polynomial-matrix.h:131-137,422-428,660-683)."""
+        tt = self.shift + t
+        if self.layout == POLFIRST:
+            return e * self.storage_size + tt
+        if self.layout == MATFIRST:
+            return e + tt * self.rows * self.cols
+        i, j = divmod(e, self.cols)
+        return i * self.cols * self.storage_size + j + tt * self.cols
+
+    def get(self, i, j, k):
+        return int(self.data[self.index(i * self.cols + j, k)])
+
+    def ref_set(self, i, j, k, v):
+        self.data[self.index(i * self.cols + j, k)] = v
+
+    # --- bulk conversion (host numpy) ----------------------------------
+    def _index_grid(self):
+        e = np.arange(self.rows * self.cols, dtype=np.int64)[:, None]
+        t = np.arange(self._size, dtype=np.int64)[None, :] + self.shift
+        if self.layout == POLFIRST:
+            return e * self.storage_size + t
+        if self.layout == MATFIRST:
+            return e + t * self.rows * self.cols
+        i, j = e // self.cols, e % self.cols
+        return i * self.cols * self.storage_size + j + t * self.cols
+
+    def to_polfirst(self):
+        """(rows*cols, size) array of the visible coefficients."""
+        return np.asarray(self.data)[self._index_grid()]
+
+    def set_polfirst(self, arr):
+        self.data[self._index_grid()] = np.asarray(arr, dtype=np.uint32).reshape(self.rows * self.cols, self._size)
+
+    @classmethod
+    def from_polfirst(cls, arr, rows, cols, layout=POLFIRST):
+        arr = np.asarray(arr, dtype=np.uint32).reshape(rows * cols, -1)
+        M = cls(rows, cols, arr.shape[1], layout)
+        M.set_polfirst(arr)
+        return M
+
+    # --- size changes ---------------------------------------------------
+    def resize(self, s):
+        """Whole matrices reallocate keeping coefficients [0,min(s,size)) (polynomial-matrix.h:184-203);
+        views can only shrink (SubPolynomialMatrix::resize, :983-989)."""
+        s = int(s)
+        if self._parent is not None:
+            if s > self._size:
+                raise LinboxError(5, "SubPolynomialMatrix: resizing with larger size than available")
+            self._size = s
+            return
+        if s == self._size:
+            return
+        old = self.to_polfirst()
+        keep = min(s, self._size)
+        self.storage_size = s
+        self._size = s
+        self.data = np.zeros(self.rows * self.cols * s, dtype=np.uint32)
+        new = np.zeros((self.rows * self.cols, s), dtype=np.uint32)
+        new[:, :keep] = old[:, :keep]
+        self.set_polfirst(new)
+
+    def at(self, i, j):
+        """View of coefficients i..j (SubPolynomialMatrix(M,i,j), polynomial-matrix.h:954-967)."""
+        if not (i < self._size and i <= j < self._size):
+            raise LinboxError(7, "invalid view range")
+        v = PolynomialMatrix.__new__(PolynomialMatrix)
+        v.__dict__.update(self.__dict__)
+        v.shift = self.shift + i
+        v._size = j - i + 1
+        v._parent = self
+        return v
+
+    def polmat(self):
+        return Polmat(_ptr(self.data), self.layout, self.rows, self.cols, self._size, self.storage_size, self.shift)
+
+
+class PolynomialMatrixFFTMulDomain:
+    """PolynomialMatrixFFTMulDomain<Modular<uint32_t,uint64_t>> (matpoly-mult-fft-wordsize.inl:39-88)."""
+
+    def __init__(self, p, ctx=None, method=METHOD_AUTO, stream=None):
+        self.p = int(p)
+        self.ctx = ctx or default_context()
+        self.method = method
+        self.stream = stream
+
+    def mul(self, c, a, b, max_rowdeg=0):
+        deg = max_rowdeg if max_rowdeg else a.size() + b.size() - 2
+        if a.coldim() != b.rowdim():
+            raise LinboxError(5, "a.coldim() != b.rowdim()")
+        c.resize(deg + 1)
+        pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+        check(lib().lbx_polymat_mul_u32(self.ctx.handle, self.p, ctypes.byref(pc), ctypes.byref(pa),
+                                        ctypes.byref(pb), max_rowdeg, self.method, self.stream))
+        return c
+
+
+class PolynomialMatrixMulDomain(PolynomialMatrixFFTMulDomain):
+    """PolynomialMatrixMulDomain::mul (polynomial-matrix-domain.h:56-75): d = a.size()+b.size();
+    FFT when d > 2, naive otherwise -- both run on the device here.  max_rowdeg is
+    accepted and, as in the reference (:61), not forwarded."""
+
+    def mul(self, c, a, b, max_rowdeg=0):
+        return PolynomialMatrixFFTMulDomain.mul(self, c, a, b, 0)
+
+
+class PolynomialMatrixFFTPrimeMulDomain:
+    """PolynomialMatrixFFTPrimeMulDomain (matpoly-mult-fft-wordsize-fast.inl:40-190)."""
+
+    def __init__(self, p, ctx=None, method=METHOD_AUTO, stream=None):
+        self.p = int(p)
+        self.ctx = ctx or default_context()
+        self.method = method
+        self.stream = stream
+
+    def mul(self, c, a, b, max_rowdeg=0):
+        # mul (:62-82) pads to pts and calls mul_fft; the C ABI does both
+        return PolynomialMatrixFFTMulDomain(self.p, self.ctx, self.method, self.stream).mul(c, a, b, max_rowdeg)
+
+    def mul_fft(self, lpts, c, a, b):
+        """a, b, c: polfirst PolynomialMatrix of size 2^lpts (a and b are not modified)."""
+        check(lib().lbx_polymat_mul_fft_u32(self.ctx.handle, self.p, lpts, a.rowdim(), a.coldim(), b.coldim(),
+                                            _ptr(a.data), _ptr(b.data), _ptr(c.data), self.method, self.stream))
+        return c
+
+
+def _ntt(fn, p, lpts, data, count, root, ctx, stream):
+    ctx = ctx or default_context()
+    used = ctypes.c_uint32(0)
+    check(fn(ctx.handle, p, lpts, root, _ptr(data), count, ctypes.byref(used), stream))
+    return used.value
+
+
+def ntt_forward(p, lpts, data, count=None, root=0, ctx=None, stream=None):
+    """In-place FFT_direct of `count` polynomials of 2^lpts words; returns the root used."""
+    count = (len(data) >> lpts) if count is None else count
+    return _ntt(lib().lbx_ntt_forward_u32, p, lpts, data, count, root, ctx, stream)
+
+
+def ntt_inverse(p, lpts, data, count=None, root=0, ctx=None, stream=None):
+    """In-place FFT_inverse (DIT, no scaling) of `count` polynomials; returns the root used."""
+    count = (len(data) >> lpts) if count is None else count
+    return _ntt(lib().lbx_ntt_inverse_u32, p, lpts, data, count, root, ctx, stream)
+
+
+class FFT:
+    """FFT<Modular<uint32_t,uint64_t>>(F, k, w) (fft.h:57-144)."""
+
+    def __init__(self, p, k, w=0, ctx=None):
+        self.p, self.k, self.ctx = int(p), int(k), ctx
+        probe = np.zeros(1 << k, dtype=np.uint32)
+        self.w = ntt_forward(self.p, self.k, probe, count=0, root=w, ctx=ctx)
+
+    def root(self):
+        return self.p - 1 if self.k == 1 else self.w
+
+    def invroot(self):
+        if self.k == 1:
+            return self.p - 1
+        return (self.p - pow(self.w, (1 << (self.k - 1)) - 1, self.p)) % self.p
+
+    def FFT_direct(self, coeffs):
+        ntt_forward(self.p, self.k, coeffs, count=1, root=self.w, ctx=self.ctx)
+        return coeffs
+
+    def FFT_inverse(self, coeffs):
+        ntt_inverse(self.p, self.k, coeffs, count=1, root=self.w, ctx=self.ctx)
+        return coeffs
+
+
+def mul_mp(P, c, a, b, ctx=None, method=METHOD_AUTO, stream=None):
+    """Product over Z/PZ for a multiprecision P (int); a, b, c hold nlimbs words per coefficient."""
+    ctx = ctx or default_context()
+    nl = a.nlimbs
+    limbs = np.array([(P >> (32 * i)) & 0xFFFFFFFF for i in range(nl)], dtype=np.uint32)
+    pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+    check(lib().lbx_polymat_mul_mp(ctx.handle, limbs.ctypes.data, nl, ctypes.byref(pc), ctypes.byref(pa),
+                                   ctypes.byref(pb), method, stream))
+    c._size = pc.size
+    return c
+
+
+def mul_batch(jobs, ctx=None, method=METHOD_AUTO, stream=None):
+    """jobs: list of dicts with p, m, k, n, sa, sb and device pointers a, b, c."""
+    ctx = ctx or default_context()
+    arr = (Job * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = Job(j["p"], j["m"], j["k"], j["n"], j["sa"], j["sb"], j["a"], j["b"], j["c"])
+    check(lib().lbx_polymat_mul_batch(ctx.handle, arr, len(jobs), method, stream))

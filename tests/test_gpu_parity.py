@@ -1,0 +1,298 @@
+"""GPU parity tests: the HIP path through the C ABI against the oracle and fixtures.
+
+Bit-exact equality is required everywhere (integer arithmetic).  Small and
+mid sizes are compared word for word with the oracle (oracle/lbx_oracle.c) and
+the committed golden fixtures; full-size configurations are checked through
+size-independent properties (random-point evaluation, agreement of the two
+pointwise forms).  Every step prints its case first so a failing run names it.
+"""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import linbox_amd as lbx
+from linbox_amd import PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain, LinboxError
+from oracle import oracle as orc
+from polycheck import check_by_evaluation
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+PRIMES = [998244353, 2013265921, 469762049, 167772161]
+METHODS = [lbx.METHOD_PLAIN_U64, lbx.METHOD_INT8_MFMA]
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lbx.Context(0)
+    yield c
+    c.close()
+
+
+def rand(rng, p, n):
+    return rng.integers(0, p, n, dtype=np.uint64).astype(np.uint32)
+
+
+def gpu_mul(ctx, p, m, k, n, a, sa, b, sb, method, layouts=("polfirst",) * 3, max_rowdeg=0):
+    A = PolynomialMatrix.from_polfirst(np.asarray(a).reshape(m * k, sa), m, k, layouts[0])
+    B = PolynomialMatrix.from_polfirst(np.asarray(b).reshape(k * n, sb), k, n, layouts[1])
+    C = PolynomialMatrix(m, n, 1, layouts[2])
+    PolynomialMatrixFFTMulDomain(p, ctx, method).mul(C, A, B, max_rowdeg)
+    return C.to_polfirst()
+
+
+# ------------------------------------------------------------------ NTT ---
+def test_ntt_golden_kat(ctx):
+    for case in load("ntt_kat.json"):
+        p, L = case["p"], case["L"]
+        say("ntt kat", p, L)
+        x = np.array(case["in"], dtype=np.uint32)
+        w = lbx.ntt_forward(p, L, x, ctx=ctx)
+        assert w == case["root"]
+        assert x.tolist() == case["out_br"]
+        y = np.array(case["in_br"], dtype=np.uint32)
+        lbx.ntt_inverse(p, L, y, root=case["root"], ctx=ctx)
+        assert y.tolist() == case["out"]
+
+
+@pytest.mark.parametrize("p", PRIMES)
+def test_ntt_vs_oracle_all_sizes(ctx, p):
+    rng = np.random.default_rng(p + 1)
+    for L in range(1, 16):
+        count = max(1, min(37, (1 << 18) >> L))
+        say("ntt vs oracle", p, L, count)
+        x = rand(rng, p, count << L)
+        y = x.copy()
+        w = lbx.ntt_forward(p, L, y, ctx=ctx)
+        ref = np.concatenate([orc.fft_direct(p, L, x[i << L:(i + 1) << L])[0] for i in range(count)])
+        assert np.array_equal(y, ref), (p, L)
+        wi = orc.invroot(p, L, w)
+        z = y.copy()
+        lbx.ntt_inverse(p, L, z, root=wi, ctx=ctx)
+        ref = np.concatenate([orc.fft_inverse(p, L, y[i << L:(i + 1) << L], root=wi)[0] for i in range(count)])
+        assert np.array_equal(z, ref), (p, L)
+        assert np.array_equal(z.astype(np.uint64), x.astype(np.uint64) * (1 << L) % p)
+
+
+def test_ntt_errors(ctx):
+    x = np.zeros(8, dtype=np.uint32)
+    with pytest.raises(LinboxError) as e:
+        lbx.ntt_forward(998244353, 0, x, count=1, ctx=ctx)
+    assert e.value.status == 3
+    with pytest.raises(LinboxError) as e:
+        lbx.ntt_forward(998244353, 3, x, root=998244352, ctx=ctx)
+    assert e.value.status == 4
+    with pytest.raises(LinboxError) as e:
+        lbx.ntt_forward(998244353, 24, x, count=0, ctx=ctx)
+    assert e.value.status == 2
+
+
+# -------------------------------------------------------------- product ---
+@pytest.mark.parametrize("method", METHODS)
+def test_mul_golden_fixtures(ctx, method):
+    for cs in load("polymat_mul.json"):
+        say("golden", cs["p"], cs["m"], cs["k"], cs["n"], cs["sa"], cs["sb"], cs["max_rowdeg"], method)
+        got = gpu_mul(ctx, cs["p"], cs["m"], cs["k"], cs["n"], cs["a"], cs["sa"], cs["b"], cs["sb"], method,
+                      max_rowdeg=cs["max_rowdeg"])
+        assert got.reshape(-1).tolist() == cs["c"]
+
+
+SHAPES = [(16, 16, 16, 256, 256), (5, 7, 3, 100, 37), (33, 65, 17, 40, 90), (1, 1, 1, 3000, 5000),
+          (64, 3, 48, 17, 17), (2, 130, 2, 9, 8)]
+
+
+@pytest.mark.parametrize("p", PRIMES)
+@pytest.mark.parametrize("method", METHODS)
+def test_mul_vs_oracle(ctx, p, method):
+    rng = np.random.default_rng(p ^ method)
+    for (m, k, n, sa, sb) in SHAPES:
+        say("mul vs oracle", p, m, k, n, sa, sb, method)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        assert st == 0
+        got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, method)
+        assert np.array_equal(got.reshape(-1), ref), (p, m, k, n, sa, sb, method)
+
+
+def test_mul_config1_vs_naive(ctx):
+    """16x16.16x16 over 998244353, degree 255: bit-exact vs naive convolution."""
+    p, m, k, n, sa, sb = 998244353, 16, 16, 16, 256, 256
+    a = orc.fill_uniform(0x4C494E424F58 + 0, p, m * k * sa)
+    b = orc.fill_uniform(0x4C494E424F58 + 100, p, k * n * sb)
+    ref = orc.mul_naive(p, m, k, n, a, sa, b, sb)
+    for method in METHODS:
+        say("config1", method)
+        got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, method)
+        assert np.array_equal(got.reshape(-1), ref)
+
+
+@pytest.mark.parametrize("layouts", [("matfirst", "matrowfirst", "polfirst"), ("matrowfirst", "matfirst", "matfirst"),
+                                     ("polfirst", "polfirst", "matrowfirst")])
+def test_mul_layouts(ctx, layouts):
+    p = 469762049
+    rng = np.random.default_rng(3)
+    m, k, n, sa, sb = 6, 5, 7, 33, 20
+    a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+    _, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+    for method in METHODS:
+        say("layouts", layouts, method)
+        got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, method, layouts=layouts)
+        assert np.array_equal(got.reshape(-1), ref)
+
+
+def test_mul_views(ctx):
+    """Operands and result as SubPolynomialMatrix windows of larger storage."""
+    p = 167772161
+    rng = np.random.default_rng(4)
+    m, k, n = 3, 4, 2
+    Abig = PolynomialMatrix.from_polfirst(rand(rng, p, m * k * 50).reshape(m * k, 50), m, k, "matfirst")
+    Bbig = PolynomialMatrix.from_polfirst(rand(rng, p, k * n * 40).reshape(k * n, 40), k, n, "polfirst")
+    A, B = Abig.at(5, 24), Bbig.at(3, 33)  # sizes 20 and 31
+    Cbig = PolynomialMatrix.from_polfirst(np.full((m * n, 80), 7, dtype=np.uint32), m, n, "matrowfirst")
+    C = Cbig.at(10, 79)  # 70 >= 50 coefficients
+    _, ref = orc.mul_fft_prime(p, m, k, n, A.to_polfirst().reshape(-1), 20, B.to_polfirst().reshape(-1), 31)
+    say("views")
+    PolynomialMatrixMulDomain(p, ctx).mul(C, A, B)
+    assert C.size() == 50
+    full = Cbig.to_polfirst()
+    assert np.array_equal(full[:, 10:60].reshape(-1), ref)
+    assert (full[:, :10] == 7).all() and (full[:, 60:] == 7).all()
+    small = Cbig.at(0, 20)
+    with pytest.raises(LinboxError) as e:
+        PolynomialMatrixMulDomain(p, ctx).mul(small, A, B)
+    assert e.value.status == 5
+
+
+def test_mul_errors(ctx):
+    p = 998244353
+    A = PolynomialMatrix(2, 3, 4)
+    B = PolynomialMatrix(2, 2, 4)
+    C = PolynomialMatrix(2, 2, 7)
+    with pytest.raises(LinboxError) as e:
+        PolynomialMatrixMulDomain(p, ctx).mul(C, A, B)
+    assert e.value.status == 5
+    a = np.ones(8, dtype=np.uint32)
+    c = np.zeros(8, dtype=np.uint32)
+    st = lbx.lib().lbx_polymat_mul_fft_u32(ctx.handle, 7, 3, 1, 1, 1, a.ctypes.data, a.ctypes.data,
+                                           c.ctypes.data, 0, None)
+    assert st == 1  # 8 does not divide 7-1: bad FFT prime
+
+
+def test_constant_matrices(ctx):
+    """a.size()+b.size() <= 2 takes the naive route of PolynomialMatrixMulDomain::mul."""
+    p = 2013265921
+    rng = np.random.default_rng(9)
+    m, k, n = 7, 9, 5
+    a, b = rand(rng, p, m * k), rand(rng, p, k * n)
+    ref = orc.mul_naive(p, m, k, n, a, 1, b, 1)
+    say("constant")
+    got = gpu_mul(ctx, p, m, k, n, a, 1, b, 1, lbx.METHOD_AUTO)
+    assert np.array_equal(got.reshape(-1), ref)
+
+
+def test_device_pointers(ctx):
+    from linbox_amd.hip import DeviceBuffer, synchronize
+    p, m, k, n, sa, sb = 2013265921, 8, 8, 8, 64, 64
+    rng = np.random.default_rng(12)
+    a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+    _, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+    da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+    dc = DeviceBuffer(m * n * (sa + sb - 1) * 4)
+    for method in METHODS:
+        say("device pointers", method)
+        dc.zero()
+        A = PolynomialMatrix(m, k, sa, data=da)
+        B = PolynomialMatrix(k, n, sb, data=db)
+        C = PolynomialMatrix(m, n, sa + sb - 1, data=dc)
+        PolynomialMatrixMulDomain(p, ctx, method).mul(C, A, B)
+        synchronize()
+        assert np.array_equal(dc.download(), ref)
+
+
+TORCH_SCRIPT = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + "/tests")
+import linbox_amd as lbx
+from linbox_amd import PolynomialMatrix, PolynomialMatrixMulDomain
+from oracle import oracle as orc
+p, m, k, n, sa, sb = 998244353, 9, 17, 33, 70, 31
+rng = np.random.default_rng(5)
+a = rng.integers(0, p, m * k * sa, dtype=np.uint64).astype(np.uint32)
+b = rng.integers(0, p, k * n * sb, dtype=np.uint64).astype(np.uint32)
+_, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+ta = torch.from_numpy(a.view(np.int32)).cuda()
+tb = torch.from_numpy(b.view(np.int32)).cuda()
+tc = torch.zeros(m * n * (sa + sb - 1), dtype=torch.int32, device="cuda")
+s = torch.cuda.current_stream().cuda_stream
+for method in (1, 2):
+    tc.zero_()
+    PolynomialMatrixMulDomain(p, method=method, stream=s).mul(
+        PolynomialMatrix(m, n, sa + sb - 1, data=tc), PolynomialMatrix(m, k, sa, data=ta),
+        PolynomialMatrix(k, n, sb, data=tb))
+    torch.cuda.synchronize()
+    assert np.array_equal(tc.cpu().numpy().view(np.uint32), ref), method
+print("torch interop ok")
+"""
+
+
+def test_torch_tensor_interop():
+    """With torch imported first, liblbx binds torch's HIP runtime and accepts its tensors."""
+    import subprocess
+    pytest.importorskip("torch")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    say("torch interop subprocess")
+    r = subprocess.run([sys.executable, "-c", TORCH_SCRIPT, root], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "torch interop ok" in r.stdout
+
+
+# ----------------------------------------------------------- full sizes ---
+def full_size_check(ctx, p, m, sa, seed):
+    from linbox_amd.hip import DeviceBuffer, synchronize
+    a = orc.fill_uniform(seed, p, m * m * sa)
+    b = orc.fill_uniform(seed + 100, p, m * m * sa)
+    da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+    sc = 2 * sa - 1
+    dc = DeviceBuffer(m * m * sc * 4)
+    outs = []
+    for method in METHODS:
+        say("full size", m, sa, p, method)
+        dc.zero()
+        A, B, C = (PolynomialMatrix(m, m, sa, data=da), PolynomialMatrix(m, m, sa, data=db),
+                   PolynomialMatrix(m, m, sc, data=dc))
+        PolynomialMatrixMulDomain(p, ctx, method).mul(C, A, B)
+        synchronize()
+        outs.append(dc.download())
+    dc.free()
+    da.free()
+    db.free()
+    lbx.lib().lbx_trim(ctx.handle)
+    assert np.array_equal(outs[0], outs[1])
+    rng = np.random.default_rng(seed)
+    xs = [int(x) for x in rng.integers(2, p, 2)]
+    say("full size evaluation check", m, sa)
+    assert check_by_evaluation(a, b, outs[0], p, m, m, m, sa, sa, sc, xs)
+
+
+def test_config2_full_size(ctx):
+    """64x64.64x64, degree 4095, p=2013265921: plain == MFMA and C(x)=A(x)B(x)."""
+    full_size_check(ctx, 2013265921, 64, 4096, 0x4C494E424F58 + 1)
+
+
+def test_config3_full_size(ctx):
+    """128x128.128x128, degree 8191, p=2013265921 (the headline shape)."""
+    full_size_check(ctx, 2013265921, 128, 8192, 0x4C494E424F58 + 2)
