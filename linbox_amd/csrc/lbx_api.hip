@@ -256,6 +256,26 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     method = choose_method(method, m, k, n);
 
     Stages st(ctx, s);
+    if (method == LBX_METHOD_INT8_MFMA && L >= 8) {
+        // split transforms: the last six forward stages write per-point limb
+        // blocks and the first six inverse stages read the point-major product
+        void *al, *bl, *cmat;
+        LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
+        LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
+        LBX_TRY(ensure(ctx, "cmat", m * n * N * 4, &cmat));
+        LBX_TRY_HIP(launch_ntt_forward_head(a, m * k, L, twf, p, (uint32_t *)ahat, s));
+        LBX_TRY_HIP(launch_ntt_forward_head(b, k * n, L, twf, p, (uint32_t *)bhat, s));
+        LBX_TRY_HIP(launch_dif_tail_limbs((const uint32_t *)ahat, (uint8_t *)al, m, k, true, L, twf, p, s));
+        LBX_TRY_HIP(launch_dif_tail_limbs((const uint32_t *)bhat, (uint8_t *)bl, n, k, false, L, twf, p, s));
+        st.mark("direct FFT_DIF");
+        LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N, p,
+                                          inv_n, s));
+        st.mark("Pointwise mult");
+        LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, L, twi, p, (uint32_t *)chat, s));
+        LBX_TRY_HIP(launch_ntt_inverse_tail((const uint32_t *)chat, m * n, L, twi, p, c, count, 1, s));
+        st.mark("inverse FFT_DIT");
+        return LBX_OK;
+    }
     LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twf, p, (uint32_t *)ahat, s));
     LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twf, p, (uint32_t *)bhat, s));
     st.mark("direct FFT_DIF");

@@ -50,6 +50,22 @@ hipError_t launch_ntt_forward(const DevView &in, size_t entries, unsigned L, con
 hipError_t launch_ntt_inverse(const uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
                               const DevView &out, size_t count, uint32_t scale, hipStream_t s);
 
+// Split transforms for the point-major pointwise form (L in [8,15]):
+// forward head = DIF stages [0, L-6) per entry, polfirst out;
+// forward tail = the last six DIF stages on 64-point blocks of 64 entries,
+//                written straight into per-point limb blocks (see launch_to_limbs);
+// inverse head = the first six DIT stages on 64-point blocks of 64 consecutive
+//                entries of the point-major product C[t][e], polfirst out;
+// inverse tail = DIT stages [6, L) per entry, written to the view.
+hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
+                                   uint32_t *out, hipStream_t s);
+hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows, size_t kdim, bool row_major, unsigned L,
+                                 const uint2 *tw, uint32_t p, hipStream_t s);
+hipError_t launch_dit_head(const uint32_t *Cm, size_t entries, unsigned L, const uint2 *tw, uint32_t p, uint32_t *Y,
+                           hipStream_t s);
+hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
+                                   const DevView &out, size_t count, uint32_t scale, hipStream_t s);
+
 // ---- pointwise stage (pointwise.hip) ----
 // C[e'][t] = scale * sum_q A[i*k+q][t] B[q*n+j][t] mod p at every point t < npts,
 // all operands polfirst with row stride npts (e' = i*n+j).

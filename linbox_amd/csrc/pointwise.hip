@@ -225,6 +225,94 @@ __global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict_
     }
 }
 
+// LDS-staged form, used when one point's A limb block fits in 64 KiB (m*k <= 16384
+// after padding): persistent workgroups, two per CU, each takes a point, copies
+// its A block to LDS once, then streams B through LDS 32 columns at a time; every
+// wave computes the 32x32 tiles of one row band.  16-byte chunks are stored
+// XOR-swizzled within each row (chunk c of row r at c ^ (r & 15)) so the 16 rows
+// a ds_read_b128 touches fall in distinct banks.
+constexpr int kLdsAMax = 65536;
+
+__device__ __forceinline__ void mfma_tile_lds(const uint4 *__restrict__ lA, const uint4 *__restrict__ lB, int cpr,
+                                              int kchunks, int arow0, int l16, int lg, v4i (&acc)[2][2][7])
+{
+#pragma unroll
+    for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int d = 0; d < 7; ++d) acc[ai][bj][d] = v4i{0, 0, 0, 0};
+    for (int kb = 0; kb < kchunks; ++kb) {
+        v4i a[2][4], b[2][4];
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int c = (kb * 16 + s * 4 + lg) ^ l16;
+                const uint4 va = lA[(arow0 + sub * 16 + l16) * cpr + c];
+                const uint4 vb = lB[(sub * 16 + l16) * cpr + c];
+                a[sub][s] = v4i{(int)va.x, (int)va.y, (int)va.z, (int)va.w};
+                b[sub][s] = v4i{(int)vb.x, (int)vb.y, (int)vb.z, (int)vb.w};
+            }
+#pragma unroll
+        for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        acc[ai][bj][s + u] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ai][s], b[bj][u], acc[ai][bj][s + u], 0, 0, 0);
+    }
+}
+
+__global__ void __launch_bounds__(256, 2) pw_mfma_lds_kernel(const uint8_t *__restrict__ Al,
+                                                             const uint8_t *__restrict__ Bl, uint32_t *__restrict__ Cmat,
+                                                             int m, int n, int kchunks, int mpad, int npad,
+                                                             size_t npts, uint32_t p, MfmaConst mc)
+{
+    extern __shared__ uint4 smem4[];
+    const int cpr = kchunks * 16;  // 16-byte chunks per row
+    uint4 *lA = smem4;
+    uint4 *lB = smem4 + (size_t)mpad * cpr;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, lg = lane >> 4;
+    const int a_chunks = mpad * cpr, b_chunks = 32 * cpr;
+    const int nrt = mpad / 32, ncb = npad / 32;
+    for (size_t t = blockIdx.x; t < npts; t += gridDim.x) {
+        const uint4 *gA = reinterpret_cast<const uint4 *>(Al) + t * (size_t)a_chunks;
+        const uint4 *gB = reinterpret_cast<const uint4 *>(Bl) + t * (size_t)npad * cpr;
+        __syncthreads();  // the previous point is done with lA and lB
+        for (int i = tid; i < a_chunks; i += 256) {
+            const int r = i / cpr, c = i - r * cpr;
+            lA[r * cpr + (c ^ (r & 15))] = gA[i];
+        }
+        uint32_t *Ct = Cmat + t * (size_t)m * n;
+        for (int cb = 0; cb < ncb; ++cb) {
+            if (cb) __syncthreads();
+            const uint4 *gBb = gB + (size_t)cb * b_chunks;
+            for (int i = tid; i < b_chunks; i += 256) {
+                const int r = i / cpr, c = i - r * cpr;
+                lB[r * cpr + (c ^ (r & 15))] = gBb[i];
+            }
+            __syncthreads();
+            for (int rt = wave; rt < nrt; rt += 4) {
+                v4i acc[2][2][7];
+                mfma_tile_lds(lA, lB, cpr, kchunks, rt * 32, l16, lg, acc);
+#pragma unroll
+                for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int i = rt * 32 + ai * 16 + lg * 4 + r, j = cb * 32 + bj * 16 + l16;
+                            if (i < m && j < n) Ct[(size_t)i * n + j] = mfma_combine(acc[ai][bj], r, p, mc);
+                        }
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__restrict__ X, uint32_t *__restrict__ Y,
                                                              size_t entries, size_t npts)
 {
@@ -297,6 +385,24 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
     mc.onep = shoup_pre(1, p);
     const size_t mpad = rpad_of(m), npad = rpad_of(n), kchunks = kpad_of(k) / 64;
     if (npts > 0x7fffffff) return hipErrorInvalidValue;
+    const size_t a_bytes = limb_block_bytes(m, k);
+    if (a_bytes <= (size_t)kLdsAMax) {
+        static int cus = 0;
+        if (!cus) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+                cus = 256;
+        }
+        const size_t lds = a_bytes + (size_t)32 * kchunks * 256;
+        hipError_t e = hipFuncSetAttribute((const void *)pw_mfma_lds_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        const size_t grid = npts < (size_t)cus * 2 ? npts : (size_t)cus * 2;
+        hipLaunchKernelGGL(pw_mfma_lds_kernel, dim3((unsigned)grid), dim3(256), lds, s, Al, Bl, Cmat, (int)m, (int)n,
+                           (int)kchunks, (int)mpad, (int)npad, npts, p, mc);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(pw_mfma_kernel, dim3((unsigned)npts), dim3(256), 0, s, Al, Bl, Cmat, (int)m, (int)n,
                        (int)kchunks, (int)mpad, (int)npad, limb_block_bytes(m, k), limb_block_bytes(n, k), p, mc);
     return hipGetLastError();

@@ -128,6 +128,26 @@ def test_mul_vs_oracle(ctx, p, method):
         assert np.array_equal(got.reshape(-1), ref), (p, m, k, n, sa, sb, method)
 
 
+@pytest.mark.parametrize("p", [2013265921, 998244353])
+def test_mul_every_transform_length(ctx, p):
+    """Product lengths 2^1..2^15 (every transform kernel instantiation, split and whole), both forms."""
+    rng = np.random.default_rng(p + 77)
+    for L in range(1, 16):
+        pts = 1 << L
+        sa = max(1, pts // 2 - (L % 3))
+        sb = pts - sa  # deg = sa+sb-2 = pts-2 -> length 2^L
+        if sa + sb < 3:
+            sa, sb = 1, 2
+        m, k, n = (3, 5, 2) if L < 13 else (2, 2, 2)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        assert st == 0
+        for method in METHODS:
+            say("length", L, p, m, k, n, sa, sb, method)
+            got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, method)
+            assert np.array_equal(got.reshape(-1), ref), (L, method)
+
+
 def test_mul_config1_vs_naive(ctx):
     """16x16.16x16 over 998244353, degree 255: bit-exact vs naive convolution."""
     p, m, k, n, sa, sb = 998244353, 16, 16, 16, 256, 256
