@@ -12,33 +12,9 @@
 #include <vector>
 
 #include "lbx/lbx.h"
-#include "lbx_internal.h"
-#include "lbx_mp.h"
+#include "lbx_ctx.h"
 
 namespace lbx {
-
-struct Buffer {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-};
-
-}  // namespace lbx
-
-struct lbx_ctx {
-    int device = 0;
-    std::mutex mu;
-    std::map<std::string, lbx::Buffer> bufs;
-    std::map<std::tuple<uint32_t, unsigned, uint32_t>, uint2 *> twiddles;
-    bool profiling = false;
-    // recorded (label, start, end) event triples, resolved by lbx_profile_get
-    std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;
-    std::vector<std::string> prof_labels;
-    std::vector<float> prof_ms;
-    std::vector<const char *> prof_label_ptrs;
-};
-
-namespace lbx {
-namespace {
 
 lbx_status from_hip(hipError_t e)
 {
@@ -47,29 +23,6 @@ lbx_status from_hip(hipError_t e)
     return LBX_ERR_HIP;
 }
 
-#define LBX_TRY_HIP(expr)                           \
-    do {                                            \
-        hipError_t e_ = (expr);                     \
-        if (e_ != hipSuccess) return from_hip(e_);  \
-    } while (0)
-#define LBX_TRY(expr)                   \
-    do {                                \
-        lbx_status s_ = (expr);         \
-        if (s_ != LBX_OK) return s_;    \
-    } while (0)
-
-class DeviceGuard {
-public:
-    explicit DeviceGuard(int dev)
-    {
-        (void)hipGetDevice(&prev_);
-        if (prev_ != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() { (void)hipSetDevice(prev_); }
-
-private:
-    int prev_ = 0;
-};
 
 lbx_status ensure(lbx_ctx *ctx, const char *name, size_t bytes, void **out)
 {
@@ -232,8 +185,6 @@ int choose_method(int method, size_t m, size_t k, size_t n)
     return (m >= 16 && n >= 16 && k >= 16) ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
 }
 
-}  // namespace
-
 // Forward transforms, pointwise products and inverse transform of one
 // single-prime product on device views.  deg+1 coefficients of c are written.
 lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
@@ -305,7 +256,7 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
 }
 
 // Constant matrices (a.size()+b.size() <= 2): one pointwise product.
-static lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c,
+lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c,
                                    size_t m, size_t k, size_t n, hipStream_t s)
 {
     void *ahat, *bhat, *chat;
@@ -320,14 +271,7 @@ static lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, c
     return LBX_OK;
 }
 
-// Host operands are staged through device copies of their whole storage.
-struct Staged {
-    lbx_polmat v;
-    void *dev = nullptr;
-    bool owned = false;
-};
-
-static lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out)
+lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out)
 {
     out->v = *src;
     if (is_device_ptr(src->data)) {
@@ -341,7 +285,7 @@ static lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_i
     return LBX_OK;
 }
 
-static void release(Staged &s)
+void release(Staged &s)
 {
     if (s.owned && s.dev) (void)hipFree(s.dev);
     s.dev = nullptr;
@@ -389,7 +333,7 @@ lbx_status lbx_destroy(lbx_ctx *ctx)
         for (auto &kv : ctx->bufs)
             if (kv.second.ptr) (void)hipFree(kv.second.ptr);
         for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
-        lbx_mp_release(ctx);
+        mp_release(ctx);
     }
     delete ctx;
     return LBX_OK;
@@ -488,8 +432,11 @@ lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lb
         ++L;
     }
     if (sa > pts || sb > pts) return LBX_ERR_SIZE_MISMATCH;
-    const bool constant = sa + sb <= 2;
-    const bool single_prime = !constant && p < (1u << 31) && (p - 1) % pts == 0;
+    // word-size kernels keep residues in [0,p) with 2p < 2^32; larger p go
+    // through the CRT route, whose primes are below 2^31
+    const bool small_p = p < (1u << 31);
+    const bool constant = sa + sb <= 2 && small_p;
+    const bool single_prime = !constant && small_p && sa + sb > 2 && (p - 1) % pts == 0;
     if (!constant && !single_prime) {
         if (max_rowdeg && max_rowdeg != sa + sb - 2) return LBX_ERR_UNSUPPORTED;
         lbx_status st = lbx_polymat_mul_mp(ctx, &p, 1, c, a, b, method, stream);

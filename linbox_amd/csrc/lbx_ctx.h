@@ -1,0 +1,95 @@
+// Context and helpers shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "lbx/lbx.h"
+#include "lbx_internal.h"
+
+namespace lbx {
+
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct MpPlan;  // cached CRT constants of the multiprecision route (mp.hip)
+
+}  // namespace lbx
+
+struct lbx_ctx {
+    int device = 0;
+    std::mutex mu;
+    std::map<std::string, lbx::Buffer> bufs;
+    std::map<std::tuple<uint32_t, unsigned, uint32_t>, uint2 *> twiddles;
+    std::map<std::string, lbx::MpPlan *> mp_plans;
+    bool profiling = false;
+    // recorded (label, start, end) event triples, resolved by lbx_profile_get
+    std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;
+    std::vector<std::string> prof_labels;
+    std::vector<float> prof_ms;
+    std::vector<const char *> prof_label_ptrs;
+};
+
+#define LBX_TRY_HIP(expr)                                 \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return ::lbx::from_hip(e_); \
+    } while (0)
+#define LBX_TRY(expr)                \
+    do {                             \
+        lbx_status s_ = (expr);      \
+        if (s_ != LBX_OK) return s_; \
+    } while (0)
+
+namespace lbx {
+
+lbx_status from_hip(hipError_t e);
+
+class DeviceGuard {
+public:
+    explicit DeviceGuard(int dev)
+    {
+        (void)hipGetDevice(&prev_);
+        if (prev_ != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev_); }
+
+private:
+    int prev_ = 0;
+};
+
+// Named device workspace, grown on demand and kept between calls.
+lbx_status ensure(lbx_ctx *ctx, const char *name, size_t bytes, void **out);
+bool is_device_ptr(const void *p);
+size_t storage_words(const lbx_polmat *v);
+lbx_status check_view(const lbx_polmat *v);
+DevView devview(const lbx_polmat *v, void *dev_base);
+
+// Host operands are staged through device copies of their whole storage
+// (word_bytes per coefficient slot).
+struct Staged {
+    lbx_polmat v;
+    void *dev = nullptr;
+    bool owned = false;
+};
+lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out);
+void release(Staged &s);
+
+// Single-prime product on device views (count = deg+1 coefficients of c).
+lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
+                             const DevView &c, size_t m, size_t k, size_t n, size_t count, int method,
+                             hipStream_t s);
+// Product of constant matrices (one coefficient each).
+lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c, size_t m,
+                            size_t k, size_t n, hipStream_t s);
+// Frees the multiprecision plans cached in a context.
+void mp_release(lbx_ctx *ctx);
+
+}  // namespace lbx

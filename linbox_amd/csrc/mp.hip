@@ -1,19 +1,460 @@
-// Multiprecision modulus route (placeholder until the CRT kernels land).
-#include "lbx_mp.h"
+// Multiprecision modulus route: PolynomialMatrixFFTMulDomain<Modular<integer>>::mul
+// (matpoly-mult-fft-multiprecision.inl:611-630) over ZRing<integer>::mul_crtla
+// (:141-277), and the same route for word-size primes that are not FFT primes
+// (the reference's few-primes domain, three-primes.inl:67-191).
+//
+// The integer product of the [0,P) representatives has coefficients below
+// bound = 2 P^2 k min(sa,sb) (:616).  It is computed modulo FFT primes
+// p_i < 2^31 with 2^(lpts+1) | p_i - 1, chosen in the deterministic order of
+// RandomFFTPrime::generatePrimes (random-fftprime.h:107-133) until their
+// product exceeds the bound, then recombined on the device:
+//   1. residue kernel: every nlimbs-word coefficient -> its residue mod each p_i
+//      (Horner over the 32-bit limbs with Shoup products);
+//   2. one single-prime NTT product per p_i (the same kernels as lbx_polymat_mul_u32);
+//   3. CRT kernel: Garner's mixed-radix digits v_i of each coefficient, then
+//      X mod P = sum_i v_i (prod_{j<i} p_j mod P) in 32-bit limbs, reduced by
+//      shift-and-subtract, written as nlimbs words.
+// The reference picks ~23-bit primes for fgemm on doubles (:162-164); the
+// primes here are 31-bit because every step is 32-bit integer arithmetic.
+// Products are unique, so the output is identical.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbx_ctx.h"
 
 namespace lbx {
-void lbx_mp_release(lbx_ctx *) {}
+
+constexpr int kMaxPrimes = 64;
+constexpr int kMaxLimbs = 16;
+
+struct MpPlan {
+    std::vector<uint32_t> primes;
+    unsigned lpts = 0;
+    int nlimbs = 0;
+    uint32_t *d_const = nullptr;  // device constant block, layout described at MpLayout
+};
+
+namespace {
+
+typedef std::vector<uint32_t> Limbs;
+
+void limbs_trim(Limbs &x)
+{
+    while (x.size() > 1 && x.back() == 0) x.pop_back();
+}
+
+void limbs_mul_small(Limbs &x, uint32_t v)
+{
+    uint64_t carry = 0;
+    for (auto &w : x) {
+        const uint64_t t = (uint64_t)w * v + carry;
+        w = (uint32_t)t;
+        carry = t >> 32;
+    }
+    if (carry) x.push_back((uint32_t)carry);
+}
+
+Limbs limbs_mul(const Limbs &a, const Limbs &b)
+{
+    Limbs z(a.size() + b.size(), 0);
+    for (size_t i = 0; i < a.size(); ++i) {
+        uint64_t carry = 0;
+        for (size_t j = 0; j < b.size(); ++j) {
+            const uint64_t t = (uint64_t)a[i] * b[j] + z[i + j] + carry;
+            z[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        z[i + b.size()] = (uint32_t)carry;
+    }
+    limbs_trim(z);
+    return z;
+}
+
+int limbs_cmp(const Limbs &a, const Limbs &b)
+{
+    const size_t n = a.size() > b.size() ? a.size() : b.size();
+    for (size_t i = n; i-- > 0;) {
+        const uint32_t x = i < a.size() ? a[i] : 0, y = i < b.size() ? b[i] : 0;
+        if (x != y) return x < y ? -1 : 1;
+    }
+    return 0;
+}
+
+int limbs_bits(const Limbs &a)
+{
+    for (size_t i = a.size(); i-- > 0;)
+        if (a[i]) return (int)(i * 32 + 32 - __builtin_clz(a[i]));
+    return 0;
+}
+
+Limbs limbs_mod(Limbs x, const Limbs &P)
+{
+    const int bp = limbs_bits(P);
+    for (int sh = limbs_bits(x) - bp; sh >= 0; --sh) {
+        Limbs t(x.size() + 1, 0);
+        for (size_t i = 0; i < P.size(); ++i) {
+            const size_t w = i + sh / 32;
+            if (w < t.size()) t[w] |= P[i] << (sh % 32);
+            if (sh % 32 && w + 1 < t.size()) t[w + 1] |= P[i] >> (32 - sh % 32);
+        }
+        if (limbs_cmp(x, t) >= 0) {
+            uint64_t borrow = 0;
+            for (size_t i = 0; i < x.size(); ++i) {
+                const uint64_t d = (uint64_t)x[i] - (i < t.size() ? t[i] : 0) - borrow;
+                x[i] = (uint32_t)d;
+                borrow = (d >> 32) & 1;
+            }
+        }
+    }
+    limbs_trim(x);
+    return x;
+}
+
+typedef unsigned __int128 u128;
+
+uint64_t mulmod_u64(uint64_t a, uint64_t b, uint64_t n) { return (uint64_t)((u128)a * b % n); }
+
+uint64_t powmod_u64(uint64_t a, uint64_t e, uint64_t n)
+{
+    uint64_t r = 1 % n;
+    a %= n;
+    while (e) {
+        if (e & 1) r = mulmod_u64(r, a, n);
+        a = mulmod_u64(a, a, n);
+        e >>= 1;
+    }
+    return r;
+}
+
+// Miller-Rabin with the first twelve prime bases: deterministic below 2^64.
+bool is_prime(uint64_t n)
+{
+    static const uint64_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (n < 2) return false;
+    for (uint64_t q : bases) {
+        if (n == q) return true;
+        if (n % q == 0) return false;
+    }
+    uint64_t d = n - 1;
+    int s = 0;
+    while (!(d & 1)) {
+        d >>= 1;
+        ++s;
+    }
+    for (uint64_t a : bases) {
+        uint64_t x = powmod_u64(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool composite = true;
+        for (int r = 1; r < s && composite; ++r) {
+            x = mulmod_u64(x, x, n);
+            if (x == n - 1) composite = false;
+        }
+        if (composite) return false;
+    }
+    return true;
+}
+
+// random-fftprime.h:107-133 generatePrimes with compute_max_m (:161-166).
+bool generate_primes(uint64_t pbound, const Limbs &prodbound, unsigned kmin, std::vector<uint32_t> &out)
+{
+    out.clear();
+    Limbs prod{1};
+    if (!kmin) ++kmin;
+    const int bits = 64 - __builtin_clzll(pbound - 2);
+    for (int k = bits - 1; k > (int)kmin; --k) {
+        const uint64_t B = pbound - 1;
+        int64_t m = (B & ((1ull << k) - 1)) ? (int64_t)(B >> k) : (int64_t)(B >> k) - 1;
+        if (!(m & 1)) --m;
+        for (; m > 0; m -= 2) {
+            const uint64_t p = ((uint64_t)m << k) + 1;
+            if (is_prime(p)) {
+                if (out.size() >= (size_t)kMaxPrimes) return false;
+                out.push_back((uint32_t)p);
+                limbs_mul_small(prod, (uint32_t)p);
+                if (limbs_cmp(prod, prodbound) > 0) return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Device constant block (32-bit words):
+//   [0..r)         primes p_i
+//   [r..2r)        floor(2^32/p_i)            (Shoup constant of 1)
+//   [2r..3r)       2^32 mod p_i
+//   [3r..4r)       Shoup constant of 2^32 mod p_i
+//   [4r..4r+2r^2)  Garner inverses (p_j mod p_i)^-1 and their Shoup constants, i*r+j
+//   then r*nlimbs  prod_{j<i} p_j mod P, then nlimbs words of P.
+struct MpLayout {
+    int r, nl;
+    __host__ __device__ int primes() const { return 0; }
+    __host__ __device__ int onep() const { return r; }
+    __host__ __device__ int r32() const { return 2 * r; }
+    __host__ __device__ int r32p() const { return 3 * r; }
+    __host__ __device__ int inv() const { return 4 * r; }
+    __host__ __device__ int mi() const { return 4 * r + 2 * r * r; }
+    __host__ __device__ int P() const { return mi() + r * nl; }
+    __host__ __device__ int words() const { return P() + nl; }
+};
+
+// Residues of nlimbs-word coefficients modulo every prime:
+// out[i][e*size + t] for e < entries, t < size.
+__global__ void __launch_bounds__(256) mp_residue_kernel(DevView in, int nl, size_t entries, size_t size,
+                                                         const uint32_t *__restrict__ K, int r,
+                                                         uint32_t *__restrict__ out)
+{
+    const size_t total = entries * size;
+    const MpLayout lay{r, nl};
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        const size_t e = idx / size, t = idx - e * size;
+        const uint32_t *src = in.base + view_index(in, e, t) * nl;
+        uint32_t limb[kMaxLimbs];
+        for (int l = 0; l < nl; ++l) limb[l] = src[l];
+        for (int i = 0; i < r; ++i) {
+            const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
+            const uint32_t c = K[lay.r32() + i], cp = K[lay.r32p() + i];
+            uint32_t v = 0;
+            for (int l = nl - 1; l >= 0; --l)
+                v = add_mod(shoup_mul(v, c, cp, p), shoup_mul(limb[l], 1u, onep, p), p);
+            out[(size_t)i * total + idx] = v;
+        }
+    }
+}
+
+// Garner recombination of r residues and reduction mod P.
+__global__ void __launch_bounds__(128) mp_crt_kernel(const uint32_t *__restrict__ res, size_t entries, size_t size,
+                                                     const uint32_t *__restrict__ K, int r, int nl, DevView out)
+{
+    const size_t total = entries * size;
+    const MpLayout lay{r, nl};
+    for (size_t idx = (size_t)blockIdx.x * 128 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 128) {
+        uint32_t v[kMaxPrimes];
+        uint32_t acc[kMaxLimbs + 2];
+        for (int l = 0; l < nl + 2; ++l) acc[l] = 0;
+        for (int i = 0; i < r; ++i) {
+            const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
+            uint32_t t = res[(size_t)i * total + idx];
+            for (int j = 0; j < i; ++j) {
+                const uint32_t vj = shoup_mul(v[j], 1u, onep, p);
+                t = shoup_mul(sub_mod(t, vj, p), K[lay.inv() + 2 * (i * r + j)], K[lay.inv() + 2 * (i * r + j) + 1], p);
+            }
+            v[i] = t;
+            // acc += v_i * (prod_{j<i} p_j mod P)
+            uint64_t carry = 0;
+            const uint32_t *mi = K + lay.mi() + i * nl;
+            for (int l = 0; l < nl; ++l) {
+                const uint64_t s = (uint64_t)mi[l] * t + acc[l] + carry;
+                acc[l] = (uint32_t)s;
+                carry = s >> 32;
+            }
+            for (int l = nl; l < nl + 2; ++l) {
+                const uint64_t s = (uint64_t)acc[l] + carry;
+                acc[l] = (uint32_t)s;
+                carry = s >> 32;
+            }
+        }
+        // acc < r 2^31 P <= 2^37 P: binary long division by P << sh, sh = 40..0
+        const uint32_t *P = K + lay.P();
+        for (int sh = 40; sh >= 0; --sh) {
+            const int ws = sh >> 5, bs = sh & 31;
+            uint32_t shifted[kMaxLimbs + 2];
+            for (int l = 0; l < nl + 2; ++l) {
+                const int src = l - ws;
+                uint32_t w = 0;
+                if (src >= 0 && src < nl) w = P[src] << bs;
+                if (bs && src - 1 >= 0 && src - 1 < nl) w |= P[src - 1] >> (32 - bs);
+                shifted[l] = w;
+            }
+            // P < 2^(32 nl) so P << sh (sh <= 40) fits the nl+2 words of acc
+            bool ge = true;
+            for (int l = nl + 1; l >= 0; --l) {
+                if (acc[l] != shifted[l]) {
+                    ge = acc[l] > shifted[l];
+                    break;
+                }
+            }
+            if (ge) {
+                uint64_t borrow = 0;
+                for (int l = 0; l < nl + 2; ++l) {
+                    const uint64_t d = (uint64_t)acc[l] - shifted[l] - borrow;
+                    acc[l] = (uint32_t)d;
+                    borrow = (d >> 32) & 1;
+                }
+            }
+        }
+        const size_t e = idx / size, t = idx - e * size;
+        uint32_t *dst = out.base + view_index(out, e, t) * nl;
+        for (int l = 0; l < nl; ++l) dst[l] = acc[l];
+    }
+}
+
+unsigned grid_of(size_t total, unsigned block)
+{
+    const size_t g = (total + block - 1) / block;
+    return (unsigned)(g < 8192 ? (g ? g : 1) : 8192);
+}
+
+lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t minsz, unsigned lpts, MpPlan **out)
+{
+    std::string key((const char *)Pw, (size_t)nl * 4);
+    key += "|" + std::to_string(k) + "|" + std::to_string(minsz) + "|" + std::to_string(lpts);
+    auto it = ctx->mp_plans.find(key);
+    if (it != ctx->mp_plans.end()) {
+        *out = it->second;
+        return LBX_OK;
+    }
+    Limbs P(Pw, Pw + nl);
+    limbs_trim(P);
+    Limbs bound = limbs_mul(P, P);  // :616 bound = 2 P^2 k min(sa,sb)
+    limbs_mul_small(bound, 2);
+    limbs_mul_small(bound, (uint32_t)k);
+    limbs_mul_small(bound, (uint32_t)minsz);
+    std::vector<uint32_t> primes;
+    if (!generate_primes((uint64_t)1 << 31, bound, lpts, primes)) return LBX_ERR_NOT_ENOUGH_PRIMES;
+    const int r = (int)primes.size();
+    const MpLayout lay{r, nl};
+    std::vector<uint32_t> K((size_t)lay.words(), 0);
+    Limbs M{1};
+    for (int i = 0; i < r; ++i) {
+        const uint32_t p = primes[i];
+        K[lay.primes() + i] = p;
+        K[lay.onep() + i] = shoup_pre(1, p);
+        const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
+        K[lay.r32() + i] = r32;
+        K[lay.r32p() + i] = shoup_pre(r32, p);
+        for (int j = 0; j < i; ++j) {
+            const uint32_t inv = invmod_h(primes[j] % p, p);
+            K[lay.inv() + 2 * (i * r + j)] = inv;
+            K[lay.inv() + 2 * (i * r + j) + 1] = shoup_pre(inv, p);
+        }
+        const Limbs mi = limbs_mod(M, P);
+        for (int l = 0; l < nl; ++l) K[lay.mi() + i * nl + l] = l < (int)mi.size() ? mi[l] : 0;
+        limbs_mul_small(M, p);
+        M = limbs_mod(M, P);
+    }
+    for (int l = 0; l < nl; ++l) K[lay.P() + l] = Pw[l];
+    MpPlan *plan = new MpPlan();
+    plan->primes = primes;
+    plan->lpts = lpts;
+    plan->nlimbs = nl;
+    hipError_t e = hipMalloc(&plan->d_const, K.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(plan->d_const, K.data(), K.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (plan->d_const) (void)hipFree(plan->d_const);
+        delete plan;
+        return from_hip(e);
+    }
+    ctx->mp_plans[key] = plan;
+    *out = plan;
+    return LBX_OK;
+}
+
+}  // namespace
+
+void mp_release(lbx_ctx *ctx)
+{
+    for (auto &kv : ctx->mp_plans) {
+        if (kv.second->d_const) (void)hipFree(kv.second->d_const);
+        delete kv.second;
+    }
+    ctx->mp_plans.clear();
+}
+
+static lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a, const DevView &b,
+                             const DevView &c, size_t m, size_t k, size_t n, int method, hipStream_t s)
+{
+    const size_t sa = a.size, sb = b.size, sc = sa + sb - 1;
+    unsigned lpts = 0;
+    size_t pts = 1;
+    while (pts < sc) {  // mul_crtla :151-154
+        pts <<= 1;
+        ++lpts;
+    }
+    MpPlan *plan;
+    LBX_TRY(get_plan(ctx, Pw, nl, k, sa < sb ? sa : sb, lpts, &plan));
+    const int r = (int)plan->primes.size();
+    void *ra, *rb, *rc;
+    LBX_TRY(ensure(ctx, "mp_ra", (size_t)r * m * k * sa * 4, &ra));
+    LBX_TRY(ensure(ctx, "mp_rb", (size_t)r * k * n * sb * 4, &rb));
+    LBX_TRY(ensure(ctx, "mp_rc", (size_t)r * m * n * sc * 4, &rc));
+    hipLaunchKernelGGL(mp_residue_kernel, dim3(grid_of(m * k * sa, 256)), dim3(256), 0, s, a, nl, m * k, sa,
+                       plan->d_const, r, (uint32_t *)ra);
+    LBX_TRY_HIP(hipGetLastError());
+    hipLaunchKernelGGL(mp_residue_kernel, dim3(grid_of(k * n * sb, 256)), dim3(256), 0, s, b, nl, k * n, sb,
+                       plan->d_const, r, (uint32_t *)rb);
+    LBX_TRY_HIP(hipGetLastError());
+    for (int i = 0; i < r; ++i) {
+        DevView va{(uint32_t *)ra + (size_t)i * m * k * sa, LBX_POLFIRST, m, k, sa, 0, sa};
+        DevView vb{(uint32_t *)rb + (size_t)i * k * n * sb, LBX_POLFIRST, k, n, sb, 0, sb};
+        DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
+        if (sc == 1)
+            LBX_TRY(constant_product(ctx, plan->primes[i], va, vb, vc, m, k, n, s));
+        else
+            LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
+    }
+    hipLaunchKernelGGL(mp_crt_kernel, dim3(grid_of(m * n * sc, 128)), dim3(128), 0, s, (const uint32_t *)rc, m * n,
+                       sc, plan->d_const, r, nl, c);
+    LBX_TRY_HIP(hipGetLastError());
+    return LBX_OK;
+}
+
 }  // namespace lbx
+
+using namespace lbx;
 
 extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nlimbs, lbx_polmat *c,
                                          const lbx_polmat *a, const lbx_polmat *b, int method, void *stream)
 {
-    (void)ctx; (void)P; (void)nlimbs; (void)c; (void)a; (void)b; (void)method; (void)stream;
-    return LBX_ERR_UNSUPPORTED;
+    if (!ctx || !P || !a || !b || !c) return LBX_ERR_INVALID_ARG;
+    if (nlimbs < 1 || nlimbs > kMaxLimbs) return LBX_ERR_PRIME_TOO_LARGE;
+    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+    LBX_TRY(check_view(a));
+    LBX_TRY(check_view(b));
+    LBX_TRY(check_view(c));
+    bool nonzero = false, big = false;
+    for (int l = 0; l < nlimbs; ++l) {
+        nonzero |= P[l] != 0;
+        if (l > 0) big |= P[l] != 0;
+    }
+    if (!nonzero || (!big && P[0] < 2)) return LBX_ERR_INVALID_ARG;
+    if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
+    if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
+    const size_t m = a->rows, k = a->cols, n = b->cols, sc = a->size + b->size - 1;
+    if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
+    if (k > 0xffffffffu || a->size > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t wb = 4 * (size_t)nlimbs;
+    Staged sa_, sb_, sc_;
+    lbx_status st = stage_in(a, wb, true, &sa_);
+    if (st == LBX_OK) st = stage_in(b, wb, true, &sb_);
+    if (st == LBX_OK) st = stage_in(c, wb, true, &sc_);
+    const bool host_io = sa_.owned || sb_.owned || sc_.owned;
+    if (st == LBX_OK)
+        st = mp_product(ctx, P, nlimbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k, n,
+                        method, s);
+    if (st == LBX_OK && sc_.owned) {
+        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * wb, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        st = from_hip(e);
+    }
+    if (host_io) (void)hipStreamSynchronize(s);
+    release(sa_);
+    release(sb_);
+    release(sc_);
+    if (st == LBX_OK) c->size = sc;
+    return st;
 }
 
 extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream)
 {
-    (void)ctx; (void)jobs; (void)njobs; (void)method; (void)stream;
+    (void)ctx;
+    (void)jobs;
+    (void)njobs;
+    (void)method;
+    (void)stream;
     return LBX_ERR_UNSUPPORTED;
 }

@@ -1,0 +1,186 @@
+// C++ counterpart of the reference's hot-path tests, written against the
+// header-only mirror include/lbx/polymat.hpp:
+//  * tests/test-fft.C:157-196 -- FFT_direct / FFT_inverse against Horner
+//    evaluation at powers of the root (bit-reversed orders);
+//  * tests/test-matpoly-mult.C:77-88,171-244 -- mul on 16x16 polynomial
+//    matrices of size 512, checked at x = 1 (check_mul, matpoly-mult-fft.h:122-163)
+//    and at random points (a check a coefficient permutation cannot pass);
+//  * tests/test-polynomial-matrix.C:224-366 -- the same product in every
+//    layout and through views gives the same result;
+//  * error behaviour: dimension mismatch and enlarging a view throw.
+// Exit status 0 and "PASS" on success.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "lbx/polymat.hpp"
+
+using namespace lbx;
+
+static int failures = 0;
+#define EXPECT(cond, msg)                                  \
+    do {                                                   \
+        if (!(cond)) {                                     \
+            std::printf("FAIL: %s (line %d)\n", msg, __LINE__); \
+            ++failures;                                    \
+        }                                                  \
+    } while (0)
+
+
+static size_t bitrev(size_t i, size_t n)
+{
+    size_t r = 0;
+    for (size_t j = 0; j < n; ++j, i >>= 1) r = (r << 1) | (i & 1);
+    return r;
+}
+
+static void test_fft(uint32_t p, std::mt19937_64 &g)
+{
+    Modular F(p);
+    for (size_t k = 1; k <= 10; ++k) {
+        const size_t n = (size_t)1 << k;
+        FFT fft(F, k);
+        const uint64_t w = fft.root();
+        std::vector<uint32_t> in(n), in_br(n), out(n), out_br(n);
+        for (auto &v : in) v = (uint32_t)(g() % p);
+        uint64_t x = 1;
+        for (size_t i = 0; i < n; ++i, x = x * w % p) {
+            uint64_t acc = 0;
+            for (size_t t = n; t-- > 0;) acc = (acc * x + in[t]) % p;
+            out[i] = (uint32_t)acc;
+            out_br[bitrev(i, k)] = out[i];
+            in_br[bitrev(i, k)] = in[i];
+        }
+        std::vector<uint32_t> y(in);
+        fft.FFT_direct(y.data());
+        EXPECT(y == out_br, "FFT_direct vs Horner");
+        y = in_br;
+        fft.FFT_inverse(y.data());
+        EXPECT(y == out, "FFT_inverse (DIT with the same root) vs Horner");
+        FFT inv(F, k, fft.invroot());
+        y = in;
+        fft.FFT_direct(y.data());
+        inv.FFT_inverse(y.data());
+        bool ok = true;
+        for (size_t i = 0; i < n; ++i) ok &= y[i] == (uint64_t)in[i] * n % p;
+        EXPECT(ok, "inverse(direct(x)) == n x");
+    }
+}
+
+template <class M>
+static std::vector<uint64_t> eval(const M &A, uint64_t x, uint64_t p)
+{
+    std::vector<uint64_t> r(A.rowdim() * A.coldim());
+    for (size_t e = 0; e < r.size(); ++e) {
+        uint64_t acc = 0;
+        for (size_t t = A.size(); t-- > 0;) acc = (acc * x + A.get(e, t)) % p;
+        r[e] = acc;
+    }
+    return r;
+}
+
+template <class MC, class MA, class MB>
+static bool check_at(const MC &C, const MA &A, const MB &B, uint64_t x, uint64_t p)
+{
+    const size_t m = A.rowdim(), k = A.coldim(), n = B.coldim();
+    auto a = eval(A, x, p), b = eval(B, x, p), c = eval(C, x, p);
+    for (size_t i = 0; i < m; ++i)
+        for (size_t j = 0; j < n; ++j) {
+            uint64_t acc = 0;
+            for (size_t q = 0; q < k; ++q) acc = (acc + a[i * k + q] * b[q * n + j]) % p;
+            if (acc != c[i * n + j]) return false;
+        }
+    return true;
+}
+
+template <PMType T>
+static void fill(PolynomialMatrix<T> &A, uint32_t p, std::mt19937_64 &g)
+{
+    for (size_t e = 0; e < A.rowdim() * A.coldim(); ++e)
+        for (size_t t = 0; t < A.size(); ++t) A.ref(e, t) = (uint32_t)(g() % p);
+}
+
+template <PMType TA, PMType TB, PMType TC>
+static void test_mul(uint32_t p, size_t n, size_t d, int method, std::mt19937_64 &g)
+{
+    Modular F(p);
+    PolynomialMatrix<TA> A(F, n, n, d);
+    PolynomialMatrix<TB> B(F, n, n, d);
+    PolynomialMatrix<TC> C(F, n, n, 1);
+    fill(A, p, g);
+    fill(B, p, g);
+    PolynomialMatrixMulDomain PMD(F, nullptr, method);
+    PMD.mul(C, A, B);
+    EXPECT(C.size() == 2 * d - 1, "c resized to deg+1");
+    EXPECT(check_at(C, A, B, 1, p), "check_mul at x=1");
+    EXPECT(check_at(C, A, B, g() % p, p), "check at a random point");
+    EXPECT(check_at(C, A, B, g() % p, p), "check at a random point");
+    // the same product in polfirst must agree coefficient by coefficient
+    PolynomialMatrix<polfirst> A2(F, n, n, d), B2(F, n, n, d), C2(F, n, n, 1);
+    for (size_t e = 0; e < n * n; ++e)
+        for (size_t t = 0; t < d; ++t) {
+            A2.ref(e, t) = A.get(e, t);
+            B2.ref(e, t) = B.get(e, t);
+        }
+    PMD.mul(C2, A2, B2);
+    bool same = true;
+    for (size_t e = 0; e < n * n; ++e)
+        for (size_t t = 0; t < C.size(); ++t) same &= C.get(e, t) == C2.get(e, t);
+    EXPECT(same, "layouts give the same product");
+}
+
+static void test_views_and_errors(uint32_t p, std::mt19937_64 &g)
+{
+    Modular F(p);
+    PolynomialMatrix<matfirst> A(F, 3, 4, 50);
+    PolynomialMatrix<polfirst> B(F, 4, 2, 40);
+    fill(A, p, g);
+    fill(B, p, g);
+    auto Av = A.at(5, 24);
+    auto Bv = B.at(3, 33);
+    PolynomialMatrix<matrowfirst> Cbig(F, 3, 2, 80);
+    auto Cv = Cbig.at(10, 79);
+    PolynomialMatrixMulDomain PMD(F);
+    PMD.mul(Cv, Av, Bv);
+    EXPECT(Cv.size() == 50, "view resized down to deg+1");
+    EXPECT(check_at(Cv, Av, Bv, g() % p, p), "view product at a random point");
+    bool threw = false;
+    try {
+        auto small = Cbig.at(0, 20);
+        PMD.mul(small, Av, Bv);
+    } catch (const LinboxError &e) {
+        threw = e.status == LBX_ERR_SIZE_MISMATCH;
+    }
+    EXPECT(threw, "enlarging a view throws");
+    threw = false;
+    try {
+        PolynomialMatrix<polfirst> X(F, 2, 3, 4), Y(F, 2, 2, 4), Z(F, 2, 2, 7);
+        PMD.mul(Z, X, Y);
+    } catch (const LinboxError &e) {
+        threw = e.status == LBX_ERR_SIZE_MISMATCH;
+    }
+    EXPECT(threw, "dimension mismatch throws");
+}
+
+int main(int argc, char **argv)
+{
+    const uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 20191;
+    std::mt19937_64 g(seed);
+    const uint32_t primes[4] = {998244353u, 2013265921u, 469762049u, 167772161u};
+    for (uint32_t p : primes) {
+        test_fft(p, g);
+        test_mul<polfirst, polfirst, polfirst>(p, 16, 512, LBX_METHOD_PLAIN_U64, g);
+        test_mul<polfirst, polfirst, polfirst>(p, 16, 512, LBX_METHOD_INT8_MFMA, g);
+        test_mul<matfirst, matrowfirst, polfirst>(p, 16, 100, LBX_METHOD_AUTO, g);
+        test_mul<matrowfirst, matfirst, matfirst>(p, 5, 33, LBX_METHOD_AUTO, g);
+        test_views_and_errors(p, g);
+    }
+    test_mul<polfirst, polfirst, polfirst>(1000000007u, 8, 64, LBX_METHOD_AUTO, g);  // CRT route
+    if (failures) {
+        std::printf("%d failures\n", failures);
+        return 1;
+    }
+    std::printf("PASS\n");
+    return 0;
+}
