@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -185,6 +186,80 @@ int choose_method(int method, size_t m, size_t k, size_t n)
     return (m >= 16 && n >= 16 && k >= 16) ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
 }
 
+// Bytes of head -> tail intermediate per chunk (2 GiB by default: one chunk
+// per operand at the sizes measured so far, where the head of B overlaps the
+// tail of A); the LBX_CHUNK_BYTES environment variable overrides it (tuning
+// and tests).
+static size_t chunk_bytes()
+{
+    const char *e = std::getenv("LBX_CHUNK_BYTES");
+    if (e && *e) {
+        const long long v = std::atoll(e);
+        if (v >= 4096) return (size_t)v;
+    }
+    return (size_t)2 << 30;
+}
+
+// Two-stream chunk pipeline: chunk c's producer runs on the caller's stream s,
+// its consumer on ctx->aux after the producer's event; the producer of chunk
+// c+2 (same scratch slot) waits for the consumer of chunk c.  Events 2c and
+// 2c+1 of the pool mark "produced" and "consumed".
+static lbx_status event_at(lbx_ctx *ctx, size_t i, hipEvent_t *ev)
+{
+    while (ctx->events.size() <= i) {
+        hipEvent_t e;
+        LBX_TRY_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->events.push_back(e);
+    }
+    *ev = ctx->events[i];
+    return LBX_OK;
+}
+
+static lbx_status pipeline_begin(lbx_ctx *ctx, hipStream_t s)
+{
+    if (!ctx->aux) LBX_TRY_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    hipEvent_t ev;
+    LBX_TRY(event_at(ctx, 0, &ev));  // fork: aux starts after everything queued on s
+    LBX_TRY_HIP(hipEventRecord(ev, s));
+    LBX_TRY_HIP(hipStreamWaitEvent(ctx->aux, ev, 0));
+    return LBX_OK;
+}
+
+static lbx_status pipeline_reuse(lbx_ctx *ctx, hipStream_t s, size_t chunk)
+{
+    if (chunk < 2) return LBX_OK;
+    hipEvent_t ev;
+    LBX_TRY(event_at(ctx, 2 + 2 * (chunk - 2) + 1, &ev));
+    LBX_TRY_HIP(hipStreamWaitEvent(s, ev, 0));
+    return LBX_OK;
+}
+
+static lbx_status pipeline_handoff(lbx_ctx *ctx, hipStream_t s, size_t chunk)
+{
+    hipEvent_t ev;
+    LBX_TRY(event_at(ctx, 2 + 2 * chunk, &ev));
+    LBX_TRY_HIP(hipEventRecord(ev, s));
+    LBX_TRY_HIP(hipStreamWaitEvent(ctx->aux, ev, 0));
+    return LBX_OK;
+}
+
+static lbx_status pipeline_done(lbx_ctx *ctx, size_t chunk)
+{
+    hipEvent_t ev;
+    LBX_TRY(event_at(ctx, 2 + 2 * chunk + 1, &ev));
+    LBX_TRY_HIP(hipEventRecord(ev, ctx->aux));
+    return LBX_OK;
+}
+
+static lbx_status pipeline_end(lbx_ctx *ctx, hipStream_t s, size_t nchunks)
+{
+    if (!nchunks) return LBX_OK;
+    hipEvent_t ev;  // join: s continues after the last consumer (aux is in order)
+    LBX_TRY(event_at(ctx, 2 + 2 * (nchunks - 1) + 1, &ev));
+    LBX_TRY_HIP(hipStreamWaitEvent(s, ev, 0));
+    return LBX_OK;
+}
+
 // Forward transforms, pointwise products and inverse transform of one
 // single-prime product on device views.  deg+1 coefficients of c are written.
 lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
@@ -208,22 +283,60 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
 
     Stages st(ctx, s);
     if (method == LBX_METHOD_INT8_MFMA && L >= 8) {
-        // split transforms: the last six forward stages write per-point limb
-        // blocks and the first six inverse stages read the point-major product
+        // Split transforms: the last six forward stages write per-point limb
+        // blocks and the first six inverse stages read the point-major
+        // product.  Entries go in chunks of about 64 MiB of intermediate so the
+        // head -> tail hand-off can stay in the Infinity Cache; heads run on
+        // the caller's stream and tails on a second one, two chunks in flight.
         void *al, *bl, *cmat;
         LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
         LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
         LBX_TRY(ensure(ctx, "cmat", m * n * N * 4, &cmat));
-        LBX_TRY_HIP(launch_ntt_forward_head(a, m * k, L, twf, p, (uint32_t *)ahat, s));
-        LBX_TRY_HIP(launch_ntt_forward_head(b, k * n, L, twf, p, (uint32_t *)bhat, s));
-        LBX_TRY_HIP(launch_dif_tail_limbs((const uint32_t *)ahat, (uint8_t *)al, m, k, true, L, twf, p, s));
-        LBX_TRY_HIP(launch_dif_tail_limbs((const uint32_t *)bhat, (uint8_t *)bl, n, k, false, L, twf, p, s));
+        const size_t target = std::max<size_t>(64, chunk_bytes() / (N * 4));  // entries per chunk
+        const size_t mpad = (m + 31) & ~(size_t)31, npad = (n + 31) & ~(size_t)31;
+        const size_t rpc = std::max<size_t>(1, target / k);  // limb rows per chunk
+        const size_t slot_entries = rpc * k;
+        void *scr;
+        LBX_TRY(ensure(ctx, "scratch", 2 * std::max(slot_entries, target) * N * 4, &scr));
+        uint32_t *slot[2] = {(uint32_t *)scr, (uint32_t *)scr + std::max(slot_entries, target) * N};
+        LBX_TRY(pipeline_begin(ctx, s));
+        size_t chunk = 0;
+        // forward: rows of A (entries i*k+q), then columns of B (entries q*n+j)
+        for (int op = 0; op < 2; ++op) {
+            const size_t rows = op == 0 ? m : n, rpadx = op == 0 ? mpad : npad;
+            for (size_t r0 = 0; r0 < rpadx; r0 += rpc, ++chunk) {
+                const size_t nr = std::min(rpc, rpadx - r0);
+                const size_t valid = r0 < rows ? std::min(nr, rows - r0) : 0;
+                uint32_t *buf = slot[chunk & 1];
+                LBX_TRY(pipeline_reuse(ctx, s, chunk));
+                if (valid) {
+                    const EntryMap map = op == 0 ? EntryMap{r0 * k, valid * k, 0} : EntryMap{r0, valid, n};
+                    LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, twf, p, buf, s));
+                }
+                LBX_TRY(pipeline_handoff(ctx, s, chunk));
+                LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
+                                                  op == 0, L, twf, p, ctx->aux));
+                LBX_TRY(pipeline_done(ctx, chunk));
+            }
+        }
+        LBX_TRY(pipeline_end(ctx, s, chunk));
         st.mark("direct FFT_DIF");
         LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N, p,
                                           inv_n, s));
         st.mark("Pointwise mult");
-        LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, L, twi, p, (uint32_t *)chat, s));
-        LBX_TRY_HIP(launch_ntt_inverse_tail((const uint32_t *)chat, m * n, L, twi, p, c, count, 1, s));
+        LBX_TRY(pipeline_begin(ctx, s));
+        chunk = 0;
+        const size_t cchunk = std::max(slot_entries, target);
+        for (size_t e0 = 0; e0 < m * n; e0 += cchunk, ++chunk) {
+            const size_t ne = std::min(cchunk, m * n - e0);
+            uint32_t *buf = slot[chunk & 1];
+            LBX_TRY(pipeline_reuse(ctx, s, chunk));
+            LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, twi, p, buf, s));
+            LBX_TRY(pipeline_handoff(ctx, s, chunk));
+            LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, twi, p, c, e0, count, 1, ctx->aux));
+            LBX_TRY(pipeline_done(ctx, chunk));
+        }
+        LBX_TRY(pipeline_end(ctx, s, chunk));
         st.mark("inverse FFT_DIT");
         return LBX_OK;
     }
@@ -333,6 +446,8 @@ lbx_status lbx_destroy(lbx_ctx *ctx)
         for (auto &kv : ctx->bufs)
             if (kv.second.ptr) (void)hipFree(kv.second.ptr);
         for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
+        for (auto ev : ctx->events) (void)hipEventDestroy(ev);
+        if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
         mp_release(ctx);
     }
     delete ctx;

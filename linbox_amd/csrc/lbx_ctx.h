@@ -29,6 +29,8 @@ struct lbx_ctx {
     std::map<std::string, lbx::Buffer> bufs;
     std::map<std::tuple<uint32_t, unsigned, uint32_t>, uint2 *> twiddles;
     std::map<std::string, lbx::MpPlan *> mp_plans;
+    hipStream_t aux = nullptr;         // second stream for chunk pipelining (default priority)
+    std::vector<hipEvent_t> events;    // pool of dependency events
     bool profiling = false;
     // recorded (label, start, end) event triples, resolved by lbx_profile_get
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;

@@ -1,0 +1,309 @@
+// Device-side building blocks of the number-theoretic transforms, shared by
+// the per-call kernels (ntt.hip) and the batch scheduler (batch.hip).
+//
+// FFT_direct = DIF, natural order in, bit-reversed order out, and
+// FFT_inverse = DIT, bit-reversed in, natural out (fft.h:65-89), with the
+// butterflies of Butterfly_DIF / Butterfly_DIT (fft-integral.inl:366-558)
+// computed on canonical [0,p) words (see modarith.h).
+//
+// One polynomial of N = 2^L <= 2^15 words stays in LDS for the whole
+// transform.  Each thread owns R = 16 words (32 for L = 15) and applies
+// log2(R) radix-2 stages in registers between two LDS exchanges ("passes"), so
+// a 2^14-point transform is 4+4+4+2 stages with three LDS round trips.  The
+// first pass of the forward transform reads the input with stride N/R (each
+// load instruction covers consecutive words across the wave) and the last
+// pass writes R consecutive words per thread, so global traffic is one
+// coalesced read of the input and one coalesced write of the output.  For
+// N < 256*R several polynomials share a 256-thread group.  Inputs of at
+// most N/2 coefficients (the zero-padded operands of a product) skip the
+// subtraction of the first stage, whose right half is zero.
+#pragma once
+
+#include "lbx_internal.h"
+
+namespace lbx {
+namespace ntt {
+
+
+template <int L>
+struct Cfg {
+    static constexpr int N = 1 << L;
+    static constexpr int RLOG = L <= 3 ? L : (L <= 14 ? 4 : 5);
+    static constexpr int R = 1 << RLOG;
+    static constexpr int TP = N / R;                      // threads per polynomial
+    static constexpr int G = TP >= 256 ? 1 : 256 / TP;    // polynomials per workgroup
+    static constexpr int BLOCK = G * TP;
+    static constexpr int STRIDE = N + (N >> 4);           // padded LDS words per polynomial
+    static constexpr size_t LDS_BYTES = L <= 3 ? 0 : (size_t)G * STRIDE * 4;
+};
+
+// One pad word after every 16: a wave's accesses at word strides 1, 4, 16 or
+// multiples of 64 then fall in distinct banks.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+
+// Element owned by slot (u,q) of a thread in a pass whose butterfly groups have
+// 2^RP elements spaced 2^HL apart inside blocks of 2^(HL+RP) words.
+template <int L, int HL, int RP>
+__device__ __forceinline__ int elem(int tp, int u, int q)
+{
+    constexpr int TP = Cfg<L>::TP;
+    const int g = tp + u * TP;
+    const int blk = g >> HL, off = g & ((1 << HL) - 1);
+    return (blk << (HL + RP)) + off + (q << HL);
+}
+
+template <int L, int HL, int RP>
+__device__ __forceinline__ void scatter(const uint32_t (&x)[Cfg<L>::R], uint32_t *lds, int tp)
+{
+    constexpr int NG = Cfg<L>::R >> RP;
+#pragma unroll
+    for (int u = 0; u < NG; ++u)
+#pragma unroll
+        for (int q = 0; q < (1 << RP); ++q) lds[pad(elem<L, HL, RP>(tp, u, q))] = x[u * (1 << RP) + q];
+}
+
+template <int L, int HL, int RP>
+__device__ __forceinline__ void gather(uint32_t (&x)[Cfg<L>::R], const uint32_t *lds, int tp)
+{
+    constexpr int NG = Cfg<L>::R >> RP;
+#pragma unroll
+    for (int u = 0; u < NG; ++u)
+#pragma unroll
+        for (int q = 0; q < (1 << RP); ++q) x[u * (1 << RP) + q] = lds[pad(elem<L, HL, RP>(tp, u, q))];
+}
+
+// DIF stages S0..S0+RP-1 (span N>>(s+1) at stage s) on the registers.
+template <int L, int S0, int RP, bool HALFZERO>
+__device__ __forceinline__ void dif_pass(uint32_t (&x)[Cfg<L>::R], int tp, const uint2 *__restrict__ tw, uint32_t p)
+{
+    constexpr int N = Cfg<L>::N, TP = Cfg<L>::TP, NG = Cfg<L>::R >> RP;
+    constexpr int HL = L - S0 - RP;
+#pragma unroll
+    for (int u = 0; u < NG; ++u) {
+        const int off = (tp + u * TP) & ((1 << HL) - 1);
+#pragma unroll
+        for (int st = 0; st < RP; ++st) {
+            const int s = S0 + st;
+            const int half = 1 << (RP - 1 - st);
+            const uint2 *tws = tw + (N - (N >> s));
+#pragma unroll
+            for (int q = 0; q < (1 << RP); ++q) {
+                if (q & half) continue;
+                uint32_t &A = x[u * (1 << RP) + q];
+                uint32_t &B = x[u * (1 << RP) + q + half];
+                if (s == L - 1) {  // span 1: the only twiddle is w^0 = 1
+                    const uint32_t a = A, b = B;
+                    A = add_mod(a, b, p);
+                    B = sub_mod(a, b, p);
+                } else {
+                    const uint2 t = tws[off + ((q & (half - 1)) << HL)];
+                    if (HALFZERO && s == 0) {  // right half of the input is zero
+                        B = shoup_mul(A, t.x, t.y, p);
+                    } else {
+                        const uint32_t a = A, b = B;
+                        A = add_mod(a, b, p);
+                        B = shoup_mul(a + (p - b), t.x, t.y, p);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// DIT stages D0..D0+RP-1 (span 2^d at stage d).
+template <int L, int D0, int RP>
+__device__ __forceinline__ void dit_pass(uint32_t (&x)[Cfg<L>::R], int tp, const uint2 *__restrict__ tw, uint32_t p)
+{
+    constexpr int N = Cfg<L>::N, TP = Cfg<L>::TP, NG = Cfg<L>::R >> RP;
+#pragma unroll
+    for (int u = 0; u < NG; ++u) {
+        const int off = (tp + u * TP) & ((1 << D0) - 1);
+#pragma unroll
+        for (int st = 0; st < RP; ++st) {
+            const int d = D0 + st;
+            const int half = 1 << st;
+            const uint2 *tws = tw + (N - (2 << d));
+#pragma unroll
+            for (int q = 0; q < (1 << RP); ++q) {
+                if (q & half) continue;
+                uint32_t &A = x[u * (1 << RP) + q];
+                uint32_t &B = x[u * (1 << RP) + q + half];
+                const uint32_t a = A;
+                uint32_t t = B;
+                if (d != 0) {
+                    const uint2 w = tws[off + ((q & (half - 1)) << D0)];
+                    t = shoup_mul(t, w.x, w.y, p);
+                }
+                A = add_mod(a, t, p);
+                B = sub_mod(a, t, p);
+            }
+        }
+    }
+}
+
+// DIF passes over stages [S0, SEND): full passes first, then the remainder.
+// On entry x holds the mapping of pass S0.
+template <int L, int S0, int SEND, bool HALFZERO>
+__device__ __forceinline__ void dif_passes(uint32_t (&x)[Cfg<L>::R], uint32_t *lds, int tp, const uint2 *tw,
+                                           uint32_t p)
+{
+    constexpr int RLOG = Cfg<L>::RLOG;
+    constexpr int RP = (SEND - S0) < RLOG ? (SEND - S0) : RLOG;
+    dif_pass<L, S0, RP, HALFZERO>(x, tp, tw, p);
+    if constexpr (S0 + RP < SEND) {
+        constexpr int S1 = S0 + RP;
+        constexpr int NRP = (SEND - S1) < RLOG ? (SEND - S1) : RLOG;
+        if constexpr (S0 > 0) __syncthreads();  // previous gather finished everywhere
+        scatter<L, L - S0 - RP, RP>(x, lds, tp);
+        __syncthreads();
+        gather<L, L - S1 - NRP, NRP>(x, lds, tp);
+        dif_passes<L, S1, SEND, false>(x, lds, tp, tw, p);
+    }
+}
+
+// Stages in a pass that starts at D0 and ends at L: the remainder first.
+template <int L, int D0>
+constexpr int dit_rp()
+{
+    return ((L - D0) % Cfg<L>::RLOG) ? ((L - D0) % Cfg<L>::RLOG) : Cfg<L>::RLOG;
+}
+
+// DIT passes over spans 2^D0 .. 2^(L-1): remainder first, then full passes.
+template <int L, int D0>
+__device__ __forceinline__ void dit_passes(uint32_t (&x)[Cfg<L>::R], uint32_t *lds, int tp, const uint2 *tw,
+                                           uint32_t p)
+{
+    constexpr int RP = dit_rp<L, D0>();
+    dit_pass<L, D0, RP>(x, tp, tw, p);
+    if constexpr (D0 + RP < L) {
+        constexpr int D1 = D0 + RP;
+        if constexpr (D0 > 0) __syncthreads();
+        scatter<L, D0, RP>(x, lds, tp);
+        __syncthreads();
+        gather<L, D1, dit_rp<L, D1>()>(x, lds, tp);
+        dit_passes<L, D1>(x, lds, tp, tw, p);
+    }
+}
+
+// Stages in the last DIF pass of [0, SEND).
+template <int L, int SEND>
+constexpr int dif_last_rp()
+{
+    return (SEND % Cfg<L>::RLOG) ? (SEND % Cfg<L>::RLOG) : Cfg<L>::RLOG;
+}
+
+// DIF stages [0, SEND) of one polynomial per TP threads.  With SEND == L the
+// result is the full transform (R consecutive words per thread are stored as
+// 16-byte vectors); with SEND < L the data is left in natural storage order
+// as 2^(L-SEND) independent blocks for the remaining stages.
+template <int L, int SEND, bool HALFZERO>
+__device__ __forceinline__ void dif_body(const DevView &in, size_t entries, const EntryMap &map,
+                                         const uint2 *__restrict__ tw, uint32_t p, uint32_t *__restrict__ out,
+                                         size_t e_base, int tid, uint32_t *lds_all)
+{
+    using C = Cfg<L>;
+    const int pib = tid / C::TP;
+    const int tp = tid % C::TP;
+    const size_t e = e_base + pib;  // output row
+    const bool active = e < entries;
+    const size_t ein = map(e);                           // entry of the input view
+    uint32_t *lds = lds_all + pib * C::STRIDE;
+
+    uint32_t x[C::R];
+    // first pass mapping (words tp + q*TP when the pass is a full radix-R pass)
+    constexpr int RP0 = SEND < C::RLOG ? SEND : C::RLOG;
+#pragma unroll
+    for (int u = 0; u < (C::R >> RP0); ++u)
+#pragma unroll
+        for (int q = 0; q < (1 << RP0); ++q) {
+            const size_t t = (size_t)elem<L, L - RP0, RP0>(tp, u, q);
+            x[u * (1 << RP0) + q] = (active && t < in.size) ? in.base[view_index(in, ein, t)] : 0u;
+        }
+    dif_passes<L, 0, SEND, HALFZERO>(x, lds, tp, tw, p);
+    constexpr int RP = dif_last_rp<L, SEND>();
+    constexpr int NG = C::R >> RP;
+    if (!active) return;
+    uint32_t *o = out + e * C::N;
+    if constexpr (SEND == L) {
+#pragma unroll
+        for (int u = 0; u < NG; ++u) {
+            const int g = tp + u * C::TP;
+            if constexpr (RP >= 2) {
+#pragma unroll
+                for (int q = 0; q < (1 << RP); q += 4)
+                    *reinterpret_cast<uint4 *>(o + (g << RP) + q) =
+                        make_uint4(x[u * (1 << RP) + q], x[u * (1 << RP) + q + 1], x[u * (1 << RP) + q + 2],
+                                   x[u * (1 << RP) + q + 3]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < (1 << RP); ++q) o[(g << RP) + q] = x[u * (1 << RP) + q];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < NG; ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << RP); ++q) o[elem<L, L - SEND, RP>(tp, u, q)] = x[u * (1 << RP) + q];
+    }
+}
+
+// DIT stages [DSTART, L) of one polynomial per TP threads.  DSTART == 0 reads
+// bit-reversed input with 16-byte vector loads; DSTART > 0 continues a
+// transform whose first DSTART stages were applied blockwise.
+template <int L, int DSTART, bool SCALE>
+__device__ __forceinline__ void dit_body(const uint32_t *__restrict__ in, size_t entries,
+                                         const uint2 *__restrict__ tw, uint32_t p, const DevView &out,
+                                         size_t out_first, size_t count, uint32_t scale, uint32_t scale_p,
+                                         size_t e_base, int tid, uint32_t *lds_all)
+{
+    using C = Cfg<L>;
+    const int pib = tid / C::TP;
+    const int tp = tid % C::TP;
+    const size_t e = e_base + pib;
+    const bool active = e < entries;
+    uint32_t *lds = lds_all + pib * C::STRIDE;
+
+    uint32_t x[C::R];
+    constexpr int RP0 = dit_rp<L, DSTART>();
+    constexpr int NG0 = C::R >> RP0;
+    const uint32_t *src = in + (active ? e : 0) * C::N;
+    if constexpr (DSTART == 0 && RP0 >= 2) {
+#pragma unroll
+        for (int u = 0; u < NG0; ++u) {
+            const int g = tp + u * C::TP;
+#pragma unroll
+            for (int q = 0; q < (1 << RP0); q += 4) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(src + (g << RP0) + q);
+                x[u * (1 << RP0) + q] = v.x;
+                x[u * (1 << RP0) + q + 1] = v.y;
+                x[u * (1 << RP0) + q + 2] = v.z;
+                x[u * (1 << RP0) + q + 3] = v.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < NG0; ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << RP0); ++q) x[u * (1 << RP0) + q] = src[elem<L, DSTART, RP0>(tp, u, q)];
+    }
+    dit_passes<L, DSTART>(x, lds, tp, tw, p);
+    // last pass: a full radix-R pass leaves the thread words tp + q*TP; a
+    // transform shorter than one pass leaves groups of 2^(L-DSTART)
+    constexpr int RPL = (L - DSTART) >= C::RLOG ? C::RLOG : (L - DSTART);
+    if (active) {
+#pragma unroll
+        for (int u = 0; u < (C::R >> RPL); ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << RPL); ++q) {
+                const size_t t = (size_t)elem<L, L - RPL, RPL>(tp, u, q);
+                if (t < count) {
+                    uint32_t v = x[u * (1 << RPL) + q];
+                    if (SCALE) v = shoup_mul(v, scale, scale_p, p);
+                    out.base[view_index(out, out_first + e, t)] = v;
+                }
+            }
+    }
+}
+
+}  // namespace ntt
+}  // namespace lbx

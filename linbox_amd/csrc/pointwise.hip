@@ -267,6 +267,27 @@ __device__ __forceinline__ void mfma_tile_lds(const uint4 *__restrict__ lA, cons
     }
 }
 
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) const void global_cvoid;
+
+// Global -> LDS copy of `chunks` 16-byte chunks (a multiple of 256) with the
+// row swizzle, using global_load_lds_dwordx4: each wave instruction fills 64
+// consecutive LDS chunks (1 KiB at M0 + 16*lane) with no register staging, so
+// every load of the copy is in flight at once.  Lane i of a wave's j-th
+// instruction fills LDS chunk P = 64*(4j + wave) + i, i.e. row r = P / cpr,
+// swizzled column P % cpr, which holds global chunk r*cpr + ((P % cpr) ^ (r & 15)).
+__device__ __forceinline__ void copy_lds(const uint4 *__restrict__ g, uint4 *l, int chunks, int cpr, int wave,
+                                         int lane)
+{
+    for (int j = 0; j * 256 < chunks; ++j) {
+        const int base = 64 * (4 * j + wave);
+        const int P = base + lane;
+        const int r = P / cpr, c = P - r * cpr;
+        __builtin_amdgcn_global_load_lds((global_cvoid *)(g + r * cpr + (c ^ (r & 15))), (lds_void *)(l + base), 16,
+                                         0, 0);
+    }
+}
+
 __global__ void __launch_bounds__(256, 2) pw_mfma_lds_kernel(const uint8_t *__restrict__ Al,
                                                              const uint8_t *__restrict__ Bl, uint32_t *__restrict__ Cmat,
                                                              int m, int n, int kchunks, int mpad, int npad,
@@ -283,18 +304,13 @@ __global__ void __launch_bounds__(256, 2) pw_mfma_lds_kernel(const uint8_t *__re
         const uint4 *gA = reinterpret_cast<const uint4 *>(Al) + t * (size_t)a_chunks;
         const uint4 *gB = reinterpret_cast<const uint4 *>(Bl) + t * (size_t)npad * cpr;
         __syncthreads();  // the previous point is done with lA and lB
-        for (int i = tid; i < a_chunks; i += 256) {
-            const int r = i / cpr, c = i - r * cpr;
-            lA[r * cpr + (c ^ (r & 15))] = gA[i];
-        }
+        copy_lds(gA, lA, a_chunks, cpr, wave, lane);
         uint32_t *Ct = Cmat + t * (size_t)m * n;
         for (int cb = 0; cb < ncb; ++cb) {
             if (cb) __syncthreads();
             const uint4 *gBb = gB + (size_t)cb * b_chunks;
-            for (int i = tid; i < b_chunks; i += 256) {
-                const int r = i / cpr, c = i - r * cpr;
-                lB[r * cpr + (c ^ (r & 15))] = gBb[i];
-            }
+            copy_lds(gBb, lB, b_chunks, cpr, wave, lane);
+            __builtin_amdgcn_s_waitcnt(0);  // this thread's LDS-direct loads have landed
             __syncthreads();
             for (int rt = wave; rt < nrt; rt += 4) {
                 v4i acc[2][2][7];
