@@ -92,6 +92,73 @@ def run_gpu(args):
     return ms_int8, wall_int8, stages_int8, ms_plain, stages_plain
 
 
+def run_batch(steps=5):
+    """Configuration 5: 1024 independent products (m,k,n in {2,...,64}, degree log-uniform in
+    [8,4096], prime uniform from four FFT primes) as one lbx_polymat_mul_batch call
+    (replayed hipGraph of persistent work-queue kernels) vs one lbx_polymat_mul_u32 per product."""
+    import linbox_amd as lbx
+    from linbox_amd import hip, PolynomialMatrix, PolynomialMatrixMulDomain
+
+    primes = [998244353, 2013265921, 469762049, 167772161]
+    rng = np.random.default_rng(0x4C494E424F58 + 4)
+    dims = [2, 4, 8, 16, 32, 64]
+    ctx = lbx.Context(0)
+    stream = hip.Stream()
+    jobs, keep = [], []
+    for _ in range(1024):
+        m, k, n = (int(rng.choice(dims)) for _ in range(3))
+        d = int(round(np.exp(rng.uniform(np.log(8), np.log(4096)))))
+        p = int(rng.choice(primes))
+        sa = sb = d + 1
+        da = hip.DeviceBuffer.from_numpy(rng.integers(0, p, m * k * sa, dtype=np.uint32))
+        db = hip.DeviceBuffer.from_numpy(rng.integers(0, p, k * n * sb, dtype=np.uint32))
+        dc = hip.DeviceBuffer(m * n * (sa + sb - 1) * 4)
+        keep += [da, db, dc]
+        jobs.append(dict(p=p, m=m, k=k, n=n, sa=sa, sb=sb, a=da.data_ptr(), b=db.data_ptr(), c=dc.data_ptr()))
+
+    def timed(fn, reps):
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream.value)
+        for _ in range(reps):
+            fn()
+        e1.record(stream.value)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    batch = lambda: lbx.mul_batch(jobs, ctx=ctx, stream=stream.value)
+    batch()  # builds the plan, tunes tiles, captures the graph
+    stream.synchronize()
+    ms_batch = timed(batch, steps)
+
+    def singles():
+        for j in jobs:
+            C = PolynomialMatrix(j["m"], j["n"], j["sa"] + j["sb"] - 1, data=_Ptr(j["c"]))
+            PolynomialMatrixMulDomain(j["p"], ctx, lbx.METHOD_AUTO, stream=stream.value).mul(
+                C, PolynomialMatrix(j["m"], j["k"], j["sa"], data=_Ptr(j["a"])),
+                PolynomialMatrix(j["k"], j["n"], j["sb"], data=_Ptr(j["b"])))
+    singles()
+    stream.synchronize()
+    ms_single = timed(singles, 1)
+    ctx.close()
+    print("batch: %.3f ms per 1024 products (graph), %.3f ms with one call per product" % (ms_batch, ms_single),
+          file=sys.stderr, flush=True)
+    return {"workload": "1024 products, m,k,n in {2..64}, degree log-uniform [8,4096], 4 primes",
+            "graph_ms": round(ms_batch, 3), "graph_products_per_s": round(1024 / (ms_batch * 1e-3), 1),
+            "per_product_ms": round(ms_single, 3),
+            "per_product_products_per_s": round(1024 / (ms_single * 1e-3), 1),
+            "speedup": round(ms_single / ms_batch, 2)}
+
+
+class _Ptr:
+    """A device address usable as PolynomialMatrix storage."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def data_ptr(self):
+        return self.ptr
+
+
 def cpu_baseline(budget_s):
     """Oracle (C restatement of mul_fft, one thread) on a bounded sample of the workload.
 
@@ -142,6 +209,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--skip-batch", action="store_true", help="omit the 1024-product batch measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -152,6 +220,7 @@ def main():
         sys.exit(2)
 
     ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
+    batch = None if args.skip_batch else run_batch()
     pts = 1 << 14
     algo = algorithmic_bytes(M, K, N, SA, SB, pts)
     st = dict(stages)
@@ -190,6 +259,7 @@ def main():
         "gmulmod_per_s": round(40265318400 / (ms_int8 * 1e-3) / 1e9, 2),
         "wall_s_timed": round(wall, 4),
         "roofline": roofline,
+        "batch_config5": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

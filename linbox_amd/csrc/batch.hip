@@ -1,0 +1,434 @@
+// Batch scheduler: many independent products of uneven shapes, degrees and
+// primes, the mix that OrderBasis::PM_Basis issues through
+// PolynomialMatrixMulDomain::mul (order-basis.h:219-326 -> :297).
+//
+// All forward transforms, all pointwise products and all inverse transforms
+// of a batch run as persistent work-queue kernels fed from device item lists:
+// one transform kernel per length 2^L present (launched as parallel graph
+// branches), one pointwise kernel for every job.  The whole sequence is
+// captured once per batch shape into a hipGraph and replayed; a call only
+// uploads the job descriptors (the caller's pointers) and launches the graph.
+// The pointwise register tile of each job is picked by an autotuner that times
+// the candidate tiles on the first batch of each size class.
+// Jobs the batch kernels do not cover (transform length outside [2^4, 2^13],
+// no 2^L-th root of unity mod p, p >= 2^31, constant matrices) run through the
+// single-product routes after the graph, on the same stream.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbx_ctx.h"
+
+namespace lbx {
+
+struct BatchPlan {
+    std::vector<BatchJob> jobs;            // host copy: workspace slices, twiddles, tiles
+    BatchJob *jobs_d = nullptr;
+    NttItem *ntt_d = nullptr;              // forward items grouped by L, then inverse items grouped by L
+    PwItem *pw_d = nullptr;
+    uint32_t *counters = nullptr;          // one per kernel
+    void *ws = nullptr;                    // transform workspace of every job
+    std::vector<unsigned> Ls;              // lengths present
+    std::vector<uint32_t> fwd_off, fwd_cnt, inv_off, inv_cnt;
+    uint32_t npw = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+namespace {
+
+int size_class(const lbx_job &j)
+{
+    const size_t d = std::max(j.m, j.n);
+    return d <= 4 ? 0 : (d <= 16 ? 1 : 2);
+}
+
+unsigned tile_dim(uint32_t tile) { return 2u << tile; }  // 0 -> 2, 1 -> 4, 2 -> 8
+
+unsigned log2_len(size_t deg)
+{
+    unsigned L = 0;
+    size_t pts = 1;
+    while (pts <= deg) {
+        pts <<= 1;
+        ++L;
+    }
+    return L;
+}
+
+bool batchable(const lbx_job &j, unsigned *Lout)
+{
+    if (!j.a || !j.b || !j.c || !j.m || !j.k || !j.n || !j.sa || !j.sb) return false;
+    if (j.sa + j.sb <= 2 || j.p >= (1u << 31) || j.p < 3) return false;
+    const unsigned L = log2_len(j.sa + j.sb - 2);
+    if (L < (unsigned)kBatchMinL || L > (unsigned)kBatchMaxL) return false;
+    if ((j.p - 1) % ((uint32_t)1 << L) != 0) return false;
+    if (j.m > 0xffffu || j.k > 0xffffu || j.n > 0xffffu) return false;
+    *Lout = L;
+    return true;
+}
+
+int cus()
+{
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                     hipSuccess || n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
+void pw_items_for(const BatchJob &J, uint32_t job, std::vector<PwItem> &out)
+{
+    const unsigned T = tile_dim(J.tile);
+    const uint32_t ti = (J.m + T - 1) / T, tj = (J.n + T - 1) / T, blocks = (((uint32_t)1 << J.L) + 255) / 256;
+    for (uint32_t b = 0; b < blocks; ++b)
+        for (uint32_t i = 0; i < ti; ++i)
+            for (uint32_t jj = 0; jj < tj; ++jj) out.push_back(PwItem{job, i, jj, b * 256});
+}
+
+lbx_status upload(void **dst, const void *src, size_t bytes, hipStream_t s)
+{
+    LBX_TRY_HIP(hipMalloc(dst, bytes ? bytes : 16));
+    if (bytes) LBX_TRY_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
+    return LBX_OK;
+}
+
+// Times the pointwise kernel of the jobs of each size class with each
+// candidate tile (the workspace contents do not affect the time) and keeps
+// the fastest per class.  LBX_BATCH_TILE=0|1|2 forces a tile instead.
+lbx_status tune_tiles(lbx_ctx *ctx, BatchPlan *plan, const std::vector<int> &cls, hipStream_t s)
+{
+    const char *force = std::getenv("LBX_BATCH_TILE");
+    if (force && *force >= '0' && *force <= '2') {
+        for (auto &J : plan->jobs) J.tile = (uint32_t)(*force - '0');
+        return LBX_OK;
+    }
+    static const std::vector<uint32_t> cand[3] = {{0, 1}, {0, 1, 2}, {1, 2}};
+    for (int c = 0; c < 3; ++c) {
+        std::vector<uint32_t> members;
+        for (uint32_t j = 0; j < plan->jobs.size(); ++j)
+            if (cls[j] == c) members.push_back(j);
+        if (members.empty()) continue;
+        if (ctx->tile_tuned[c] < 0) {
+            float best = 1e30f;
+            uint32_t best_tile = cand[c][0];
+            for (uint32_t t : cand[c]) {
+                std::vector<BatchJob> jobs = plan->jobs;
+                for (uint32_t j : members) jobs[j].tile = t;
+                std::vector<PwItem> items;
+                for (uint32_t j : members) pw_items_for(jobs[j], j, items);
+                void *jd = nullptr, *id = nullptr, *cnt = nullptr;
+                lbx_status st = upload(&jd, jobs.data(), jobs.size() * sizeof(BatchJob), s);
+                if (st == LBX_OK) st = upload(&id, items.data(), items.size() * sizeof(PwItem), s);
+                if (st == LBX_OK) st = from_hip(hipMalloc(&cnt, 16));
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (st == LBX_OK) st = from_hip(hipEventCreate(&e0));
+                if (st == LBX_OK) st = from_hip(hipEventCreate(&e1));
+                const unsigned grid = (unsigned)std::min<size_t>(items.size(), (size_t)cus() * 8);
+                float ms = 0;
+                for (int rep = 0; rep < 2 && st == LBX_OK; ++rep) {  // first run warms up
+                    st = from_hip(launch_zero_counters((uint32_t *)cnt, 1, s));
+                    if (st == LBX_OK) st = from_hip(hipEventRecord(e0, s));
+                    if (st == LBX_OK)
+                        st = from_hip(launch_batch_pointwise((const BatchJob *)jd, (const PwItem *)id,
+                                                             (uint32_t)items.size(), (uint32_t *)cnt, grid, s));
+                    if (st == LBX_OK) st = from_hip(hipEventRecord(e1, s));
+                    if (st == LBX_OK) st = from_hip(hipEventSynchronize(e1));
+                    if (st == LBX_OK) st = from_hip(hipEventElapsedTime(&ms, e0, e1));
+                }
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                (void)hipStreamSynchronize(s);
+                if (jd) (void)hipFree(jd);
+                if (id) (void)hipFree(id);
+                if (cnt) (void)hipFree(cnt);
+                if (st != LBX_OK) return st;
+                if (ms < best) {
+                    best = ms;
+                    best_tile = t;
+                }
+            }
+            ctx->tile_tuned[c] = (int)best_tile;
+        }
+        for (uint32_t j : members) plan->jobs[j].tile = (uint32_t)ctx->tile_tuned[c];
+    }
+    return LBX_OK;
+}
+
+// Capture: zero counters; forward kernels of every length as parallel
+// branches; pointwise; inverse kernels as parallel branches.
+lbx_status capture(lbx_ctx *ctx, BatchPlan *plan)
+{
+    if (!ctx->aux) LBX_TRY_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    const size_t nb = plan->Ls.size();
+    std::vector<hipStream_t> br(nb, nullptr);
+    std::vector<hipEvent_t> evs;
+    auto new_event = [&](hipEvent_t *e) -> lbx_status {
+        LBX_TRY_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        evs.push_back(*e);
+        return LBX_OK;
+    };
+    lbx_status st = LBX_OK;
+    for (size_t i = 0; i < nb && st == LBX_OK; ++i) st = from_hip(hipStreamCreateWithFlags(&br[i], hipStreamNonBlocking));
+    hipStream_t cs = ctx->aux;
+    bool capturing = false;
+    if (st == LBX_OK) {
+        st = from_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        capturing = st == LBX_OK;
+    }
+    const uint32_t nctr = (uint32_t)(2 * nb + 1);
+    if (st == LBX_OK) st = from_hip(launch_zero_counters(plan->counters, (int)nctr, cs));
+    for (int phase = 0; phase < 3 && st == LBX_OK; ++phase) {
+        if (phase == 1) {
+            const unsigned grid = (unsigned)std::min<size_t>(plan->npw, (size_t)cus() * 8);
+            if (plan->npw)
+                st = from_hip(launch_batch_pointwise(plan->jobs_d, plan->pw_d, plan->npw, plan->counters + 2 * nb,
+                                                     grid, cs));
+            continue;
+        }
+        const bool inv = phase == 2;
+        hipEvent_t fork;
+        if ((st = new_event(&fork)) != LBX_OK) break;
+        if ((st = from_hip(hipEventRecord(fork, cs))) != LBX_OK) break;
+        for (size_t i = 0; i < nb && st == LBX_OK; ++i) {
+            const uint32_t off = inv ? plan->inv_off[i] : plan->fwd_off[i];
+            const uint32_t cnt = inv ? plan->inv_cnt[i] : plan->fwd_cnt[i];
+            st = from_hip(hipStreamWaitEvent(br[i], fork, 0));
+            const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(cnt, (size_t)cus() * 4));
+            if (st == LBX_OK && cnt)
+                st = from_hip(launch_batch_ntt(plan->Ls[i], plan->jobs_d, plan->ntt_d + off, cnt,
+                                               plan->counters + (inv ? nb : 0) + i, inv, grid, br[i]));
+            hipEvent_t join;
+            if (st == LBX_OK) st = new_event(&join);
+            if (st == LBX_OK) st = from_hip(hipEventRecord(join, br[i]));
+            if (st == LBX_OK) st = from_hip(hipStreamWaitEvent(cs, join, 0));
+        }
+    }
+    if (capturing) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (st == LBX_OK) st = from_hip(e);
+        if (st == LBX_OK) st = from_hip(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+        plan->graph = g;
+    }
+    for (auto e : evs) (void)hipEventDestroy(e);
+    for (auto s : br)
+        if (s) (void)hipStreamDestroy(s);
+    return st;
+}
+
+lbx_status build_plan(lbx_ctx *ctx, const std::vector<lbx_job> &jobs, const std::vector<unsigned> &Ls,
+                      hipStream_t s, BatchPlan **out)
+{
+    BatchPlan *plan = new BatchPlan();
+    auto fail = [&](lbx_status st) {
+        if (plan->jobs_d) (void)hipFree(plan->jobs_d);
+        if (plan->ntt_d) (void)hipFree(plan->ntt_d);
+        if (plan->pw_d) (void)hipFree(plan->pw_d);
+        if (plan->counters) (void)hipFree(plan->counters);
+        if (plan->ws) (void)hipFree(plan->ws);
+        delete plan;
+        return st;
+    };
+    // workspace slices and descriptors
+    size_t words = 0;
+    std::vector<size_t> off(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        off[j] = words;
+        words += (jobs[j].m * jobs[j].k + jobs[j].k * jobs[j].n + jobs[j].m * jobs[j].n) << Ls[j];
+    }
+    lbx_status st = from_hip(hipMalloc(&plan->ws, words * 4 + 16));
+    if (st != LBX_OK) return fail(st);
+    std::vector<int> cls(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        const lbx_job &x = jobs[j];
+        const unsigned L = Ls[j];
+        BatchJob J;
+        std::memset(&J, 0, sizeof(J));
+        J.p = x.p;
+        J.L = L;
+        J.m = (uint32_t)x.m;
+        J.k = (uint32_t)x.k;
+        J.n = (uint32_t)x.n;
+        J.sa = (uint32_t)x.sa;
+        J.sb = (uint32_t)x.sb;
+        J.sc = (uint32_t)(x.sa + x.sb - 1);
+        J.tile = 1;
+        uint32_t *w = (uint32_t *)plan->ws + off[j];
+        J.ahat = w;
+        J.bhat = w + ((x.m * x.k) << L);
+        J.chat = w + ((x.m * x.k + x.k * x.n) << L);
+        uint32_t root;
+        if ((st = primitive_root(x.p, L, &root)) != LBX_OK) return fail(st);
+        if ((st = twiddles(ctx, x.p, L, root, &J.twf)) != LBX_OK) return fail(st);
+        if ((st = twiddles(ctx, x.p, L, invroot(x.p, L, root), &J.twi)) != LBX_OK) return fail(st);
+        J.pc = plain_const(x.p, invmod_h((uint32_t)(((uint64_t)1 << L) % x.p), x.p));
+        plan->jobs.push_back(J);
+        cls[j] = size_class(x);
+    }
+    if ((st = tune_tiles(ctx, plan, cls, s)) != LBX_OK) return fail(st);
+    // transform items grouped by length, biggest operands first within a length
+    for (unsigned L = kBatchMaxL; L >= (unsigned)kBatchMinL; --L)
+        if (std::find(Ls.begin(), Ls.end(), L) != Ls.end()) plan->Ls.push_back(L);
+    std::vector<NttItem> items;
+    for (int inv = 0; inv < 2; ++inv) {
+        for (unsigned L : plan->Ls) {
+            (inv ? plan->inv_off : plan->fwd_off).push_back((uint32_t)items.size());
+            const uint32_t per = batch_polys_per_item(L);
+            for (uint32_t j = 0; j < jobs.size(); ++j) {
+                if (Ls[j] != L) continue;
+                for (uint32_t op = inv ? 2 : 0; op < (inv ? 3u : 2u); ++op) {
+                    const size_t E = op == 0 ? jobs[j].m * jobs[j].k : (op == 1 ? jobs[j].k * jobs[j].n : jobs[j].m * jobs[j].n);
+                    for (size_t e0 = 0; e0 < E; e0 += per)
+                        items.push_back(NttItem{j, op, (uint32_t)e0, (uint32_t)std::min<size_t>(per, E - e0)});
+                }
+            }
+            (inv ? plan->inv_cnt : plan->fwd_cnt).push_back((uint32_t)items.size() - (inv ? plan->inv_off : plan->fwd_off).back());
+        }
+    }
+    // pointwise items, largest jobs first
+    std::vector<uint32_t> order(jobs.size());
+    for (uint32_t j = 0; j < order.size(); ++j) order[j] = j;
+    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        const double cx = (double)jobs[x].m * jobs[x].k * jobs[x].n * ((size_t)1 << Ls[x]);
+        const double cy = (double)jobs[y].m * jobs[y].k * jobs[y].n * ((size_t)1 << Ls[y]);
+        return cx > cy;
+    });
+    std::vector<PwItem> pw;
+    for (uint32_t j : order) pw_items_for(plan->jobs[j], j, pw);
+    plan->npw = (uint32_t)pw.size();
+    void *p;
+    if ((st = upload(&p, plan->jobs.data(), plan->jobs.size() * sizeof(BatchJob), s)) != LBX_OK) return fail(st);
+    plan->jobs_d = (BatchJob *)p;
+    if ((st = upload(&p, items.data(), items.size() * sizeof(NttItem), s)) != LBX_OK) return fail(st);
+    plan->ntt_d = (NttItem *)p;
+    if ((st = upload(&p, pw.data(), pw.size() * sizeof(PwItem), s)) != LBX_OK) return fail(st);
+    plan->pw_d = (PwItem *)p;
+    if ((st = from_hip(hipMalloc(&p, 4 * (2 * plan->Ls.size() + 1) + 16))) != LBX_OK) return fail(st);
+    plan->counters = (uint32_t *)p;
+    if ((st = from_hip(hipStreamSynchronize(s))) != LBX_OK) return fail(st);
+    if ((st = capture(ctx, plan)) != LBX_OK) {
+        if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+        if (plan->graph) (void)hipGraphDestroy(plan->graph);
+        return fail(st);
+    }
+    *out = plan;
+    return LBX_OK;
+}
+
+std::string signature(const std::vector<lbx_job> &jobs)
+{
+    std::string sig;
+    sig.reserve(jobs.size() * 24);
+    for (const auto &j : jobs) {
+        const uint32_t v[6] = {j.p, (uint32_t)j.m, (uint32_t)j.k, (uint32_t)j.n, (uint32_t)j.sa, (uint32_t)j.sb};
+        sig.append((const char *)v, sizeof(v));
+    }
+    return sig;
+}
+
+}  // namespace
+
+void batch_release(lbx_ctx *ctx)
+{
+    for (auto &kv : ctx->batch_plans) {
+        BatchPlan *p = kv.second;
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        if (p->jobs_d) (void)hipFree(p->jobs_d);
+        if (p->ntt_d) (void)hipFree(p->ntt_d);
+        if (p->pw_d) (void)hipFree(p->pw_d);
+        if (p->counters) (void)hipFree(p->counters);
+        if (p->ws) (void)hipFree(p->ws);
+        delete p;
+    }
+    ctx->batch_plans.clear();
+}
+
+}  // namespace lbx
+
+using namespace lbx;
+
+extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream)
+{
+    if (!ctx || (!jobs && njobs)) return LBX_ERR_INVALID_ARG;
+    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+    for (size_t i = 0; i < njobs; ++i) {
+        const lbx_job &j = jobs[i];
+        if (!j.a || !j.b || !j.c || !j.m || !j.k || !j.n || !j.sa || !j.sb || j.p < 2) return LBX_ERR_INVALID_ARG;
+    }
+    if (!njobs) return LBX_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    // operands must be device memory; pointers already seen by the cached plan
+    // of this shape are not looked up again
+    auto pointers_ok = [&](const BatchPlan *plan, size_t j, const lbx_job &x) {
+        if (plan && j < plan->jobs.size() && plan->jobs[j].a == x.a && plan->jobs[j].b == x.b &&
+            plan->jobs[j].c == x.c)
+            return true;
+        return is_device_ptr(x.a) && is_device_ptr(x.b) && is_device_ptr(x.c);
+    };
+    DeviceGuard guard(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<lbx_job> bj;
+    std::vector<unsigned> Ls;
+    std::vector<size_t> rest;
+    for (size_t i = 0; i < njobs; ++i) {
+        unsigned L;
+        if (batchable(jobs[i], &L)) {
+            bj.push_back(jobs[i]);
+            Ls.push_back(L);
+        } else {
+            rest.push_back(i);
+        }
+    }
+    if (!bj.empty()) {
+        const std::string sig = signature(bj);
+        BatchPlan *plan = nullptr;
+        auto it = ctx->batch_plans.find(sig);
+        if (it != ctx->batch_plans.end()) plan = it->second;
+        for (size_t j = 0; j < bj.size(); ++j)
+            if (!pointers_ok(plan, j, bj[j])) return LBX_ERR_INVALID_ARG;
+        if (!plan) {
+            if (ctx->batch_plans.size() >= 8) batch_release(ctx);  // bounded cache
+            LBX_TRY(build_plan(ctx, bj, Ls, s, &plan));
+            ctx->batch_plans[sig] = plan;
+        }
+        for (size_t j = 0; j < bj.size(); ++j) {  // this call's operands
+            plan->jobs[j].a = bj[j].a;
+            plan->jobs[j].b = bj[j].b;
+            plan->jobs[j].c = bj[j].c;
+        }
+        LBX_TRY_HIP(hipMemcpyAsync(plan->jobs_d, plan->jobs.data(), plan->jobs.size() * sizeof(BatchJob),
+                                   hipMemcpyHostToDevice, s));
+        LBX_TRY_HIP(hipGraphLaunch(plan->exec, s));
+    }
+    for (size_t i : rest) {  // outside the batch kernels' range: single-product routes
+        const lbx_job &j = jobs[i];
+        if (!pointers_ok(nullptr, 0, j)) return LBX_ERR_INVALID_ARG;
+        lbx_polmat a{(void *)j.a, LBX_POLFIRST, j.m, j.k, j.sa, j.sa, 0};
+        lbx_polmat b{(void *)j.b, LBX_POLFIRST, j.k, j.n, j.sb, j.sb, 0};
+        lbx_polmat c{(void *)j.c, LBX_POLFIRST, j.m, j.n, j.sa + j.sb - 1, j.sa + j.sb - 1, 0};
+        const size_t deg = j.sa + j.sb - 2;
+        const unsigned L = log2_len(deg);
+        const bool single = j.sa + j.sb > 2 && j.p < (1u << 31) && L <= (unsigned)kMaxLdsLog &&
+                            (j.p - 1) % ((uint32_t)1 << L) == 0;
+        if (single) {
+            LBX_TRY(fft_prime_product(ctx, j.p, L, devview(&a, (void *)j.a), devview(&b, (void *)j.b),
+                                      devview(&c, j.c), j.m, j.k, j.n, deg + 1, method, s));
+        } else if (j.sa + j.sb <= 2 && j.p < (1u << 31)) {
+            LBX_TRY(constant_product(ctx, j.p, devview(&a, (void *)j.a), devview(&b, (void *)j.b), devview(&c, j.c),
+                                     j.m, j.k, j.n, s));
+        } else {
+            ctx->mu.unlock();  // the multiprecision entry point takes the lock itself
+            lbx_status st = lbx_polymat_mul_mp(ctx, &j.p, 1, &c, &a, &b, method, stream);
+            ctx->mu.lock();
+            LBX_TRY(st);
+        }
+    }
+    return LBX_OK;
+}

@@ -449,6 +449,7 @@ lbx_status lbx_destroy(lbx_ctx *ctx)
         for (auto ev : ctx->events) (void)hipEventDestroy(ev);
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
         mp_release(ctx);
+        batch_release(ctx);
     }
     delete ctx;
     return LBX_OK;

@@ -19,7 +19,8 @@ struct Buffer {
     size_t bytes = 0;
 };
 
-struct MpPlan;  // cached CRT constants of the multiprecision route (mp.hip)
+struct MpPlan;     // cached CRT constants of the multiprecision route (mp.hip)
+struct BatchPlan;  // cached item lists and graph of a batch shape (batch.hip)
 
 }  // namespace lbx
 
@@ -29,6 +30,8 @@ struct lbx_ctx {
     std::map<std::string, lbx::Buffer> bufs;
     std::map<std::tuple<uint32_t, unsigned, uint32_t>, uint2 *> twiddles;
     std::map<std::string, lbx::MpPlan *> mp_plans;
+    std::map<std::string, lbx::BatchPlan *> batch_plans;
+    int tile_tuned[3] = {-1, -1, -1};  // batch pointwise tile per size class, -1 = not tuned yet
     hipStream_t aux = nullptr;         // second stream for chunk pipelining (default priority)
     std::vector<hipEvent_t> events;    // pool of dependency events
     bool profiling = false;
@@ -93,5 +96,13 @@ lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const De
                             size_t k, size_t n, hipStream_t s);
 // Frees the multiprecision plans cached in a context.
 void mp_release(lbx_ctx *ctx);
+// Frees the batch plans (graphs, descriptors) cached in a context.
+void batch_release(lbx_ctx *ctx);
+
+// Root of unity of order 2^k (fft-utils.h:83-104), its inverse (fft.h:119-130)
+// and the cached device twiddle table of a root (fft-integral.inl:1614-1650).
+lbx_status primitive_root(uint32_t p, unsigned k, uint32_t *w_out);
+uint32_t invroot(uint32_t p, unsigned L, uint32_t w);
+lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint2 **out);
 
 }  // namespace lbx

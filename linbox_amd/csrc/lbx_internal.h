@@ -103,6 +103,48 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
 // Point-major [t][e] -> polfirst [e][t] transpose of `entries` x npts words.
 hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries, size_t npts, hipStream_t s);
 
+// ---- batch scheduler (batch.hip, with kernels in ntt.hip / pointwise.hip) ----
+// Constants of the plain pointwise epilogue: V = hi 2^64 + ah 2^32 + al,
+// V*scale mod p = hi*c2 + ah*c1 + al*c0 mod p.
+struct PlainConst {
+    uint32_t c0, c0p, c1, c1p, c2, c2p;
+};
+PlainConst plain_const(uint32_t p, uint32_t scale);
+
+// One product of a batch as the device sees it: operands polfirst, per-job
+// slices of the batch workspace for the transforms.
+struct BatchJob {
+    uint32_t p, L, m, k, n, sa, sb, sc;
+    uint32_t tile;            // pointwise register tile: 0 = 2x2, 1 = 4x4, 2 = 8x8
+    uint32_t pad_;
+    const uint32_t *a, *b;
+    uint32_t *c;
+    uint32_t *ahat, *bhat, *chat;  // m*k, k*n, m*n rows of 2^L words
+    const uint2 *twf, *twi;        // twiddles of the root and of its inverse
+    PlainConst pc;                 // epilogue constants with scale 2^-L
+};
+// Transform work item: entries [e0, e0+count) of operand op (0 a, 1 b, 2 c).
+struct NttItem {
+    uint32_t job, op, e0, count;
+};
+// Pointwise work item: 256 points from pt0 of output tile (ti, tj).
+struct PwItem {
+    uint32_t job, ti, tj, pt0;
+};
+constexpr int kBatchMinL = 4, kBatchMaxL = 13;
+constexpr int kBatchNttThreads = 512;
+// Polynomials of length 2^L one 512-thread transform work item covers.
+inline uint32_t batch_polys_per_item(unsigned L) { return L >= 13 ? 1u : (1u << (13 - L)); }
+size_t batch_ntt_lds_bytes();
+// Persistent work-queue kernels; counter[0] must be zero at launch.  The
+// transform kernel serves the items of one length 2^L.
+hipError_t launch_batch_ntt(unsigned L, const BatchJob *jobs, const NttItem *items, uint32_t nitems,
+                            uint32_t *counter, bool inverse, unsigned grid, hipStream_t s);
+hipError_t launch_batch_pointwise(const BatchJob *jobs, const PwItem *items, uint32_t nitems, uint32_t *counter,
+                                  unsigned grid, hipStream_t s);
+// Zero `n` counters with a kernel (graph-safe, no memset node).
+hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t s);
+
 // ---- layout helpers (layout.hip) ----
 // out[e*stride + t] = view(e, t) for t < count (zero for t >= view.size).
 hipError_t launch_gather(const DevView &in, size_t entries, size_t count, uint32_t *out, size_t stride,

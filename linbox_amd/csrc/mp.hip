@@ -448,13 +448,3 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
     if (st == LBX_OK) c->size = sc;
     return st;
 }
-
-extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream)
-{
-    (void)ctx;
-    (void)jobs;
-    (void)njobs;
-    (void)method;
-    (void)stream;
-    return LBX_ERR_UNSUPPORTED;
-}

@@ -255,6 +255,63 @@ __global__ void __launch_bounds__(256) dit_head_kernel(const uint32_t *__restric
     }
 }
 
+// ---- batch transforms ---------------------------------------------------------
+// Persistent work-queue kernel for the transforms of one length 2^L across a
+// batch: each 512-thread workgroup repeatedly takes an item (entries
+// [e0, e0+count) of one operand of one job) from a device counter.  A 2^13
+// transform uses all 512 threads; shorter ones run as two independent
+// 256-thread groups of Cfg<L>::G polynomials each.
+template <int L>
+__global__ void __launch_bounds__(kBatchNttThreads) batch_ntt_kernel(const BatchJob *__restrict__ jobs,
+                                                                     const NttItem *__restrict__ items,
+                                                                     uint32_t nitems, uint32_t *counter, int inverse)
+{
+    using C = Cfg<L>;
+    constexpr bool whole = C::BLOCK == kBatchNttThreads;
+    extern __shared__ uint32_t lds_all[];
+    __shared__ uint32_t item_sh;
+    const int sub = whole ? 0 : (int)(threadIdx.x >> 8);
+    const int tid = whole ? (int)threadIdx.x : (int)(threadIdx.x & 255);
+    uint32_t *lds = lds_all + sub * C::G * C::STRIDE;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t it = item_sh;
+        __syncthreads();
+        if (it >= nitems) return;
+        const NttItem w = items[it];
+        const BatchJob &J = jobs[w.job];
+        const size_t e_base = (size_t)w.e0 + (size_t)sub * C::G;
+        const size_t e_end = (size_t)w.e0 + w.count;
+        if (!inverse) {
+            const bool isa = w.op == 0;
+            DevView in;
+            in.base = const_cast<uint32_t *>(isa ? J.a : J.b);
+            in.layout = 0;
+            in.rows = isa ? J.m : J.k;
+            in.cols = isa ? J.k : J.n;
+            in.stor = in.size = isa ? J.sa : J.sb;
+            in.shift = 0;
+            const EntryMap id{0, ~(size_t)0, 0};
+            uint32_t *out = isa ? J.ahat : J.bhat;
+            if (in.size * 2 <= (size_t)C::N)
+                dif_body<L, L, true>(in, e_end, id, J.twf, J.p, out, e_base, tid, lds);
+            else
+                dif_body<L, L, false>(in, e_end, id, J.twf, J.p, out, e_base, tid, lds);
+        } else {
+            DevView out;
+            out.base = J.c;
+            out.layout = 0;
+            out.rows = J.m;
+            out.cols = J.n;
+            out.stor = out.size = J.sc;
+            out.shift = 0;
+            dit_body<L, 0, false>(J.chat, e_end, J.twi, J.p, out, 0, J.sc, 1u, 0u, e_base, tid, lds);
+        }
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
 }  // namespace
 
 hipError_t launch_ntt_forward(const DevView &in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
@@ -348,6 +405,31 @@ hipError_t launch_dit_head(const uint32_t *Cm, size_t entries, size_t e_first, s
         break;
         LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11) LBX_CASE(12) LBX_CASE(13) LBX_CASE(14) LBX_CASE(15)
 #undef LBX_CASE
+    }
+    return hipGetLastError();
+}
+
+size_t batch_ntt_lds_bytes()
+{
+    // 2 groups x G x padded stride for L <= 12, one padded 2^13 row for L = 13: all equal
+    return (size_t)Cfg<13>::STRIDE * 4;
+}
+
+hipError_t launch_batch_ntt(unsigned L, const BatchJob *jobs, const NttItem *items, uint32_t nitems,
+                            uint32_t *counter, bool inverse, unsigned grid, hipStream_t s)
+{
+    static_assert(2 * Cfg<12>::G * Cfg<12>::STRIDE == Cfg<13>::STRIDE, "batch LDS layout");
+    static_assert(2 * Cfg<4>::G * Cfg<4>::STRIDE == Cfg<13>::STRIDE, "batch LDS layout");
+    switch (L) {
+#define LBX_CASE(l)                                                                                          \
+    case l:                                                                                                  \
+        hipLaunchKernelGGL(batch_ntt_kernel<l>, dim3(grid), dim3(kBatchNttThreads), batch_ntt_lds_bytes(), s, \
+                           jobs, items, nitems, counter, inverse ? 1 : 0);                                   \
+        break;
+        LBX_CASE(4) LBX_CASE(5) LBX_CASE(6) LBX_CASE(7) LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11)
+        LBX_CASE(12) LBX_CASE(13)
+#undef LBX_CASE
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -29,21 +29,11 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-// Constants of the plain epilogue: V = hi 2^64 + ah 2^32 + al,
-// V*scale mod p = hi*c2 + ah*c1 + al*c0 mod p.
-struct PlainConst {
-    uint32_t c0, c0p, c1, c1p, c2, c2p;
-};
-
 template <int TI, int TJ>
-__global__ void __launch_bounds__(256) pw_plain_kernel(const uint32_t *__restrict__ A, const uint32_t *__restrict__ B,
-                                                       uint32_t *__restrict__ C, int m, int k, int n, size_t npts,
-                                                       uint32_t p, PlainConst pc)
+__device__ __forceinline__ void pw_plain_body(const uint32_t *__restrict__ A, const uint32_t *__restrict__ B,
+                                              uint32_t *__restrict__ C, int m, int k, int n, size_t npts,
+                                              uint32_t p, const PlainConst &pc, size_t t, int i0, int j0)
 {
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= npts) return;
-    const int tiles_j = (n + TJ - 1) / TJ;
-    const int i0 = (blockIdx.y / tiles_j) * TI, j0 = (blockIdx.y % tiles_j) * TJ;
 
     uint64_t acc[TI][TJ];
     uint32_t hi[TI][TJ];
@@ -83,6 +73,50 @@ __global__ void __launch_bounds__(256) pw_plain_kernel(const uint32_t *__restric
             r = add_mod(r, shoup_mul(al, pc.c0, pc.c0p, p), p);
             C[((size_t)(i0 + ii) * n + j0 + jj) * npts + t] = r;
         }
+}
+
+template <int TI, int TJ>
+__global__ void __launch_bounds__(256) pw_plain_kernel(const uint32_t *__restrict__ A, const uint32_t *__restrict__ B,
+                                                       uint32_t *__restrict__ C, int m, int k, int n, size_t npts,
+                                                       uint32_t p, PlainConst pc)
+{
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= npts) return;
+    const int tiles_j = (n + TJ - 1) / TJ;
+    pw_plain_body<TI, TJ>(A, B, C, m, k, n, npts, p, pc, t, (blockIdx.y / tiles_j) * TI, (blockIdx.y % tiles_j) * TJ);
+}
+
+// Persistent work-queue form over the items of a batch: each workgroup takes
+// 256 points of one output tile of one job at a time.
+__global__ void __launch_bounds__(256) batch_pointwise_kernel(const BatchJob *__restrict__ jobs,
+                                                              const PwItem *__restrict__ items, uint32_t nitems,
+                                                              uint32_t *counter)
+{
+    __shared__ uint32_t item_sh;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t it = item_sh;
+        __syncthreads();
+        if (it >= nitems) return;
+        const PwItem w = items[it];
+        const BatchJob &J = jobs[w.job];
+        const size_t npts = (size_t)1 << J.L;
+        const size_t t = (size_t)w.pt0 + threadIdx.x;
+        if (t < npts) {
+            if (J.tile == 0)
+                pw_plain_body<2, 2>(J.ahat, J.bhat, J.chat, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 2, w.tj * 2);
+            else if (J.tile == 1)
+                pw_plain_body<4, 4>(J.ahat, J.bhat, J.chat, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 4, w.tj * 4);
+            else
+                pw_plain_body<8, 8>(J.ahat, J.bhat, J.chat, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 8, w.tj * 8);
+        }
+    }
+}
+
+__global__ void zero_counters_kernel(uint32_t *c, int n)
+{
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
 }
 
 // ---- limb blocks -------------------------------------------------------
@@ -352,10 +386,8 @@ __global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__r
 
 }  // namespace
 
-hipError_t launch_pointwise_plain(const uint32_t *A, const uint32_t *B, uint32_t *C, size_t m, size_t k, size_t n,
-                                  size_t npts, uint32_t p, uint32_t scale, hipStream_t s)
+PlainConst plain_const(uint32_t p, uint32_t scale)
 {
-    constexpr int TI = 8, TJ = 8;
     PlainConst pc;
     const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
     const uint32_t r64 = (uint32_t)((uint64_t)r32 * r32 % p);
@@ -365,6 +397,27 @@ hipError_t launch_pointwise_plain(const uint32_t *A, const uint32_t *B, uint32_t
     pc.c0p = shoup_pre(pc.c0, p);
     pc.c1p = shoup_pre(pc.c1, p);
     pc.c2p = shoup_pre(pc.c2, p);
+    return pc;
+}
+
+hipError_t launch_batch_pointwise(const BatchJob *jobs, const PwItem *items, uint32_t nitems, uint32_t *counter,
+                                  unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_pointwise_kernel, dim3(grid), dim3(256), 0, s, jobs, items, nitems, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise_plain(const uint32_t *A, const uint32_t *B, uint32_t *C, size_t m, size_t k, size_t n,
+                                  size_t npts, uint32_t p, uint32_t scale, hipStream_t s)
+{
+    constexpr int TI = 8, TJ = 8;
+    const PlainConst pc = plain_const(p, scale);
     const size_t tiles = ((m + TI - 1) / TI) * ((n + TJ - 1) / TJ);
     if (tiles > 65535) return hipErrorInvalidValue;
     dim3 grid((unsigned)((npts + 255) / 256), (unsigned)tiles);
