@@ -8,14 +8,18 @@ resident in HBM.  `value` is products/s of the INT8-MFMA form; the plain
 uint64 vector-ALU form is timed the same way for the ratio.
 
 The JSON line also carries:
-  roofline      the dominant stage's algorithmic HBM bytes per launch divided by
-                its average duration, measured live with HIP events recorded on
-                the launch stream during the timed steps;
+  roofline      for the kernel with the largest time per product: its algorithmic
+                HBM bytes per launch divided by its average launch time, measured
+                live with HIP events recorded on the launch stream during the
+                timed steps; `traffic` is the measured HBM bytes per launch from
+                the committed rocprofv3 PMC summary (profiles/hbm_traffic_c3.txt);
+  kernels       the same figures for every kernel of the product;
+  batch_config5 1024 uneven products as one batch call vs one call per product;
   cpu_baseline  the oracle's restatement of the reference's single-prime
                 mul_fft timed on this host on a bounded sample of the same
                 workload and extrapolated to one product.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-batch]
 """
 import argparse
 import json
@@ -34,16 +38,56 @@ SA = SB = 8192
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(m, k, n, sa, sb, pts):
-    """HBM bytes each stage must move at least once (4-byte words)."""
+def kernel_bytes(m, k, n, sa, sb, pts):
+    """Algorithmic HBM bytes of one launch of each kernel of the product: every
+    word the launch must read or write once (4 bytes per coefficient, residue or
+    4-digit limb group).  With the default chunk size each kernel runs once per
+    operand per product at this shape."""
     deg = sa + sb - 2
     return {
-        "direct FFT_DIF": 4 * (m * k * (sa + pts) + k * n * (sb + pts)),
-        "Polfirst to Matfirst": 4 * 2 * (m * k + k * n) * pts,
-        "Pointwise mult": 4 * (m * k + k * n + m * n) * pts,
-        "Matfirst to Polfirst": 4 * 2 * m * n * pts,
-        "inverse FFT_DIT": 4 * m * n * (pts + deg + 1),
+        "kernel dif_head a": 4 * m * k * (sa + pts),        # polynomial in, 2^14 words after 8 stages out
+        "kernel dif_head b": 4 * k * n * (sb + pts),
+        "kernel dif_tail_limbs a": 4 * m * k * pts * 2,     # 2^14 words in, 2^14 limb groups out
+        "kernel dif_tail_limbs b": 4 * k * n * pts * 2,
+        "kernel pw_mfma": 4 * (m * k + k * n + m * n) * pts,  # per point: A, B limb blocks in, C out
+        "kernel dit_head": 4 * m * n * pts * 2,             # point-major C in, 6 stages, entry blocks out
+        "kernel dit_tail": 4 * m * n * (pts + deg + 1),     # 2^14 words in, deg+1 coefficients out
     }
+
+
+def traffic_table():
+    """Measured HBM bytes per launch, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a device-to-device copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c3.txt")
+    table = {}
+    if os.path.exists(path):
+        for line in open(path):  # rows "| kernel <label> | ... | <bytes per launch> |"
+            if line.startswith("| kernel "):
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                try:
+                    table[cells[0]] = int(cells[-1])
+                except ValueError:
+                    pass
+    return table
+
+
+def kernel_roofline(times, bytes_per_launch):
+    """Roofline of the kernel with the largest time per product (one launch per
+    label per product): achieved = algorithmic bytes / average launch time."""
+    t = {label: ms for label, ms in times if label in bytes_per_launch}
+    if not t:
+        return None, {}
+    traffic = traffic_table()
+    kernels = {label: {"ms": round(ms, 4),
+                       "algorithmic_bytes": bytes_per_launch[label],
+                       "achieved_gbs": round(bytes_per_launch[label] / (ms * 1e-3) / 1e9, 1),
+                       "traffic": traffic.get(label)} for label, ms in t.items()}
+    dom = max(t, key=lambda label: t[label])
+    achieved = bytes_per_launch[dom] / (t[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
+                "algorithmic_bytes": bytes_per_launch[dom], "launch_ms": round(t[dom], 4)}
+    return roofline, kernels
 
 
 def run_gpu(args):
@@ -68,7 +112,7 @@ def run_gpu(args):
             dom.mul(C, A, B)
         stream.synchronize()
         hip.synchronize()
-        ctx.profile(True)
+        ctx.profile(2)  # stage and per-kernel events on the launch streams
         ctx.stage_times()  # drop anything recorded before the timed region
         e0, e1 = hip.Event(), hip.Event()
         t0 = time.perf_counter()
@@ -159,6 +203,19 @@ class _Ptr:
         return self.ptr
 
 
+def copy_probe(mib):
+    """Three device-to-device copies of a known size: the runtime's blit kernel
+    reads and writes exactly mib MiB each, which calibrates the HBM byte counters."""
+    from linbox_amd import hip
+    nbytes = mib << 20
+    src, dst = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+    src.zero()
+    for _ in range(3):
+        hip._ok(hip.runtime().hipMemcpy(dst.ptr, src.ptr, nbytes, hip.D2D), "hipMemcpy D2D")
+    hip.synchronize()
+    print("copied %d MiB x3" % mib, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(budget_s):
     """Oracle (C restatement of mul_fft, one thread) on a bounded sample of the workload.
 
@@ -210,25 +267,24 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--skip-batch", action="store_true", help="omit the 1024-product batch measurement")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run three device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py measures one GPU: products are not sharded across GPUs, so --gpus must be 1",
               file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs as a single process on one GPU", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
     batch = None if args.skip_batch else run_batch()
     pts = 1 << 14
-    algo = algorithmic_bytes(M, K, N, SA, SB, pts)
-    st = dict(stages)
-    dominant = max((l for l in st if l in algo), key=lambda l: st[l])
-    achieved = algo[dominant] / (st[dominant] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "stage": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": algo[dominant]}
+    roofline, kernels = kernel_roofline(stages, kernel_bytes(M, K, N, SA, SB, pts))
     if args.skip_cpu:
         cpu = None
     else:
@@ -253,9 +309,10 @@ def main():
                    "m": M, "k": K, "n": N, "degree": SA - 1, "p": P, "ntt_points": pts,
                    "pointwise": "int8_mfma", "parallelism": "single_gpu"},
         "plain_u64": {"value": round(1000.0 / ms_plain, 3), "ms_per_step": round(ms_plain, 4),
-                      "stages_ms": {k: round(v, 4) for k, v in stages_plain}},
+                      "stages_ms": {k: round(v, 4) for k, v in stages_plain if not k.startswith("kernel ")}},
         "int8_vs_plain": round(ms_plain / ms_int8, 3),
-        "stages_ms": {k: round(v, 4) for k, v in stages},
+        "stages_ms": {k: round(v, 4) for k, v in stages if not k.startswith("kernel ")},
+        "kernels": kernels,
         "gmulmod_per_s": round(40265318400 / (ms_int8 * 1e-3) / 1e9, 2),
         "wall_s_timed": round(wall, 4),
         "roofline": roofline,

@@ -47,7 +47,7 @@ typedef enum lbx_status {
     LBX_ERR_PRIME_TOO_LARGE = 8,    /* modulus outside the supported range (three-primes.inl:57-64 analogue) */
     LBX_ERR_HIP = 9,                /* a HIP runtime call failed */
     LBX_ERR_OUT_OF_MEMORY = 10,     /* device allocation failed */
-    LBX_ERR_UNSUPPORTED = 11,       /* shape beyond what this build implements */
+    LBX_ERR_UNSUPPORTED = 11,       /* shape beyond what the library implements */
     LBX_ERR_NO_DEVICE = 12          /* no gfx950 device at the requested index */
 } lbx_status;
 
@@ -142,14 +142,17 @@ typedef struct lbx_job {
  * lbx_polymat_mul_u32 per job. */
 lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream);
 
-/* FFT_PROFILER analogue (matpoly-mult-fft.h:43-77): while enabled, calls
+/* FFT_PROFILER analogue (matpoly-mult-fft.h:43-77): with enable = 1, calls
  * record device events between their stages, labelled as in mul_fft
  * ("direct FFT_DIF", "Polfirst to Matfirst", "Pointwise mult",
  * "Matfirst to Polfirst", "inverse FFT_DIT"; wordsize-fast.inl:116-189),
- * without any host synchronisation.  lbx_profile_get waits for the recorded
- * events, returns the per-stage average time (ms) over the calls made since
- * the previous get, copies up to `max` (label, ms) pairs and returns their
- * count. */
+ * without any host synchronisation.  enable = 2 also brackets every kernel
+ * launch of the single-prime product with events on the stream it runs on,
+ * labelled "kernel <name>[ a|b]" (a and b name the operand); enable = 0 stops.
+ * lbx_profile_get waits for the recorded events, returns the average time (ms)
+ * per label over the records made since the previous get (per call for a
+ * stage, per launch for a kernel), copies up to `max` (label, ms) pairs and
+ * returns their count. */
 lbx_status lbx_profile_enable(lbx_ctx *ctx, int enable);
 int lbx_profile_get(lbx_ctx *ctx, const char **labels, float *ms, int max);
 

@@ -113,12 +113,13 @@ class Context:
             pass
 
     def profile(self, enable=True):
-        check(lib().lbx_profile_enable(self.handle, 1 if enable else 0))
+        """enable: False/0 off, True/1 stage times, 2 stage and per-kernel times."""
+        check(lib().lbx_profile_enable(self.handle, int(enable)))
 
     def stage_times(self):
-        labels = (ctypes.c_char_p * 16)()
-        ms = (ctypes.c_float * 16)()
-        n = lib().lbx_profile_get(self.handle, labels, ms, 16)
+        labels = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_float * 32)()
+        n = lib().lbx_profile_get(self.handle, labels, ms, 32)
         return [(labels[i].decode(), ms[i]) for i in range(n)]
 
 

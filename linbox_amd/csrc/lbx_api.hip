@@ -179,6 +179,30 @@ private:
     hipEvent_t last_ = nullptr;
 };
 
+// Kernel timer (profiling level 2): events around one launch on the stream it
+// is queued on, so concurrent launches on other streams do not blur it.
+class KernelTimer {
+public:
+    KernelTimer(lbx_ctx *ctx, hipStream_t s, const char *label) : ctx_(ctx), s_(s), label_(label)
+    {
+        if (ctx_->profiling >= 2 && hipEventCreate(&start_) == hipSuccess) (void)hipEventRecord(start_, s_);
+    }
+    ~KernelTimer()
+    {
+        if (!start_) return;
+        hipEvent_t end = nullptr;
+        (void)hipEventCreate(&end);
+        (void)hipEventRecord(end, s_);
+        ctx_->prof_pending.emplace_back(label_, start_, end);
+    }
+
+private:
+    lbx_ctx *ctx_;
+    hipStream_t s_;
+    const char *label_;
+    hipEvent_t start_ = nullptr;
+};
+
 int choose_method(int method, size_t m, size_t k, size_t n)
 {
     if (method == LBX_METHOD_PLAIN_U64 || method == LBX_METHOD_INT8_MFMA) return method;
@@ -186,10 +210,11 @@ int choose_method(int method, size_t m, size_t k, size_t n)
     return (m >= 16 && n >= 16 && k >= 16) ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
 }
 
-// Bytes of head -> tail intermediate per chunk (2 GiB by default: one chunk
-// per operand at the sizes measured so far, where the head of B overlaps the
-// tail of A); the LBX_CHUNK_BYTES environment variable overrides it (tuning
-// and tests).
+// Bytes of head -> tail intermediate per chunk.  The default, 2 GiB, gives one
+// chunk per operand up to 128x128 entries of 2^14 points: smaller chunks
+// measured slower there (the hand-off through the last-level cache gains less
+// than the extra launches and partial waves cost).  The LBX_CHUNK_BYTES
+// environment variable overrides it (tuning and tests).
 static size_t chunk_bytes()
 {
     const char *e = std::getenv("LBX_CHUNK_BYTES");
@@ -260,6 +285,20 @@ static lbx_status pipeline_end(lbx_ctx *ctx, hipStream_t s, size_t nchunks)
     return LBX_OK;
 }
 
+// The pipeline above when `on`; otherwise (a single chunk per operand, where
+// there is nothing to overlap) producer and consumer both run on s in order.
+struct Pipe {
+    lbx_ctx *ctx;
+    hipStream_t s;
+    bool on;
+    hipStream_t consumer() const { return on ? ctx->aux : s; }
+    lbx_status begin() { return on ? pipeline_begin(ctx, s) : LBX_OK; }
+    lbx_status reuse(size_t c) { return on ? pipeline_reuse(ctx, s, c) : LBX_OK; }
+    lbx_status handoff(size_t c) { return on ? pipeline_handoff(ctx, s, c) : LBX_OK; }
+    lbx_status done(size_t c) { return on ? pipeline_done(ctx, c) : LBX_OK; }
+    lbx_status end(size_t n) { return on ? pipeline_end(ctx, s, n) : LBX_OK; }
+};
+
 // Forward transforms, pointwise products and inverse transform of one
 // single-prime product on device views.  deg+1 coefficients of c are written.
 lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
@@ -274,10 +313,6 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     const uint2 *twf, *twi;
     LBX_TRY(twiddles(ctx, p, L, w, &twf));
     LBX_TRY(twiddles(ctx, p, L, wi, &twi));
-    void *ahat, *bhat, *chat;
-    LBX_TRY(ensure(ctx, "ahat", m * k * N * 4, &ahat));
-    LBX_TRY(ensure(ctx, "bhat", k * n * N * 4, &bhat));
-    LBX_TRY(ensure(ctx, "chat", m * n * N * 4, &chat));
     const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
     method = choose_method(method, m, k, n);
 
@@ -285,9 +320,9 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     if (method == LBX_METHOD_INT8_MFMA && L >= 8) {
         // Split transforms: the last six forward stages write per-point limb
         // blocks and the first six inverse stages read the point-major
-        // product.  Entries go in chunks of about 64 MiB of intermediate so the
-        // head -> tail hand-off can stay in the Infinity Cache; heads run on
-        // the caller's stream and tails on a second one, two chunks in flight.
+        // product.  Entries go in chunks of chunk_bytes() of head -> tail
+        // intermediate; with several chunks, heads run on the caller's stream
+        // and tails on a second one, two chunks in flight.
         void *al, *bl, *cmat;
         LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
         LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
@@ -296,10 +331,18 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         const size_t mpad = (m + 31) & ~(size_t)31, npad = (n + 31) & ~(size_t)31;
         const size_t rpc = std::max<size_t>(1, target / k);  // limb rows per chunk
         const size_t slot_entries = rpc * k;
+        const size_t cchunk = std::max(slot_entries, target);  // product entries per inverse chunk
+        const bool fwd_pipe = (mpad + rpc - 1) / rpc > 1 || (npad + rpc - 1) / rpc > 1;
+        const bool inv_pipe = m * n > cchunk;
+        // a slot holds the largest chunk either direction uses; one slot suffices
+        // when nothing is pipelined
+        const size_t slot_words = std::max(std::min(rpc, std::max(m, n)) * k, std::min(cchunk, m * n)) * N;
+        const size_t nslots = (fwd_pipe || inv_pipe) ? 2 : 1;
         void *scr;
-        LBX_TRY(ensure(ctx, "scratch", 2 * std::max(slot_entries, target) * N * 4, &scr));
-        uint32_t *slot[2] = {(uint32_t *)scr, (uint32_t *)scr + std::max(slot_entries, target) * N};
-        LBX_TRY(pipeline_begin(ctx, s));
+        LBX_TRY(ensure(ctx, "scratch", nslots * slot_words * 4, &scr));
+        uint32_t *slot[2] = {(uint32_t *)scr, (uint32_t *)scr + (nslots - 1) * slot_words};
+        Pipe fp{ctx, s, fwd_pipe};
+        LBX_TRY(fp.begin());
         size_t chunk = 0;
         // forward: rows of A (entries i*k+q), then columns of B (entries q*n+j)
         for (int op = 0; op < 2; ++op) {
@@ -308,40 +351,63 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
                 const size_t nr = std::min(rpc, rpadx - r0);
                 const size_t valid = r0 < rows ? std::min(nr, rows - r0) : 0;
                 uint32_t *buf = slot[chunk & 1];
-                LBX_TRY(pipeline_reuse(ctx, s, chunk));
+                LBX_TRY(fp.reuse(chunk));
                 if (valid) {
                     const EntryMap map = op == 0 ? EntryMap{r0 * k, valid * k, 0} : EntryMap{r0, valid, n};
+                    KernelTimer kt(ctx, s, op == 0 ? "kernel dif_head a" : "kernel dif_head b");
                     LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, twf, p, buf, s));
                 }
-                LBX_TRY(pipeline_handoff(ctx, s, chunk));
-                LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
-                                                  op == 0, L, twf, p, ctx->aux));
-                LBX_TRY(pipeline_done(ctx, chunk));
+                LBX_TRY(fp.handoff(chunk));
+                {
+                    KernelTimer kt(ctx, fp.consumer(), op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b");
+                    LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
+                                                      op == 0, L, twf, p, fp.consumer()));
+                }
+                LBX_TRY(fp.done(chunk));
             }
         }
-        LBX_TRY(pipeline_end(ctx, s, chunk));
+        LBX_TRY(fp.end(chunk));
         st.mark("direct FFT_DIF");
-        LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N, p,
-                                          inv_n, s));
+        {
+            KernelTimer kt(ctx, s, "kernel pw_mfma");
+            LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N,
+                                              p, inv_n, s));
+        }
         st.mark("Pointwise mult");
-        LBX_TRY(pipeline_begin(ctx, s));
+        Pipe ip{ctx, s, inv_pipe};
+        LBX_TRY(ip.begin());
         chunk = 0;
-        const size_t cchunk = std::max(slot_entries, target);
         for (size_t e0 = 0; e0 < m * n; e0 += cchunk, ++chunk) {
             const size_t ne = std::min(cchunk, m * n - e0);
             uint32_t *buf = slot[chunk & 1];
-            LBX_TRY(pipeline_reuse(ctx, s, chunk));
-            LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, twi, p, buf, s));
-            LBX_TRY(pipeline_handoff(ctx, s, chunk));
-            LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, twi, p, c, e0, count, 1, ctx->aux));
-            LBX_TRY(pipeline_done(ctx, chunk));
+            LBX_TRY(ip.reuse(chunk));
+            {
+                KernelTimer kt(ctx, s, "kernel dit_head");
+                LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, twi, p, buf, s));
+            }
+            LBX_TRY(ip.handoff(chunk));
+            {
+                KernelTimer kt(ctx, ip.consumer(), "kernel dit_tail");
+                LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, twi, p, c, e0, count, 1, ip.consumer()));
+            }
+            LBX_TRY(ip.done(chunk));
         }
-        LBX_TRY(pipeline_end(ctx, s, chunk));
+        LBX_TRY(ip.end(chunk));
         st.mark("inverse FFT_DIT");
         return LBX_OK;
     }
-    LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twf, p, (uint32_t *)ahat, s));
-    LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twf, p, (uint32_t *)bhat, s));
+    void *ahat, *bhat, *chat;
+    LBX_TRY(ensure(ctx, "ahat", m * k * N * 4, &ahat));
+    LBX_TRY(ensure(ctx, "bhat", k * n * N * 4, &bhat));
+    LBX_TRY(ensure(ctx, "chat", m * n * N * 4, &chat));
+    {
+        KernelTimer kt(ctx, s, "kernel dif a");
+        LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twf, p, (uint32_t *)ahat, s));
+    }
+    {
+        KernelTimer kt(ctx, s, "kernel dif b");
+        LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twf, p, (uint32_t *)bhat, s));
+    }
     st.mark("direct FFT_DIF");
     if (method == LBX_METHOD_INT8_MFMA) {
         void *al, *bl;
@@ -359,11 +425,17 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         LBX_TRY_HIP(launch_point_to_entry((const uint32_t *)cmat, (uint32_t *)chat, m * n, N, s));
         st.mark("Matfirst to Polfirst");
     } else {
-        LBX_TRY_HIP(launch_pointwise_plain((const uint32_t *)ahat, (const uint32_t *)bhat, (uint32_t *)chat, m, k, n,
-                                           N, p, inv_n, s));
+        {
+            KernelTimer kt(ctx, s, "kernel pw_plain");
+            LBX_TRY_HIP(launch_pointwise_plain((const uint32_t *)ahat, (const uint32_t *)bhat, (uint32_t *)chat, m, k,
+                                               n, N, p, inv_n, s));
+        }
         st.mark("Pointwise mult");
     }
-    LBX_TRY_HIP(launch_ntt_inverse((const uint32_t *)chat, m * n, L, twi, p, c, count, 1, s));
+    {
+        KernelTimer kt(ctx, s, "kernel dit");
+        LBX_TRY_HIP(launch_ntt_inverse((const uint32_t *)chat, m * n, L, twi, p, c, count, 1, s));
+    }
     st.mark("inverse FFT_DIT");
     return LBX_OK;
 }
@@ -479,7 +551,7 @@ lbx_status lbx_profile_enable(lbx_ctx *ctx, int enable)
 {
     if (!ctx) return LBX_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->profiling = enable != 0;
+    ctx->profiling = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
     return LBX_OK;
 }
 

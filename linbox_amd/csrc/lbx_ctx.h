@@ -34,7 +34,7 @@ struct lbx_ctx {
     int tile_tuned[3] = {-1, -1, -1};  // batch pointwise tile per size class, -1 = not tuned yet
     hipStream_t aux = nullptr;         // second stream for chunk pipelining (default priority)
     std::vector<hipEvent_t> events;    // pool of dependency events
-    bool profiling = false;
+    int profiling = 0;  // 0 off, 1 stages, 2 stages and kernels
     // recorded (label, start, end) event triples, resolved by lbx_profile_get
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;
     std::vector<std::string> prof_labels;

@@ -261,29 +261,37 @@ __global__ void __launch_bounds__(256) dit_head_kernel(const uint32_t *__restric
 // [e0, e0+count) of one operand of one job) from a device counter.  A 2^13
 // transform uses all 512 threads; shorter ones run as two independent
 // 256-thread groups of Cfg<L>::G polynomials each.
-template <int L>
-__global__ void __launch_bounds__(kBatchNttThreads) batch_ntt_kernel(const BatchJob *__restrict__ jobs,
-                                                                     const NttItem *__restrict__ items,
-                                                                     uint32_t nitems, uint32_t *counter, int inverse)
+// Waves per SIMD the register budget targets: 8 (four groups per CU) where the
+// forward body fits in 64 VGPRs without spilling, else 4.
+template <int L, bool INVERSE>
+constexpr int batch_waves() { return (INVERSE || L == 12) ? 4 : 8; }
+
+template <int L, bool INVERSE>
+__global__ void __launch_bounds__(kBatchNttThreads, (batch_waves<L, INVERSE>()))
+    batch_ntt_kernel(const BatchJob *__restrict__ jobs, const NttItem *__restrict__ items, uint32_t nitems,
+                     uint32_t *counter)
 {
     using C = Cfg<L>;
     constexpr bool whole = C::BLOCK == kBatchNttThreads;
     extern __shared__ uint32_t lds_all[];
     __shared__ uint32_t item_sh;
     const int sub = whole ? 0 : (int)(threadIdx.x >> 8);
-    const int tid = whole ? (int)threadIdx.x : (int)(threadIdx.x & 255);
     uint32_t *lds = lds_all + sub * C::G * C::STRIDE;
     for (;;) {
+        // thread index re-derived every item: otherwise the per-slot LDS and
+        // global offsets are hoisted out of the loop and held in registers
+        int tid = whole ? (int)threadIdx.x : (int)(threadIdx.x & 255);
+        asm volatile("" : "+v"(tid));
         if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1u);
         __syncthreads();
-        const uint32_t it = item_sh;
+        const uint32_t it = __builtin_amdgcn_readfirstlane(item_sh);  // uniform: scalar loads of the item
         __syncthreads();
         if (it >= nitems) return;
         const NttItem w = items[it];
         const BatchJob &J = jobs[w.job];
         const size_t e_base = (size_t)w.e0 + (size_t)sub * C::G;
         const size_t e_end = (size_t)w.e0 + w.count;
-        if (!inverse) {
+        if constexpr (!INVERSE) {
             const bool isa = w.op == 0;
             DevView in;
             in.base = const_cast<uint32_t *>(isa ? J.a : J.b);
@@ -423,8 +431,12 @@ hipError_t launch_batch_ntt(unsigned L, const BatchJob *jobs, const NttItem *ite
     switch (L) {
 #define LBX_CASE(l)                                                                                          \
     case l:                                                                                                  \
-        hipLaunchKernelGGL(batch_ntt_kernel<l>, dim3(grid), dim3(kBatchNttThreads), batch_ntt_lds_bytes(), s, \
-                           jobs, items, nitems, counter, inverse ? 1 : 0);                                   \
+        if (inverse)                                                                                         \
+            hipLaunchKernelGGL((batch_ntt_kernel<l, true>), dim3(grid), dim3(kBatchNttThreads),              \
+                               batch_ntt_lds_bytes(), s, jobs, items, nitems, counter);                       \
+        else                                                                                                 \
+            hipLaunchKernelGGL((batch_ntt_kernel<l, false>), dim3(grid), dim3(kBatchNttThreads),             \
+                               batch_ntt_lds_bytes(), s, jobs, items, nitems, counter);                       \
         break;
         LBX_CASE(4) LBX_CASE(5) LBX_CASE(6) LBX_CASE(7) LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11)
         LBX_CASE(12) LBX_CASE(13)

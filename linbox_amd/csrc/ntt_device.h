@@ -212,13 +212,24 @@ __device__ __forceinline__ void dif_body(const DevView &in, size_t entries, cons
     uint32_t x[C::R];
     // first pass mapping (words tp + q*TP when the pass is a full radix-R pass)
     constexpr int RP0 = SEND < C::RLOG ? SEND : C::RLOG;
+    if (in.layout == 0) {  // polfirst: one row base, word offsets t
+        const uint32_t *src = in.base + (active ? ein : 0) * in.stor + in.shift;
 #pragma unroll
-    for (int u = 0; u < (C::R >> RP0); ++u)
+        for (int u = 0; u < (C::R >> RP0); ++u)
 #pragma unroll
-        for (int q = 0; q < (1 << RP0); ++q) {
-            const size_t t = (size_t)elem<L, L - RP0, RP0>(tp, u, q);
-            x[u * (1 << RP0) + q] = (active && t < in.size) ? in.base[view_index(in, ein, t)] : 0u;
-        }
+            for (int q = 0; q < (1 << RP0); ++q) {
+                const int t = elem<L, L - RP0, RP0>(tp, u, q);
+                x[u * (1 << RP0) + q] = (active && (size_t)t < in.size) ? src[t] : 0u;
+            }
+    } else {
+#pragma unroll
+        for (int u = 0; u < (C::R >> RP0); ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << RP0); ++q) {
+                const size_t t = (size_t)elem<L, L - RP0, RP0>(tp, u, q);
+                x[u * (1 << RP0) + q] = (active && t < in.size) ? in.base[view_index(in, ein, t)] : 0u;
+            }
+    }
     dif_passes<L, 0, SEND, HALFZERO>(x, lds, tp, tw, p);
     constexpr int RP = dif_last_rp<L, SEND>();
     constexpr int NG = C::R >> RP;
@@ -290,7 +301,20 @@ __device__ __forceinline__ void dit_body(const uint32_t *__restrict__ in, size_t
     // last pass: a full radix-R pass leaves the thread words tp + q*TP; a
     // transform shorter than one pass leaves groups of 2^(L-DSTART)
     constexpr int RPL = (L - DSTART) >= C::RLOG ? C::RLOG : (L - DSTART);
-    if (active) {
+    if (active && out.layout == 0) {  // polfirst: one row base, word offsets t
+        uint32_t *o = out.base + (out_first + e) * out.stor + out.shift;
+#pragma unroll
+        for (int u = 0; u < (C::R >> RPL); ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << RPL); ++q) {
+                const int t = elem<L, L - RPL, RPL>(tp, u, q);
+                if ((size_t)t < count) {
+                    uint32_t v = x[u * (1 << RPL) + q];
+                    if (SCALE) v = shoup_mul(v, scale, scale_p, p);
+                    o[t] = v;
+                }
+            }
+    } else if (active) {
 #pragma unroll
         for (int u = 0; u < (C::R >> RPL); ++u)
 #pragma unroll

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) batch_pointwise_kernel(const BatchJob *__
     for (;;) {
         if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1u);
         __syncthreads();
-        const uint32_t it = item_sh;
+        const uint32_t it = __builtin_amdgcn_readfirstlane(item_sh);  // uniform: scalar loads of the item
         __syncthreads();
         if (it >= nitems) return;
         const PwItem w = items[it];
