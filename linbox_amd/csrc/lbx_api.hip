@@ -323,6 +323,7 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         // product.  Entries go in chunks of chunk_bytes() of head -> tail
         // intermediate; with several chunks, heads run on the caller's stream
         // and tails on a second one, two chunks in flight.
+        const unsigned BL = kSplitBL;
         void *al, *bl, *cmat;
         LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
         LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
@@ -355,13 +356,13 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
                 if (valid) {
                     const EntryMap map = op == 0 ? EntryMap{r0 * k, valid * k, 0} : EntryMap{r0, valid, n};
                     KernelTimer kt(ctx, s, op == 0 ? "kernel dif_head a" : "kernel dif_head b");
-                    LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, twf, p, buf, s));
+                    LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, BL, twf, p, buf, s));
                 }
                 LBX_TRY(fp.handoff(chunk));
                 {
                     KernelTimer kt(ctx, fp.consumer(), op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b");
                     LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
-                                                      op == 0, L, twf, p, fp.consumer()));
+                                                      op == 0, L, BL, twf, p, fp.consumer()));
                 }
                 LBX_TRY(fp.done(chunk));
             }
@@ -383,12 +384,12 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
             LBX_TRY(ip.reuse(chunk));
             {
                 KernelTimer kt(ctx, s, "kernel dit_head");
-                LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, twi, p, buf, s));
+                LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, BL, twi, p, buf, s));
             }
             LBX_TRY(ip.handoff(chunk));
             {
                 KernelTimer kt(ctx, ip.consumer(), "kernel dit_tail");
-                LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, twi, p, c, e0, count, 1, ip.consumer()));
+                LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, BL, twi, p, c, e0, count, 1, ip.consumer()));
             }
             LBX_TRY(ip.done(chunk));
         }

@@ -62,27 +62,30 @@ struct EntryMap {
     }
 };
 
-// Split transforms for the point-major pointwise form (L in [8,15]):
-// forward head = DIF stages [0, L-6) per entry, polfirst out (row l <- entry map(l));
-// forward tail = the last six DIF stages on 64-point blocks of 64 entries,
+// Split transforms for the point-major pointwise form, with blocks of 2^BL
+// points (BL = kSplitBL, BL + 2 <= L <= 15):
+// forward head = DIF stages [0, L-BL) per entry, polfirst out (row l <- entry map(l));
+// forward tail = the last BL DIF stages on 2^BL-point blocks of 64 entries,
 //                written straight into rows [row0, row0+rows_chunk) of the
 //                per-point limb blocks (rows >= rows_valid are zero padding;
 //                input entry of (row r, inner q) is r*kdim+q if row_major,
 //                else q*rows_valid+r);
-// inverse head = the first six DIT stages on 64-point blocks of the entries
+// inverse head = the first BL DIT stages on 2^BL-point blocks of the entries
 //                [e_first, e_first+e_count) of the point-major product C[t][e],
 //                polfirst out (row = e - e_first);
-// inverse tail = DIT stages [6, L) per entry, written to entries
+// inverse tail = DIT stages [BL, L) per entry, written to entries
 //                out_first + row of the view.
-hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const EntryMap &map, unsigned L,
+constexpr unsigned kSplitBL = 6;
+hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const EntryMap &map, unsigned L, unsigned BL,
                                    const uint2 *tw, uint32_t p, uint32_t *out, hipStream_t s);
 hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows_total, size_t row0, size_t rows_chunk,
-                                 size_t rows_valid, size_t kdim, bool row_major, unsigned L, const uint2 *tw,
-                                 uint32_t p, hipStream_t s);
+                                 size_t rows_valid, size_t kdim, bool row_major, unsigned L, unsigned BL,
+                                 const uint2 *tw, uint32_t p, hipStream_t s);
 hipError_t launch_dit_head(const uint32_t *Cm, size_t entries, size_t e_first, size_t e_count, unsigned L,
-                           const uint2 *tw, uint32_t p, uint32_t *Y, hipStream_t s);
-hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
-                                   const DevView &out, size_t out_first, size_t count, uint32_t scale, hipStream_t s);
+                           unsigned BL, const uint2 *tw, uint32_t p, uint32_t *Y, hipStream_t s);
+hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned L, unsigned BL, const uint2 *tw,
+                                   uint32_t p, const DevView &out, size_t out_first, size_t count, uint32_t scale,
+                                   hipStream_t s);
 
 // ---- pointwise stage (pointwise.hip) ----
 // C[e'][t] = scale * sum_q A[i*k+q][t] B[q*n+j][t] mod p at every point t < npts,

@@ -63,194 +63,210 @@ hipError_t inv_launch(const uint32_t *in, size_t entries, const uint2 *tw, uint3
 }
 
 
-// ---- blockwise stages on 64-point blocks of 64 entries ---------------------
-// After DIF stages [0, L-6) every polynomial is 2^(L-6) independent blocks of
-// 64 consecutive words, and DIT stages [0, 6) act inside such blocks.  These
-// kernels run those six stages for 64 entries at once, which lets them write
-// (forward) or read (inverse) the point-major layout of the pointwise stage
-// with 256-byte runs: the transposition rides along with the butterflies.
-constexpr int kBlk = 64;
-constexpr int kBlkStride = 65;  // LDS row stride (words) of a [64 entries][64 points] tile
+// ---- blockwise stages on 2^BL-point blocks of 64 entries -------------------
+// After DIF stages [0, L-BL) every polynomial is 2^(L-BL) independent blocks
+// of 2^BL consecutive words, and DIT stages [0, BL) act inside such blocks.
+// These kernels run those BL stages for 64 entries at once, which lets them
+// write (forward) or read (inverse) the point-major layout of the pointwise
+// stage with 256-byte runs: the transposition rides along with the
+// butterflies.  TPE = 2^(BL-4) threads own one entry row, 16 words each: in
+// one pass the words sub + TPE*v (the four stages of spans >= TPE), in the
+// other TPE-word groups TPE*(sub + TPE*u) + w (the stages of spans < TPE).
+template <int BL>
+struct Blk {
+    static constexpr int P = 1 << BL;         // points per block
+    static constexpr int TPE = 1 << (BL - 4);  // threads per entry row
+    static constexpr int T = 64 * TPE;         // threads per workgroup
+    static constexpr int STRIDE = P + 1;       // LDS words per entry row
+    static constexpr size_t LDS_BYTES = (size_t)64 * STRIDE * 4;
+};
 
-// DIF on the 6 stages of block `blk` held as x[16] per thread: thread owns
-// entry q = tid>>2 and local positions sub + 4v (pass 1) then 4(sub+4u)+w (pass 2).
-template <int L>
-__device__ __forceinline__ void dif_block_stages(uint32_t (*sm)[kBlkStride], int q, int sub, const uint2 *tw, uint32_t p)
+// DIF butterfly on canonical words; tw index 0 of every span table is w^0 = 1.
+__device__ __forceinline__ void dif_bfly(uint32_t &A, uint32_t &B, const uint2 t, bool trivial, uint32_t p)
 {
-    constexpr int N = 1 << L;
-    uint32_t x[16];
+    const uint32_t a = A, b = B;
+    A = add_mod(a, b, p);
+    B = trivial ? sub_mod(a, b, p) : shoup_mul(a + (p - b), t.x, t.y, p);
+}
+
+__device__ __forceinline__ void dit_bfly(uint32_t &A, uint32_t &B, const uint2 t, bool trivial, uint32_t p)
+{
+    const uint32_t a = A;
+    const uint32_t u = trivial ? B : shoup_mul(B, t.x, t.y, p);
+    A = add_mod(a, u, p);
+    B = sub_mod(a, u, p);
+}
+
+// The last BL DIF stages of one block row (spans 2^(BL-1) .. 1).  On return x
+// holds the group mapping; the row in LDS still holds the stride mapping.
+template <int L, int BL>
+__device__ __forceinline__ void dif_block_stages(uint32_t *row, int sub, const uint2 *__restrict__ tw, uint32_t p,
+                                                 uint32_t (&x)[16])
+{
+    constexpr int N = 1 << L, TPE = Blk<BL>::TPE;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) x[v] = sm[q][sub + 4 * v];
+    for (int v = 0; v < 16; ++v) x[v] = row[sub + TPE * v];
 #pragma unroll
-    for (int ls = 0; ls < 4; ++ls) {  // local spans 32, 16, 8, 4
-        const int s = L - 6 + ls;
+    for (int ls = 0; ls < 4; ++ls) {  // spans 8*TPE .. TPE
+        const int s = L - BL + ls;
         const int half = 8 >> ls;
         const uint2 *tws = tw + (N - (N >> s));
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             if (v & half) continue;
-            const uint2 t = tws[sub + 4 * (v & (half - 1))];
-            const uint32_t a = x[v], b = x[v + half];
-            x[v] = add_mod(a, b, p);
-            x[v + half] = shoup_mul(a + (p - b), t.x, t.y, p);
+            dif_bfly(x[v], x[v + half], tws[sub + TPE * (v & (half - 1))], false, p);
         }
     }
 #pragma unroll
-    for (int v = 0; v < 16; ++v) sm[q][sub + 4 * v] = x[v];
+    for (int v = 0; v < 16; ++v) row[sub + TPE * v] = x[v];
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 16 / TPE; ++u)
 #pragma unroll
-        for (int w = 0; w < 4; ++w) x[4 * u + w] = sm[q][4 * (sub + 4 * u) + w];
-    const uint2 t1 = tw[N - 4 + 1];  // span 2: twiddles {1, w^(N/4)}
+        for (int w = 0; w < TPE; ++w) x[u * TPE + w] = row[TPE * (sub + TPE * u) + w];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        uint32_t *y = x + 4 * u;
-        // span 2
-        uint32_t a = y[0], b = y[2];
-        y[0] = add_mod(a, b, p);
-        y[2] = sub_mod(a, b, p);
-        a = y[1], b = y[3];
-        y[1] = add_mod(a, b, p);
-        y[3] = shoup_mul(a + (p - b), t1.x, t1.y, p);
-        // span 1
+    for (int h = TPE / 2; h >= 1; h >>= 1) {  // spans TPE/2 .. 1, twiddles uniform across threads
+        const uint2 *tws = tw + (N - 2 * h);
 #pragma unroll
-        for (int w = 0; w < 4; w += 2) {
-            a = y[w], b = y[w + 1];
-            y[w] = add_mod(a, b, p);
-            y[w + 1] = sub_mod(a, b, p);
-        }
+        for (int u = 0; u < 16 / TPE; ++u)
+#pragma unroll
+            for (int w = 0; w < TPE; ++w) {
+                if (w & h) continue;
+                const int ti = w & (h - 1);
+                dif_bfly(x[u * TPE + w], x[u * TPE + w + h], ti ? tws[ti] : make_uint2(1, 0), ti == 0, p);
+            }
     }
-    // each thread rewrites only the positions it read in this pass
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) sm[q][4 * (sub + 4 * u) + w] = to_limbs(x[4 * u + w], p);
 }
 
-// Last six DIF stages of block blockIdx.x for the 64 entries of limb row
+// Last BL DIF stages of block blockIdx.x for the 64 entries of limb row
 // blockIdx.z, inner chunk blockIdx.y; writes that 256-byte run of every
 // point's limb block.
-template <int L>
-__global__ void __launch_bounds__(256) dif_tail_limbs_kernel(const uint32_t *__restrict__ X, uint8_t *__restrict__ Y,
-                                                             int rows_valid, int row0, int kdim,
-                                                             int row_major, const uint2 *__restrict__ tw, uint32_t p,
-                                                             size_t block_bytes, int kchunks)
+template <int L, int BL>
+__global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32_t *__restrict__ X,
+                                                                    uint8_t *__restrict__ Y, int rows_valid, int row0,
+                                                                    int kdim, int row_major,
+                                                                    const uint2 *__restrict__ tw, uint32_t p,
+                                                                    size_t block_bytes, int kchunks)
 {
-    constexpr int N = 1 << L;
-    __shared__ uint32_t sm[kBlk][kBlkStride];
+    using B = Blk<BL>;
+    constexpr int N = 1 << L, P = B::P, T = B::T, CPR = P / 4;  // 16-byte chunks per block row
+    extern __shared__ uint32_t sm[];
     const int tid = threadIdx.x;
     const int b = blockIdx.x, c = blockIdx.y, r = blockIdx.z;  // r: row within the chunk
-    {   // 16 lanes per entry row read its 256-byte block run
-        const int ch = tid & 15;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int q = (tid >> 4) + 16 * u;
-            const int kk = c * kBlk + q;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (r < rows_valid && kk < kdim) {
-                const size_t e = row_major ? (size_t)r * kdim + kk : (size_t)kk * rows_valid + r;
-                v = *reinterpret_cast<const uint4 *>(X + e * N + (size_t)b * kBlk + ch * 4);
-            }
-            sm[q][ch * 4] = v.x;
-            sm[q][ch * 4 + 1] = v.y;
-            sm[q][ch * 4 + 2] = v.z;
-            sm[q][ch * 4 + 3] = v.w;
+    for (int u = 0; u < 4; ++u) {  // CPR lanes per entry row read its 4P-byte block run
+        const int id = tid + T * u;
+        const int q = id / CPR, ch = id % CPR;
+        const int kk = c * 64 + q;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (r < rows_valid && kk < kdim) {
+            const size_t e = row_major ? (size_t)r * kdim + kk : (size_t)kk * rows_valid + r;
+            v = *reinterpret_cast<const uint4 *>(X + e * N + (size_t)b * P + ch * 4);
         }
+        uint32_t *d = sm + q * B::STRIDE + ch * 4;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
     }
     __syncthreads();
-    dif_block_stages<L>(sm, tid >> 2, tid & 3, tw, p);
+    const int q = tid / B::TPE, sub = tid % B::TPE;
+    uint32_t *row = sm + q * B::STRIDE;
+    uint32_t x[16];
+    dif_block_stages<L, BL>(row, sub, tw, p, x);
+    // each thread rewrites only the positions it read in the group mapping
+#pragma unroll
+    for (int u = 0; u < 16 / B::TPE; ++u)
+#pragma unroll
+        for (int w = 0; w < B::TPE; ++w) row[B::TPE * (sub + B::TPE * u) + w] = to_limbs(x[u * B::TPE + w], p);
     __syncthreads();
-    // point t = b*64 + lam: word w of the run = plane w>>4, entries 4(w&15)..+3
+    // point t = b*P + lam: word w of the run = plane w>>4, entries 4(w&15)..+3
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
-        const int lam = it * 4 + (tid >> 6), w = tid & 63;
+        const int id = tid + T * it;
+        const int lam = id >> 6, w = id & 63;
         const int sft = 8 * (w >> 4), q4 = (w & 15) * 4;
-        const uint32_t word = ((sm[q4][lam] >> sft) & 0xffu) | (((sm[q4 + 1][lam] >> sft) & 0xffu) << 8) |
-                              (((sm[q4 + 2][lam] >> sft) & 0xffu) << 16) | (((sm[q4 + 3][lam] >> sft) & 0xffu) << 24);
-        const size_t t = (size_t)b * kBlk + lam;
+        const uint32_t *col = sm + q4 * B::STRIDE + lam;
+        const uint32_t word = ((col[0] >> sft) & 0xffu) | (((col[B::STRIDE] >> sft) & 0xffu) << 8) |
+                              (((col[2 * B::STRIDE] >> sft) & 0xffu) << 16) |
+                              (((col[3 * B::STRIDE] >> sft) & 0xffu) << 24);
+        const size_t t = (size_t)b * P + lam;
         *reinterpret_cast<uint32_t *>(Y + t * block_bytes + ((size_t)(row0 + r) * kchunks + c) * 256 + (w >> 4) * 64 +
                                       (w & 15) * 4) = word;
     }
 }
 
-// First six DIT stages of block blockIdx.x for entries 64*blockIdx.y .. +63,
+// First BL DIT stages of block blockIdx.x for entries 64*blockIdx.y .. +63,
 // read from the point-major result C[t][e]; writes polfirst scratch.
-template <int L>
-__global__ void __launch_bounds__(256) dit_head_kernel(const uint32_t *__restrict__ Cm, size_t entries,
-                                                       size_t e_first, size_t e_count, uint32_t *__restrict__ Y,
-                                                       const uint2 *__restrict__ tw, uint32_t p)
+template <int L, int BL>
+__global__ void __launch_bounds__(Blk<BL>::T) dit_head_kernel(const uint32_t *__restrict__ Cm, size_t entries,
+                                                              size_t e_first, size_t e_count,
+                                                              uint32_t *__restrict__ Y, const uint2 *__restrict__ tw,
+                                                              uint32_t p)
 {
-    constexpr int N = 1 << L;
-    __shared__ uint32_t sm[kBlk][kBlkStride];
+    using B = Blk<BL>;
+    constexpr int N = 1 << L, P = B::P, T = B::T, TPE = B::TPE, CPR = P / 4;
+    extern __shared__ uint32_t sm[];
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
-    const size_t e0 = (size_t)blockIdx.y * kBlk;  // relative to e_first
+    const size_t e0 = (size_t)blockIdx.y * 64;  // relative to e_first
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {  // 64 lanes read 64 consecutive entries of one point
-        const int q = tid & 63, lam = (tid >> 6) + 4 * it;
-        const size_t t = (size_t)b * kBlk + lam;
-        sm[q][lam] = (e0 + q < e_count) ? Cm[t * entries + e_first + e0 + q] : 0u;
+        const int id = tid + T * it;
+        const int q = id & 63, lam = id >> 6;
+        const size_t t = (size_t)b * P + lam;
+        sm[q * B::STRIDE + lam] = (e0 + q < e_count) ? Cm[t * entries + e_first + e0 + q] : 0u;
     }
     __syncthreads();
-    const int q = tid >> 2, sub = tid & 3;
+    const int q = tid / TPE, sub = tid % TPE;
+    uint32_t *row = sm + q * B::STRIDE;
     uint32_t x[16];
-    // pass 1: spans 1 and 2 on groups of four consecutive positions
+    // spans 1 .. TPE/2 on TPE-word groups, twiddles uniform across threads
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 16 / TPE; ++u)
 #pragma unroll
-        for (int w = 0; w < 4; ++w) x[4 * u + w] = sm[q][4 * (sub + 4 * u) + w];
-    const uint2 t1 = tw[N - 4 + 1];
+        for (int w = 0; w < TPE; ++w) x[u * TPE + w] = row[TPE * (sub + TPE * u) + w];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        uint32_t *y = x + 4 * u;
+    for (int h = 1; h < TPE; h <<= 1) {
+        const uint2 *tws = tw + (N - 2 * h);
 #pragma unroll
-        for (int w = 0; w < 4; w += 2) {
-            const uint32_t a = y[w], bb = y[w + 1];
-            y[w] = add_mod(a, bb, p);
-            y[w + 1] = sub_mod(a, bb, p);
-        }
-        uint32_t a = y[0], tt = y[2];
-        y[0] = add_mod(a, tt, p);
-        y[2] = sub_mod(a, tt, p);
-        a = y[1];
-        tt = shoup_mul(y[3], t1.x, t1.y, p);
-        y[1] = add_mod(a, tt, p);
-        y[3] = sub_mod(a, tt, p);
+        for (int u = 0; u < 16 / TPE; ++u)
+#pragma unroll
+            for (int w = 0; w < TPE; ++w) {
+                if (w & h) continue;
+                const int ti = w & (h - 1);
+                dit_bfly(x[u * TPE + w], x[u * TPE + w + h], ti ? tws[ti] : make_uint2(1, 0), ti == 0, p);
+            }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 16 / TPE; ++u)
 #pragma unroll
-        for (int w = 0; w < 4; ++w) sm[q][4 * (sub + 4 * u) + w] = x[4 * u + w];
+        for (int w = 0; w < TPE; ++w) row[TPE * (sub + TPE * u) + w] = x[u * TPE + w];
     __syncthreads();
-    // pass 2: spans 4..32 on positions sub + 4v
+    // spans TPE .. 8*TPE on the words sub + TPE*v
 #pragma unroll
-    for (int v = 0; v < 16; ++v) x[v] = sm[q][sub + 4 * v];
+    for (int v = 0; v < 16; ++v) x[v] = row[sub + TPE * v];
 #pragma unroll
-    for (int d = 2; d < 6; ++d) {
-        const int half = 1 << (d - 2);
-        const uint2 *tws = tw + (N - (2 << d));
+    for (int ds = 0; ds < 4; ++ds) {
+        const int h = TPE << ds, half = 1 << ds;
+        const uint2 *tws = tw + (N - 2 * h);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             if (v & half) continue;
-            const uint2 w = tws[sub + 4 * (v & (half - 1))];
-            const uint32_t a = x[v];
-            const uint32_t tt = shoup_mul(x[v + half], w.x, w.y, p);
-            x[v] = add_mod(a, tt, p);
-            x[v + half] = sub_mod(a, tt, p);
+            dit_bfly(x[v], x[v + half], tws[sub + TPE * (v & (half - 1))], false, p);
         }
     }
 #pragma unroll
-    for (int v = 0; v < 16; ++v) sm[q][sub + 4 * v] = x[v];
+    for (int v = 0; v < 16; ++v) row[sub + TPE * v] = x[v];
     __syncthreads();
-    {   // 16 lanes per entry write its 256-byte run
-        const int ch = tid & 15;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int qq = (tid >> 4) + 16 * u;
-            if (e0 + qq < e_count)
-                *reinterpret_cast<uint4 *>(Y + (e0 + qq) * N + (size_t)b * kBlk + ch * 4) =
-                    make_uint4(sm[qq][ch * 4], sm[qq][ch * 4 + 1], sm[qq][ch * 4 + 2], sm[qq][ch * 4 + 3]);
+    for (int u = 0; u < 4; ++u) {  // CPR lanes per entry write its 4P-byte run
+        const int id = tid + T * u;
+        const int qq = id / CPR, ch = id % CPR;
+        if (e0 + qq < e_count) {
+            const uint32_t *src = sm + qq * B::STRIDE + ch * 4;
+            *reinterpret_cast<uint4 *>(Y + (e0 + qq) * N + (size_t)b * P + ch * 4) =
+                make_uint4(src[0], src[1], src[2], src[3]);
         }
     }
 }
@@ -351,70 +367,104 @@ hipError_t launch_ntt_inverse(const uint32_t *in, size_t entries, unsigned L, co
     }
 }
 
-hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const EntryMap &map, unsigned L,
+// Block sizes 2^BL of the split transforms: the kernels are written for
+// BL in [6, 8]; BL = 6 is instantiated (7 and 8 measured slower at 2^14 points:
+// what the per-entry kernels save, the blockwise ones lose), with L in [8, 15].
+static bool split_ok(unsigned L, unsigned BL) { return BL == kSplitBL && L >= BL + 2 && L <= 15; }
+
+hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const EntryMap &map, unsigned L, unsigned BL,
                                    const uint2 *tw, uint32_t p, uint32_t *out, hipStream_t s)
 {
     if (entries == 0) return hipSuccess;
-    switch (L) {
-#define LBX_CASE(l) \
-    case l: return fwd_launch<l, l - 6>(in, entries, map, tw, p, out, s);
-        LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11) LBX_CASE(12) LBX_CASE(13) LBX_CASE(14) LBX_CASE(15)
+    if (!split_ok(L, BL)) return hipErrorInvalidValue;
+    switch (L * 16 + BL) {
+#define LBX_CASE(l, bl) \
+    case l * 16 + bl: return fwd_launch<l, l - bl>(in, entries, map, tw, p, out, s);
+        LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
+        LBX_CASE(15, 6)
 #undef LBX_CASE
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
-                                   const DevView &out, size_t out_first, size_t count, uint32_t scale, hipStream_t s)
+hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned L, unsigned BL, const uint2 *tw,
+                                   uint32_t p, const DevView &out, size_t out_first, size_t count, uint32_t scale,
+                                   hipStream_t s)
 {
     if (entries == 0) return hipSuccess;
-    switch (L) {
-#define LBX_CASE(l) \
-    case l: return inv_launch<l, 6>(in, entries, tw, p, out, out_first, count, scale, s);
-        LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11) LBX_CASE(12) LBX_CASE(13) LBX_CASE(14) LBX_CASE(15)
+    if (!split_ok(L, BL)) return hipErrorInvalidValue;
+    switch (L * 16 + BL) {
+#define LBX_CASE(l, bl) \
+    case l * 16 + bl: return inv_launch<l, bl>(in, entries, tw, p, out, out_first, count, scale, s);
+        LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
+        LBX_CASE(15, 6)
 #undef LBX_CASE
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int L, int BL>
+static hipError_t tail_launch(dim3 grid, const uint32_t *X, uint8_t *Y, int rows_valid, int row0, int kdim,
+                              int row_major, const uint2 *tw, uint32_t p, size_t block_bytes, int kchunks,
+                              hipStream_t s)
+{
+    auto k = dif_tail_limbs_kernel<L, BL>;
+    hipError_t e = allow_lds(k, Blk<BL>::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, grid, dim3(Blk<BL>::T), Blk<BL>::LDS_BYTES, s, X, Y, rows_valid, row0, kdim, row_major,
+                       tw, p, block_bytes, kchunks);
+    return hipGetLastError();
+}
+
+template <int L, int BL>
+static hipError_t dit_head_launch(dim3 grid, const uint32_t *Cm, size_t entries, size_t e_first, size_t e_count,
+                                  uint32_t *Y, const uint2 *tw, uint32_t p, hipStream_t s)
+{
+    auto k = dit_head_kernel<L, BL>;
+    hipError_t e = allow_lds(k, Blk<BL>::LDS_BYTES);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, grid, dim3(Blk<BL>::T), Blk<BL>::LDS_BYTES, s, Cm, entries, e_first, e_count, Y, tw, p);
+    return hipGetLastError();
 }
 
 hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows_total, size_t row0, size_t rows_chunk,
-                                 size_t rows_valid, size_t kdim, bool row_major, unsigned L, const uint2 *tw,
-                                 uint32_t p, hipStream_t s)
+                                 size_t rows_valid, size_t kdim, bool row_major, unsigned L, unsigned BL,
+                                 const uint2 *tw, uint32_t p, hipStream_t s)
 {
     const size_t rpad = (rows_total + 31) & ~(size_t)31, kchunks = (kdim + 63) / 64;
     if (rows_chunk == 0) return hipSuccess;
-    if (row0 + rows_chunk > rpad || rows_valid > rows_chunk || rows_chunk > 65535 || kchunks > 65535 || L < 8 ||
-        L > 15)
+    if (row0 + rows_chunk > rpad || rows_valid > rows_chunk || rows_chunk > 65535 || kchunks > 65535 ||
+        !split_ok(L, BL))
         return hipErrorInvalidValue;
     const size_t block_bytes = rpad * kchunks * 256;
-    dim3 grid((unsigned)((size_t)1 << (L - 6)), (unsigned)kchunks, (unsigned)rows_chunk);
-    switch (L) {
-#define LBX_CASE(l)                                                                                                  \
-    case l:                                                                                                          \
-        hipLaunchKernelGGL(dif_tail_limbs_kernel<l>, grid, dim3(256), 0, s, X, Y, (int)rows_valid, (int)row0,       \
-                           (int)kdim, row_major ? 1 : 0, tw, p, block_bytes, (int)kchunks);                          \
-        break;
-        LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11) LBX_CASE(12) LBX_CASE(13) LBX_CASE(14) LBX_CASE(15)
+    dim3 grid((unsigned)((size_t)1 << (L - BL)), (unsigned)kchunks, (unsigned)rows_chunk);
+    switch (L * 16 + BL) {
+#define LBX_CASE(l, bl)                                                                                            \
+    case l * 16 + bl:                                                                                              \
+        return tail_launch<l, bl>(grid, X, Y, (int)rows_valid, (int)row0, (int)kdim, row_major ? 1 : 0, tw, p,     \
+                                  block_bytes, (int)kchunks, s);
+        LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
+        LBX_CASE(15, 6)
 #undef LBX_CASE
+    default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_dit_head(const uint32_t *Cm, size_t entries, size_t e_first, size_t e_count, unsigned L,
-                           const uint2 *tw, uint32_t p, uint32_t *Y, hipStream_t s)
+                           unsigned BL, const uint2 *tw, uint32_t p, uint32_t *Y, hipStream_t s)
 {
     if (e_count == 0) return hipSuccess;
     const size_t groups = (e_count + 63) / 64;
-    if (groups > 65535 || L < 8 || L > 15 || e_first + e_count > entries) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((size_t)1 << (L - 6)), (unsigned)groups);
-    switch (L) {
-#define LBX_CASE(l)                                                                                               \
-    case l: hipLaunchKernelGGL(dit_head_kernel<l>, grid, dim3(256), 0, s, Cm, entries, e_first, e_count, Y, tw, p); \
-        break;
-        LBX_CASE(8) LBX_CASE(9) LBX_CASE(10) LBX_CASE(11) LBX_CASE(12) LBX_CASE(13) LBX_CASE(14) LBX_CASE(15)
+    if (groups > 65535 || !split_ok(L, BL) || e_first + e_count > entries) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((size_t)1 << (L - BL)), (unsigned)groups);
+    switch (L * 16 + BL) {
+#define LBX_CASE(l, bl) \
+    case l * 16 + bl: return dit_head_launch<l, bl>(grid, Cm, entries, e_first, e_count, Y, tw, p, s);
+        LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
+        LBX_CASE(15, 6)
 #undef LBX_CASE
+    default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 size_t batch_ntt_lds_bytes()

@@ -52,14 +52,35 @@ __device__ __forceinline__ int elem(int tp, int u, int q)
     return (blk << (HL + RP)) + off + (q << HL);
 }
 
+// Padded LDS word of slot (u,q): for HL >= 4, pad(base + q*2^HL) =
+// pad(base) + q*(2^HL + 2^(HL-4)), so one address per group and immediate
+// offsets for its 2^RP words.
+template <int L, int HL, int RP>
+__device__ __forceinline__ int slot_base(int tp, int u)
+{
+    return pad(elem<L, HL, RP>(tp, u, 0));
+}
+template <int HL>
+constexpr int slot_step()
+{
+    return (1 << HL) + (HL >= 4 ? (1 << (HL - 4)) : 0);
+}
+
 template <int L, int HL, int RP>
 __device__ __forceinline__ void scatter(const uint32_t (&x)[Cfg<L>::R], uint32_t *lds, int tp)
 {
     constexpr int NG = Cfg<L>::R >> RP;
 #pragma unroll
-    for (int u = 0; u < NG; ++u)
+    for (int u = 0; u < NG; ++u) {
+        if constexpr (HL >= 4) {
+            uint32_t *b = lds + slot_base<L, HL, RP>(tp, u);
 #pragma unroll
-        for (int q = 0; q < (1 << RP); ++q) lds[pad(elem<L, HL, RP>(tp, u, q))] = x[u * (1 << RP) + q];
+            for (int q = 0; q < (1 << RP); ++q) b[q * slot_step<HL>()] = x[u * (1 << RP) + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < (1 << RP); ++q) lds[pad(elem<L, HL, RP>(tp, u, q))] = x[u * (1 << RP) + q];
+        }
+    }
 }
 
 template <int L, int HL, int RP>
@@ -67,9 +88,16 @@ __device__ __forceinline__ void gather(uint32_t (&x)[Cfg<L>::R], const uint32_t 
 {
     constexpr int NG = Cfg<L>::R >> RP;
 #pragma unroll
-    for (int u = 0; u < NG; ++u)
+    for (int u = 0; u < NG; ++u) {
+        if constexpr (HL >= 4) {
+            const uint32_t *b = lds + slot_base<L, HL, RP>(tp, u);
 #pragma unroll
-        for (int q = 0; q < (1 << RP); ++q) x[u * (1 << RP) + q] = lds[pad(elem<L, HL, RP>(tp, u, q))];
+            for (int q = 0; q < (1 << RP); ++q) x[u * (1 << RP) + q] = b[q * slot_step<HL>()];
+        } else {
+#pragma unroll
+            for (int q = 0; q < (1 << RP); ++q) x[u * (1 << RP) + q] = lds[pad(elem<L, HL, RP>(tp, u, q))];
+        }
+    }
 }
 
 // DIF stages S0..S0+RP-1 (span N>>(s+1) at stage s) on the registers.

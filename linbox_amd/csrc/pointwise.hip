@@ -22,6 +22,8 @@
 //    The digit split happens once per element in launch_to_limbs, which also
 //    transposes the NTT output into per-point blocks laid out for direct
 //    16-byte operand loads.
+#include <cstdlib>
+
 #include "lbx_internal.h"
 
 namespace lbx {
@@ -166,7 +168,7 @@ __global__ void __launch_bounds__(256) to_limbs_kernel(const uint32_t *__restric
 
 // Constants of the MFMA epilogue.
 struct MfmaConst {
-    int64_t cd[7];    // 2^(8d) * scale mod p
+    int32_t cd[7];    // 2^(8d) * scale mod p (< p < 2^31, so a signed 32-bit value)
     uint64_t off;     // multiple of p >= 2^62
     uint32_t r32, r32p;  // 2^32 mod p and its Shoup constant
     uint32_t onep;       // floor(2^32/p): Shoup constant of 1
@@ -176,7 +178,8 @@ __device__ __forceinline__ uint32_t mfma_combine(const v4i (&acc)[7], int r, uin
 {
     int64_t S = 0;
 #pragma unroll
-    for (int d = 0; d < 7; ++d) S += (int64_t)acc[d][r] * mc.cd[d];
+    // 32 x 32 -> 64-bit signed products (one v_mad_i64_i32 each)
+    for (int d = 0; d < 7; ++d) S += (int64_t)acc[d][r] * (int64_t)mc.cd[d];
     const uint64_t u = (uint64_t)S + mc.off;
     const uint32_t hi = (uint32_t)(u >> 32), lo = (uint32_t)u;
     return add_mod(shoup_mul(hi, mc.r32, mc.r32p, p), shoup_mul(lo, 1u, mc.onep, p), p);
@@ -444,7 +447,7 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
     MfmaConst mc;
     uint64_t pw = scale % p;
     for (int d = 0; d < 7; ++d) {
-        mc.cd[d] = (int64_t)pw;
+        mc.cd[d] = (int32_t)pw;
         pw = pw * 256 % p;
     }
     const uint64_t two62 = (uint64_t)1 << 62;
@@ -455,14 +458,14 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
     const size_t mpad = rpad_of(m), npad = rpad_of(n), kchunks = kpad_of(k) / 64;
     if (npts > 0x7fffffff) return hipErrorInvalidValue;
     const size_t a_bytes = limb_block_bytes(m, k);
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
     if (a_bytes <= (size_t)kLdsAMax) {
-        static int cus = 0;
-        if (!cus) {
-            int dev = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-                cus = 256;
-        }
         const size_t lds = a_bytes + (size_t)32 * kchunks * 256;
         hipError_t e = hipFuncSetAttribute((const void *)pw_mfma_lds_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -164,6 +164,44 @@ def test_mul_chunked_pipeline(ctx, chunk_bytes, monkeypatch):
         assert np.array_equal(got.reshape(-1), ref)
 
 
+def test_mul_split_all_lengths(ctx, monkeypatch):
+    """Split transforms (forward head + blockwise tail, blockwise inverse head +
+    per-entry tail) at every transform length from 2^8 to 2^15, with the
+    head -> tail chunking off and on."""
+    cases = [(998244353, 16, 16, 16, 129, 128), (2013265921, 17, 33, 19, 300, 213),
+             (469762049, 32, 20, 40, 600, 500), (2013265921, 16, 17, 18, 1025, 1000),
+             (998244353, 20, 16, 16, 2048, 2049), (2013265921, 16, 16, 16, 4096, 4000),
+             (2013265921, 16, 16, 17, 8192, 8192), (469762049, 16, 16, 16, 16384, 16384)]
+    for (p, m, k, n, sa, sb) in cases:
+        rng = np.random.default_rng(m * n * sa)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        assert st == 0
+        for chunk in (None, 1 << 20):
+            if chunk:
+                monkeypatch.setenv("LBX_CHUNK_BYTES", str(chunk))
+            else:
+                monkeypatch.delenv("LBX_CHUNK_BYTES", raising=False)
+            say("split", p, m, k, n, sa, sb, chunk)
+            got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, lbx.METHOD_INT8_MFMA)
+            assert np.array_equal(got.reshape(-1), ref)
+
+
+def test_mul_mfma_shapes(ctx):
+    """INT8 pointwise stage on shapes with partial row bands and column blocks,
+    several inner-dimension chunks, and fewer points than workgroups."""
+    for (p, m, k, n, sa, sb) in [(2013265921, 33, 100, 70, 200, 57), (998244353, 128, 128, 128, 40, 41),
+                                 (469762049, 100, 64, 130, 300, 300), (2013265921, 16, 200, 16, 130, 127),
+                                 (998244353, 97, 17, 33, 5, 4)]:
+        rng = np.random.default_rng(m * k + n)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        assert st == 0
+        say("mfma shape", p, m, k, n, sa, sb)
+        got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, lbx.METHOD_INT8_MFMA)
+        assert np.array_equal(got.reshape(-1), ref)
+
+
 def test_mul_config1_vs_naive(ctx):
     """16x16.16x16 over 998244353, degree 255: bit-exact vs naive convolution."""
     p, m, k, n, sa, sb = 998244353, 16, 16, 16, 256, 256
