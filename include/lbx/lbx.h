@@ -142,6 +142,30 @@ typedef struct lbx_job {
  * lbx_polymat_mul_u32 per job. */
 lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream);
 
+/* Middle product, PolynomialMatrixFFTMulDomain::midproduct
+ * (matpoly-mult-fft-wordsize.inl:90-127 -> matpoly-mult-fft-wordsize-fast.inl:209-316;
+ * multiprecision route :93-117,427-520): hdeg = n0 ? n0 : c->size,
+ * deg = n1 ? n1 : 2*hdeg-1, pts = the least power of two >= deg.  c->size
+ * coefficients are written: the cyclic correlation, over pts points, of the
+ * small operand (a if small_left, else b) with its first hdeg coefficients
+ * reversed, and the other operand; coefficients 0..deg-hdeg of it are
+ * coefficients hdeg-1..deg-1 of a.b.  c->size is not changed.  Requires
+ * c->size >= deg-hdeg, small operand size <= hdeg, other operand size
+ * < hdeg+deg (wordsize-fast.inl:214-218), every size <= pts (else
+ * LBX_ERR_SIZE_MISMATCH); deg <= 1 gives LBX_ERR_K_NOT_POSITIVE.  Every p < 2^32:
+ * single FFT prime when p < 2^31 and pts | p-1, FFT primes and CRT otherwise. */
+lbx_status lbx_polymat_midproduct_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
+                                      const lbx_polmat *b, int small_left, size_t n0, size_t n1, int method,
+                                      void *stream);
+
+/* PolynomialMatrixNaiveMulDomain::midproduct (matpoly-mult-naive.h:85-125),
+ * the route of PolynomialMatrixMulDomain::midproductgen for c.size() <= 4
+ * (polynomial-matrix-domain.h:95-103): c[kk-hdeg+1] = coefficient kk of a.b
+ * for hdeg-1 <= kk < min(a.size+b.size-1, deg); other coefficients of c are
+ * left unchanged. */
+lbx_status lbx_polymat_midproduct_naive_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
+                                            const lbx_polmat *b, size_t n0, size_t n1, void *stream);
+
 /* FFT_PROFILER analogue (matpoly-mult-fft.h:43-77): with enable = 1, calls
  * record device events between their stages, labelled as in mul_fft
  * ("direct FFT_DIF", "Polfirst to Matfirst", "Pointwise mult",

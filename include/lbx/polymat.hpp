@@ -190,6 +190,18 @@ public:
         c.after_mul(dc);
     }
 
+    // c = (a*b x^-(hdeg-1)) mod x^pts on c.size() coefficients, hdeg = n0 ? n0 : c.size(),
+    // deg = n1 ? n1 : 2*hdeg-1 (matpoly-mult-fft-wordsize.inl:90-127); c keeps its size.
+    template <typename Matrix1, typename Matrix2, typename Matrix3>
+    void midproduct(Matrix1 &c, const Matrix2 &a, const Matrix3 &b, bool smallLeft = true, size_t n0 = 0,
+                    size_t n1 = 0) const
+    {
+        if (a.coldim() != b.rowdim()) throw LinboxError(LBX_ERR_SIZE_MISMATCH, "a.coldim() != b.rowdim()");
+        lbx_polmat dc = c.desc(), da = a.desc(), db = b.desc();
+        check(lbx_polymat_midproduct_u32(ctx_, (uint32_t)fld_->cardinality(), &dc, &da, &db, smallLeft ? 1 : 0, n0,
+                                         n1, method_, stream_));
+    }
+
 private:
     const Field *fld_;
     lbx_ctx *ctx_;
@@ -197,15 +209,39 @@ private:
     void *stream_;
 };
 
-// PolynomialMatrixMulDomain::mul: d = a.size()+b.size() > 2 takes the FFT
-// domain, otherwise the naive one; both run on the device here.  Like the
-// reference (polynomial-matrix-domain.h:61), max_rowdeg is not forwarded.
+// PolynomialMatrixNaiveMulDomain::midproduct (matpoly-mult-naive.h:85-125).
+class PolynomialMatrixNaiveMulDomain {
+public:
+    typedef Modular Field;
+    explicit PolynomialMatrixNaiveMulDomain(const Field &F, lbx_ctx *ctx = nullptr, void *stream = nullptr)
+        : fld_(&F), ctx_(ctx ? ctx : default_ctx()), stream_(stream) {}
+    template <typename Matrix1, typename Matrix2, typename Matrix3>
+    void midproduct(Matrix1 &c, const Matrix2 &a, const Matrix3 &b, bool smallLeft = true, size_t n0 = 0,
+                    size_t n1 = 0) const
+    {
+        (void)smallLeft;  // both branches of :91-123 sum the same coefficient
+        if (a.coldim() != b.rowdim()) throw LinboxError(LBX_ERR_SIZE_MISMATCH, "a.coldim() != b.rowdim()");
+        lbx_polmat dc = c.desc(), da = a.desc(), db = b.desc();
+        check(lbx_polymat_midproduct_naive_u32(ctx_, (uint32_t)fld_->cardinality(), &dc, &da, &db, n0, n1, stream_));
+    }
+
+private:
+    const Field *fld_;
+    lbx_ctx *ctx_;
+    void *stream_;
+};
+
+// PolynomialMatrixMulDomain (polynomial-matrix-domain.h:40-104).  mul:
+// d = a.size()+b.size() > 2 takes the FFT domain, otherwise the naive one;
+// both run on the device here.  Like the reference (:61), max_rowdeg is not
+// forwarded.  midproduct: FFT when b.size() > 2, naive otherwise (:77-91);
+// midproductgen: naive when c.size() <= 4, FFT otherwise (:94-103).
 class PolynomialMatrixMulDomain {
 public:
     typedef Modular Field;
     explicit PolynomialMatrixMulDomain(const Field &F, lbx_ctx *ctx = nullptr, int method = LBX_METHOD_AUTO,
                                        void *stream = nullptr)
-        : fft_(F, ctx, method, stream) {}
+        : fft_(F, ctx, method, stream), naive_(F, ctx, stream) {}
     const Field &field() const { return fft_.field(); }
     template <typename Matrix1, typename Matrix2, typename Matrix3>
     void mul(Matrix1 &c, const Matrix2 &a, const Matrix3 &b, size_t max_rowdeg = 0) const
@@ -213,9 +249,27 @@ public:
         (void)max_rowdeg;
         fft_.mul(c, a, b);
     }
+    template <typename Matrix1, typename Matrix2, typename Matrix3>
+    void midproduct(Matrix1 &c, const Matrix2 &a, const Matrix3 &b) const
+    {
+        if (b.size() > 2)
+            fft_.midproduct(c, a, b);
+        else
+            naive_.midproduct(c, a, b);
+    }
+    template <typename Matrix1, typename Matrix2, typename Matrix3>
+    void midproductgen(Matrix1 &c, const Matrix2 &a, const Matrix3 &b, bool smallLeft = true, size_t n0 = 0,
+                       size_t n1 = 0) const
+    {
+        if (c.size() <= 4)
+            naive_.midproduct(c, a, b, smallLeft, n0, n1);
+        else
+            fft_.midproduct(c, a, b, smallLeft, n0, n1);
+    }
 
 private:
     PolynomialMatrixFFTMulDomain fft_;
+    PolynomialMatrixNaiveMulDomain naive_;
 };
 
 class PolynomialMatrixFFTPrimeMulDomain {

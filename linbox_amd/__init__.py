@@ -26,7 +26,7 @@ METHOD_AUTO, METHOD_PLAIN_U64, METHOD_INT8_MFMA = 0, 1, 2
 EXPORTED = [
     "lbx_init", "lbx_destroy", "lbx_strerror", "lbx_trim", "lbx_polymat_mul_u32", "lbx_polymat_mul_fft_u32",
     "lbx_ntt_forward_u32", "lbx_ntt_inverse_u32", "lbx_polymat_mul_mp", "lbx_polymat_mul_batch",
-    "lbx_profile_enable", "lbx_profile_get",
+    "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_profile_enable", "lbx_profile_get",
 ]
 
 
@@ -79,6 +79,8 @@ def lib():
         L.lbx_ntt_inverse_u32.argtypes = [vp, u32, ctypes.c_uint, u32, vp, sz, ctypes.POINTER(u32), vp]
         L.lbx_polymat_mul_mp.argtypes = [vp, vp, i32, PM, PM, PM, i32, vp]
         L.lbx_polymat_mul_batch.argtypes = [vp, ctypes.POINTER(Job), sz, i32, vp]
+        L.lbx_polymat_midproduct_u32.argtypes = [vp, u32, PM, PM, PM, i32, sz, sz, i32, vp]
+        L.lbx_polymat_midproduct_naive_u32.argtypes = [vp, u32, PM, PM, PM, sz, sz, vp]
         L.lbx_profile_enable.argtypes = [vp, i32]
         L.lbx_profile_get.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]
         L.lbx_profile_get.restype = i32
@@ -134,9 +136,10 @@ def default_context():
 
 
 from .polymat import (PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain,  # noqa: E402
-                      PolynomialMatrixFFTPrimeMulDomain, FFT, ntt_forward, ntt_inverse, mul_mp, mul_batch)
+                      PolynomialMatrixFFTPrimeMulDomain, PolynomialMatrixNaiveMulDomain, FFT, ntt_forward,
+                      ntt_inverse, mul_mp, mul_batch)
 
 __all__ = ["Context", "LinboxError", "PolynomialMatrix", "PolynomialMatrixMulDomain", "PolynomialMatrixFFTMulDomain",
-           "PolynomialMatrixFFTPrimeMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch",
+           "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch",
            "POLFIRST", "MATFIRST", "MATROWFIRST", "METHOD_AUTO", "METHOD_PLAIN_U64", "METHOD_INT8_MFMA", "lib",
            "build", "default_context", "EXPORTED"]

@@ -301,9 +301,9 @@ struct Pipe {
 
 // Forward transforms, pointwise products and inverse transform of one
 // single-prime product on device views.  deg+1 coefficients of c are written.
-lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
+lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a_in, const DevView &b_in,
                              const DevView &c, size_t m, size_t k, size_t n, size_t count, int method,
-                             hipStream_t s)
+                             hipStream_t s, const MidSpec *mid)
 {
     const size_t N = (size_t)1 << L;
     if (L > (unsigned)kMaxLdsLog) return LBX_ERR_UNSUPPORTED;
@@ -313,6 +313,22 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     const uint2 *twf, *twi;
     LBX_TRY(twiddles(ctx, p, L, w, &twf));
     LBX_TRY(twiddles(ctx, p, L, wi, &twi));
+    // product: both operands forward with w, inverse with w^-1; midproduct
+    // (wordsize-fast.inl:277-313): small operand with w, other with w^-1,
+    // inverse with w, small operand reversed over its first hdeg coefficients
+    DevView a = a_in, b = b_in;
+    const uint2 *twa = twf, *twb = twf, *twc = twi;
+    if (mid) {
+        twc = twf;
+        (mid->small_left ? twb : twa) = twi;
+        const DevView &sm = mid->small_left ? a_in : b_in;
+        const size_t se = mid->small_left ? m * k : k * n;
+        void *rev;
+        LBX_TRY(ensure(ctx, "mid_rev", se * mid->hdeg * 4, &rev));
+        LBX_TRY_HIP(launch_reverse_gather(sm, se, mid->hdeg, (uint32_t *)rev, s));
+        const DevView rv{(uint32_t *)rev, LBX_POLFIRST, sm.rows, sm.cols, mid->hdeg, 0, mid->hdeg};
+        (mid->small_left ? a : b) = rv;
+    }
     const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
     method = choose_method(method, m, k, n);
 
@@ -356,13 +372,13 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
                 if (valid) {
                     const EntryMap map = op == 0 ? EntryMap{r0 * k, valid * k, 0} : EntryMap{r0, valid, n};
                     KernelTimer kt(ctx, s, op == 0 ? "kernel dif_head a" : "kernel dif_head b");
-                    LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, BL, twf, p, buf, s));
+                    LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, BL, op == 0 ? twa : twb, p, buf, s));
                 }
                 LBX_TRY(fp.handoff(chunk));
                 {
                     KernelTimer kt(ctx, fp.consumer(), op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b");
                     LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
-                                                      op == 0, L, BL, twf, p, fp.consumer()));
+                                                      op == 0, L, BL, op == 0 ? twa : twb, p, fp.consumer()));
                 }
                 LBX_TRY(fp.done(chunk));
             }
@@ -384,12 +400,12 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
             LBX_TRY(ip.reuse(chunk));
             {
                 KernelTimer kt(ctx, s, "kernel dit_head");
-                LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, BL, twi, p, buf, s));
+                LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, BL, twc, p, buf, s));
             }
             LBX_TRY(ip.handoff(chunk));
             {
                 KernelTimer kt(ctx, ip.consumer(), "kernel dit_tail");
-                LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, BL, twi, p, c, e0, count, 1, ip.consumer()));
+                LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, BL, twc, p, c, e0, count, 1, ip.consumer()));
             }
             LBX_TRY(ip.done(chunk));
         }
@@ -403,11 +419,11 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     LBX_TRY(ensure(ctx, "chat", m * n * N * 4, &chat));
     {
         KernelTimer kt(ctx, s, "kernel dif a");
-        LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twf, p, (uint32_t *)ahat, s));
+        LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twa, p, (uint32_t *)ahat, s));
     }
     {
         KernelTimer kt(ctx, s, "kernel dif b");
-        LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twf, p, (uint32_t *)bhat, s));
+        LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twb, p, (uint32_t *)bhat, s));
     }
     st.mark("direct FFT_DIF");
     if (method == LBX_METHOD_INT8_MFMA) {
@@ -435,7 +451,7 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     }
     {
         KernelTimer kt(ctx, s, "kernel dit");
-        LBX_TRY_HIP(launch_ntt_inverse((const uint32_t *)chat, m * n, L, twi, p, c, count, 1, s));
+        LBX_TRY_HIP(launch_ntt_inverse((const uint32_t *)chat, m * n, L, twc, p, c, count, 1, s));
     }
     st.mark("inverse FFT_DIT");
     return LBX_OK;

@@ -87,10 +87,26 @@ struct Staged {
 lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out);
 void release(Staged &s);
 
-// Single-prime product on device views (count = deg+1 coefficients of c).
+// Midproduct variant of a transform product (matpoly-mult-fft-wordsize-fast.inl:209-316):
+// the first hdeg coefficients of the small operand (a if small_left) are
+// reversed, that operand is transformed with the root w and the other with
+// w^-1, and the inverse transform uses w.  Requires small operand size <= hdeg.
+struct MidSpec {
+    bool small_left;
+    size_t hdeg;
+};
+
+// Single-prime product on device views (count = deg+1 coefficients of c), or
+// with mid != nullptr the midproduct (count = c.size coefficients).
 lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
                              const DevView &c, size_t m, size_t k, size_t n, size_t count, int method,
-                             hipStream_t s);
+                             hipStream_t s, const MidSpec *mid = nullptr);
+// Product (mid == nullptr; c holds sa+sb-1 coefficients) or midproduct with
+// 2^L points over Z/PZ (P of nl little-endian limbs) through FFT primes and
+// CRT (matpoly-mult-fft-multiprecision.inl:141-277, 427-520).
+lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a, const DevView &b, const DevView &c,
+                      size_t m, size_t k, size_t n, int method, hipStream_t s, const MidSpec *mid = nullptr,
+                      unsigned mid_L = 0);
 // Product of constant matrices (one coefficient each).
 lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c, size_t m,
                             size_t k, size_t n, hipStream_t s);

@@ -152,6 +152,9 @@ hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t s);
 // out[e*stride + t] = view(e, t) for t < count (zero for t >= view.size).
 hipError_t launch_gather(const DevView &in, size_t entries, size_t count, uint32_t *out, size_t stride,
                          hipStream_t s);
+// out[e*hdeg + i] = in(e, hdeg-1-i) (0 past the view's size), i < hdeg: the
+// reversal of the small midproduct operand (wordsize-fast.inl:228-235).
+hipError_t launch_reverse_gather(const DevView &in, size_t entries, size_t hdeg, uint32_t *out, hipStream_t s);
 // view(e, t) = in[e*stride + t] for t < count.
 hipError_t launch_scatter(const uint32_t *in, size_t stride, size_t entries, size_t count, const DevView &out,
                           hipStream_t s);

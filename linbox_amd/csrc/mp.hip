@@ -361,15 +361,19 @@ void mp_release(lbx_ctx *ctx)
     ctx->mp_plans.clear();
 }
 
-static lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a, const DevView &b,
-                             const DevView &c, size_t m, size_t k, size_t n, int method, hipStream_t s)
+lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a, const DevView &b, const DevView &c,
+                      size_t m, size_t k, size_t n, int method, hipStream_t s, const MidSpec *mid, unsigned mid_L)
 {
-    const size_t sa = a.size, sb = b.size, sc = sa + sb - 1;
+    const size_t sa = a.size, sb = b.size, sc = mid ? c.size : sa + sb - 1;
     unsigned lpts = 0;
     size_t pts = 1;
     while (pts < sc) {  // mul_crtla :151-154
         pts <<= 1;
         ++lpts;
+    }
+    if (mid) {  // midproduct_crtla :446-447: points from deg, not from the output size
+        lpts = mid_L;
+        pts = (size_t)1 << lpts;
     }
     MpPlan *plan;
     LBX_TRY(get_plan(ctx, Pw, nl, k, sa < sb ? sa : sb, lpts, &plan));
@@ -388,7 +392,9 @@ static lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const Dev
         DevView va{(uint32_t *)ra + (size_t)i * m * k * sa, LBX_POLFIRST, m, k, sa, 0, sa};
         DevView vb{(uint32_t *)rb + (size_t)i * k * n * sb, LBX_POLFIRST, k, n, sb, 0, sb};
         DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
-        if (sc == 1)
+        if (mid)
+            LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s, mid));
+        else if (sc == 1)
             LBX_TRY(constant_product(ctx, plan->primes[i], va, vb, vc, m, k, n, s));
         else
             LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));

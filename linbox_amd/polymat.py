@@ -158,14 +158,56 @@ class PolynomialMatrixFFTMulDomain:
                                         ctypes.byref(pb), max_rowdeg, self.method, self.stream))
         return c
 
+    def midproduct(self, c, a, b, small_left=True, n0=0, n1=0):
+        """c = (a*b x^-(hdeg-1)) mod x^pts on c.size() coefficients, hdeg = n0 or c.size(),
+        deg = n1 or 2*hdeg-1 (matpoly-mult-fft-wordsize.inl:90-127); c is not resized."""
+        if a.coldim() != b.rowdim():
+            raise LinboxError(5, "a.coldim() != b.rowdim()")
+        pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+        check(lib().lbx_polymat_midproduct_u32(self.ctx.handle, self.p, ctypes.byref(pc), ctypes.byref(pa),
+                                               ctypes.byref(pb), 1 if small_left else 0, n0, n1, self.method,
+                                               self.stream))
+        return c
+
+
+class PolynomialMatrixNaiveMulDomain:
+    """PolynomialMatrixNaiveMulDomain::midproduct (matpoly-mult-naive.h:85-125), on the device."""
+
+    def __init__(self, p, ctx=None, stream=None):
+        self.p = int(p)
+        self.ctx = ctx or default_context()
+        self.stream = stream
+
+    def midproduct(self, c, a, b, small_left=True, n0=0, n1=0):
+        if a.coldim() != b.rowdim():
+            raise LinboxError(5, "a.coldim() != b.rowdim()")
+        pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+        check(lib().lbx_polymat_midproduct_naive_u32(self.ctx.handle, self.p, ctypes.byref(pc), ctypes.byref(pa),
+                                                     ctypes.byref(pb), n0, n1, self.stream))
+        return c
+
 
 class PolynomialMatrixMulDomain(PolynomialMatrixFFTMulDomain):
-    """PolynomialMatrixMulDomain::mul (polynomial-matrix-domain.h:56-75): d = a.size()+b.size();
-    FFT when d > 2, naive otherwise -- both run on the device here.  max_rowdeg is
-    accepted and, as in the reference (:61), not forwarded."""
+    """PolynomialMatrixMulDomain (polynomial-matrix-domain.h:40-104).
+
+    mul: d = a.size()+b.size(); FFT when d > 2, naive otherwise -- both run on
+    the device here.  max_rowdeg is accepted and, as in the reference (:61), not
+    forwarded.  midproduct (:77-91): FFT when b.size() > 2, naive otherwise.
+    midproductgen (:94-103): naive when c.size() <= 4, FFT otherwise."""
 
     def mul(self, c, a, b, max_rowdeg=0):
         return PolynomialMatrixFFTMulDomain.mul(self, c, a, b, 0)
+
+    def midproduct(self, c, a, b):
+        if b.size() > 2:
+            return PolynomialMatrixFFTMulDomain.midproduct(self, c, a, b)
+        return PolynomialMatrixNaiveMulDomain(self.p, self.ctx, self.stream).midproduct(c, a, b)
+
+    def midproductgen(self, c, a, b, small_left=True, n0=0, n1=0):
+        if c.size() <= 4:
+            return PolynomialMatrixNaiveMulDomain(self.p, self.ctx, self.stream).midproduct(c, a, b, small_left, n0,
+                                                                                          n1)
+        return PolynomialMatrixFFTMulDomain.midproduct(self, c, a, b, small_left, n0, n1)
 
 
 class PolynomialMatrixFFTPrimeMulDomain:

@@ -356,6 +356,96 @@ int or_polymat_mul_naive(uint32_t p, size_t m, size_t k, size_t n,
     return OR_OK;
 }
 
+/* matpoly-mult-fft-wordsize-fast.inl:209-316. */
+int or_polymat_midproduct_fft_prime(uint32_t p, size_t m, size_t k, size_t n,
+                                    const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
+                                    uint32_t *c, size_t sc, int small_left, size_t n0, size_t n1)
+{
+    if (!sa || !sb || !sc || !m || !k || !n) return OR_ERR_INVALID_ARG;
+    size_t hdeg = n0 ? n0 : sc;
+    size_t deg = n1 ? n1 : 2 * hdeg - 1;
+    if (deg < hdeg || sc < deg - hdeg) return OR_ERR_SIZE_MISMATCH;                   /* :214 */
+    if ((small_left ? sb : sa) >= hdeg + deg) return OR_ERR_SIZE_MISMATCH;           /* :215-218 */
+    unsigned lpts = 0;
+    size_t pts = 1;
+    while (pts < deg) { pts <<= 1; ++lpts; }                                          /* :221 */
+    if (sa > pts || sb > pts || sc > pts || hdeg > pts) return OR_ERR_SIZE_MISMATCH;  /* copies into pts words */
+    if ((p - 1) % pts != 0) return OR_ERR_BAD_FFT_PRIME;                              /* :265-272 */
+    if (lpts == 0) return OR_ERR_K_NOT_POSITIVE;                                      /* fft.h:133-144 */
+    uint32_t w;
+    int st = or_compute_primitive_root(p, lpts, &w);
+    if (st) return st;
+    uint32_t winv = or_fft_invroot(p, lpts, w);                                       /* :274 FFTinv */
+
+    uint32_t *pw = (uint32_t *)malloc(sizeof(uint32_t) * pts);
+    uint32_t *pwi = (uint32_t *)malloc(sizeof(uint32_t) * pts);
+    uint32_t *a2 = (uint32_t *)calloc(m * k * pts, sizeof(uint32_t));
+    uint32_t *b2 = (uint32_t *)calloc(k * n * pts, sizeof(uint32_t));
+    uint32_t *c2 = (uint32_t *)calloc(m * n * pts, sizeof(uint32_t));
+    if (!pw || !pwi || !a2 || !b2 || !c2) {
+        free(pw); free(pwi); free(a2); free(b2); free(c2);
+        return OR_ERR_INVALID_ARG;
+    }
+    or_init_powers(p, lpts, w, pw, NULL);
+    or_init_powers(p, lpts, winv, pwi, NULL);
+    for (size_t e = 0; e < m * k; ++e) memcpy(a2 + e * pts, a + e * sa, sizeof(uint32_t) * sa);
+    for (size_t e = 0; e < k * n; ++e) memcpy(b2 + e * pts, b + e * sb, sizeof(uint32_t) * sb);
+    /* :228-235 reverse the first hdeg coefficients of the small operand */
+    uint32_t *sm = small_left ? a2 : b2;
+    size_t se = small_left ? m * k : k * n;
+    for (size_t e = 0; e < se; ++e)
+        for (size_t i = 0; i < hdeg / 2; ++i) {
+            uint32_t t = sm[e * pts + i];
+            sm[e * pts + i] = sm[e * pts + hdeg - 1 - i];
+            sm[e * pts + hdeg - 1 - i] = t;
+        }
+    /* :279-290 forward transforms */
+    or_stage_forward(p, lpts, small_left ? pw : pwi, a2, m * k);
+    or_stage_forward(p, lpts, small_left ? pwi : pw, b2, k * n);
+    /* :300-305 pointwise a.b */
+    or_stage_pointwise(p, m, k, n, pts, a2, b2, c2, 0, pts);
+    /* :311-313 FFTer.FFT_inverse: DIT with the root w */
+    or_stage_inverse(p, lpts, pw, c2, m * n);
+    /* :316-324 fscalin by pts^-1, then c.copy(c2, 0, c.size()-1) (:242) */
+    uint64_t inv_pts = or_invmod((uint32_t)(pts % p), p);
+    for (size_t e = 0; e < m * n; ++e)
+        for (size_t t = 0; t < sc; ++t) c[e * sc + t] = (uint32_t)(c2[e * pts + t] * inv_pts % p);
+    free(pw); free(pwi); free(a2); free(b2); free(c2);
+    return OR_OK;
+}
+
+/* matpoly-mult-naive.h:85-125. */
+int or_polymat_midproduct_naive(uint32_t p, size_t m, size_t k, size_t n,
+                                const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
+                                uint32_t *c, size_t sc, int small_left, size_t n0, size_t n1)
+{
+    if (!sa || !sb || !sc) return OR_ERR_INVALID_ARG;
+    size_t hdeg = n0 ? n0 : sc;
+    size_t deg = n1 ? n1 : 2 * hdeg - 1;
+    if (hdeg == 0) return OR_ERR_INVALID_ARG;
+    size_t kend = sa + sb - 1 < deg ? sa + sb - 1 : deg;
+    for (size_t kk = hdeg - 1; kk < kend; ++kk) {
+        size_t t = kk - hdeg + 1;
+        if (t >= sc) return OR_ERR_SIZE_MISMATCH;
+        /* smallLeft walks a from idx_a = kk - min(kk, sb-1), the other branch walks b
+         * from kk - min(kk, sa-1); both sum a[u] b[kk-u] over every valid u */
+        size_t u_lo = kk + 1 > sb ? kk + 1 - sb : 0;
+        size_t u_hi = kk < sa - 1 ? kk : sa - 1;
+        (void)small_left;
+        for (size_t i = 0; i < m; ++i)
+            for (size_t j = 0; j < n; ++j) {
+                u128 acc = 0;
+                for (size_t q = 0; q < k; ++q) {
+                    const uint32_t *pa = a + (i * k + q) * sa;
+                    const uint32_t *pb = b + (q * n + j) * sb;
+                    for (size_t u = u_lo; u <= u_hi; ++u) acc += (uint64_t)pa[u] * pb[kk - u];
+                }
+                c[(i * n + j) * sc + t] = (uint32_t)(acc % p);
+            }
+    }
+    return OR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Small fixed-width multiprecision (little-endian u32 limbs).               */
 /* ------------------------------------------------------------------------ */

@@ -17,6 +17,9 @@
  *   or_polymat_mul_naive       matpoly-mult-naive.h:56-82  pinned: Python big-int golden
  *   or_generate_fft_primes     random-fftprime.h:107-166   pinned: sympy-free MR + golden list
  *   or_polymat_mul_crt         matpoly-mult-fft-multiprecision.inl:141-277,611-630 pinned: golden
+ *   or_polymat_midproduct_fft_prime  matpoly-mult-fft-wordsize-fast.inl:209-316
+ *                              pinned: cyclic-correlation brute force + golden
+ *   or_polymat_midproduct_naive matpoly-mult-naive.h:85-125 pinned: coefficients of or_polymat_mul_naive
  *
  * Arithmetic is exact integer arithmetic in uint64_t / unsigned __int128.
  * The reference's lazy Harvey butterflies keep representatives in [0,2p) or
@@ -119,6 +122,27 @@ int or_generate_fft_primes(uint64_t pbound, const uint32_t *prodbound, int nlimb
 int or_polymat_mul_crt(const uint32_t *P, int nlimbs, size_t m, size_t k, size_t n,
                        const uint32_t *a, size_t sa, const uint32_t *b, size_t sb, uint32_t *c,
                        uint32_t *primes_used, int *nprimes);
+
+/* PolynomialMatrixFFTPrimeMulDomain::midproduct (matpoly-mult-fft-wordsize-fast.inl:209-245)
+ * + midproduct_fft (:249-316).  hdeg = n0 ? n0 : sc, deg = n1 ? n1 : 2*hdeg-1,
+ * pts = the least power of two >= deg.  The operands are zero-padded to pts;
+ * the first hdeg coefficients of the small operand (a if small_left, else b)
+ * are reversed by the swaps of :228-235; with small_left a is transformed with
+ * the root w and b with w^-1 (roles exchanged otherwise); the pointwise
+ * product a.b is transformed back with FFT_inverse of root w and scaled by
+ * pts^-1; c (polfirst, stride sc) receives coefficients 0..sc-1.  Checks of
+ * :211-219 (c.size >= deg-hdeg, other operand size < hdeg+deg), every size
+ * <= pts, and the FFT-prime checks of :265-272. */
+int or_polymat_midproduct_fft_prime(uint32_t p, size_t m, size_t k, size_t n,
+                                    const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
+                                    uint32_t *c, size_t sc, int small_left, size_t n0, size_t n1);
+
+/* PolynomialMatrixNaiveMulDomain::midproduct (matpoly-mult-naive.h:85-125):
+ * for kk in [hdeg-1, min(sa+sb-1, deg)), c[kk-hdeg+1] = coefficient kk of a.b;
+ * the other coefficients of c are left as they are. */
+int or_polymat_midproduct_naive(uint32_t p, size_t m, size_t k, size_t n,
+                                const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
+                                uint32_t *c, size_t sc, int small_left, size_t n0, size_t n1);
 
 #ifdef __cplusplus
 }

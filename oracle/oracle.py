@@ -47,6 +47,8 @@ def lib():
         L.or_generate_fft_primes.argtypes = [u64, _u32p, i32, ctypes.c_uint, _u32p, i32]
         L.or_polymat_mul_crt.argtypes = [_u32p, i32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, _u32p,
                                          ctypes.POINTER(i32)]
+        L.or_polymat_midproduct_fft_prime.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz, i32, sz, sz]
+        L.or_polymat_midproduct_naive.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz, i32, sz, sz]
         L.or_stage_forward.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
         L.or_stage_inverse.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
         L.or_stage_pointwise.argtypes = [u32, sz, sz, sz, sz, _u32p, _u32p, _u32p, sz, sz]
@@ -120,6 +122,27 @@ def mul_naive(p, m, k, n, a, sa, b, sb):
     st = lib().or_polymat_mul_naive(p, m, k, n, _p(a), sa, _p(b), sb, _p(c))
     assert st == 0
     return c
+
+
+def midproduct_fft_prime(p, m, k, n, a, sa, b, sb, sc, small_left=True, n0=0, n1=0):
+    """FFT-prime midproduct (polfirst); returns (status, c) with c of shape (m*n*sc,)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    c = np.zeros(m * n * sc, dtype=np.uint32)
+    st = lib().or_polymat_midproduct_fft_prime(p, m, k, n, _p(a), sa, _p(b), sb, _p(c), sc, 1 if small_left else 0,
+                                               n0, n1)
+    return st, c
+
+
+def midproduct_naive(p, m, k, n, a, sa, b, sb, c, small_left=True, n0=0, n1=0):
+    """Naive midproduct into a copy of c (coefficients it does not compute keep c's values)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    c = np.array(c, dtype=np.uint32, copy=True)
+    sc = c.size // (m * n)
+    st = lib().or_polymat_midproduct_naive(p, m, k, n, _p(a), sa, _p(b), sb, _p(c), sc, 1 if small_left else 0,
+                                           n0, n1)
+    return st, c
 
 
 def generate_fft_primes(pbound, prodbound, kmin, max_primes=64):

@@ -163,6 +163,40 @@ static void test_views_and_errors(uint32_t p, std::mt19937_64 &g)
     EXPECT(threw, "dimension mismatch throws");
 }
 
+// midproduct(c, a, b) with the defaults (hdeg = c.size(), deg = 2*hdeg-1) is
+// coefficients hdeg-1 .. 2*hdeg-2 of a*b (the check of matpoly-mult-fft.h:168-215
+// in closed form); midproductgen with c.size() <= 4 leaves the coefficients
+// past the middle product untouched (matpoly-mult-naive.h:85-125).
+static void test_midproduct(uint32_t p, std::mt19937_64 &g)
+{
+    Modular F(p);
+    const size_t hdeg = 40;
+    PolynomialMatrix<polfirst> A(F, 3, 5, hdeg), B(F, 5, 4, 2 * hdeg - 1), C(F, 3, 4, hdeg), P(F, 3, 4, 1);
+    fill(A, p, g);
+    fill(B, p, g);
+    PolynomialMatrixMulDomain PMD(F);
+    PMD.midproduct(C, A, B);
+    PMD.mul(P, A, B);
+    bool same = C.size() == hdeg;
+    for (size_t e = 0; e < 12; ++e)
+        for (size_t t = 0; t < hdeg; ++t) same &= C.get(e, t) == P.get(e, t + hdeg - 1);
+    EXPECT(same, "midproduct = middle coefficients of the product");
+    PolynomialMatrix<matfirst> Cs(F, 3, 4, 4);
+    for (size_t e = 0; e < 12; ++e)
+        for (size_t t = 0; t < 4; ++t) Cs.ref(e, t) = 7;
+    PolynomialMatrix<polfirst> As(F, 3, 5, 2), Bs(F, 5, 4, 3), Ps(F, 3, 4, 1);
+    fill(As, p, g);
+    fill(Bs, p, g);
+    PMD.midproductgen(Cs, As, Bs, true, 2, 0);  // hdeg 2, deg 3: coefficients 1..2 of a*b
+    PMD.mul(Ps, As, Bs);
+    bool ok = true;
+    for (size_t e = 0; e < 12; ++e) {
+        ok &= Cs.get(e, 0) == Ps.get(e, 1) && Cs.get(e, 1) == Ps.get(e, 2);
+        ok &= Cs.get(e, 2) == 7 && Cs.get(e, 3) == 7;
+    }
+    EXPECT(ok, "naive midproduct writes only its window");
+}
+
 int main(int argc, char **argv)
 {
     const uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 20191;
@@ -175,6 +209,7 @@ int main(int argc, char **argv)
         test_mul<matfirst, matrowfirst, polfirst>(p, 16, 100, LBX_METHOD_AUTO, g);
         test_mul<matrowfirst, matfirst, matfirst>(p, 5, 33, LBX_METHOD_AUTO, g);
         test_views_and_errors(p, g);
+        test_midproduct(p, g);
     }
     test_mul<polfirst, polfirst, polfirst>(1000000007u, 8, 64, LBX_METHOD_AUTO, g);  // CRT route
     if (failures) {

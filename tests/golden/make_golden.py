@@ -14,6 +14,13 @@ output is computed from the mathematical definition with Python integers:
                       matpoly-mult-naive.h:56-82), including max_rowdeg truncation
                       (cyclic wrap, matpoly-mult-fft-wordsize-fast.inl:64-80).
 * crt_mul.json     -- the same over Z/PZ for a seeded random 192-bit prime P.
+* midproduct.json  -- the FFT-prime midproduct written as the cyclic
+                      correlation it computes (matpoly-mult-fft-wordsize-fast.inl:209-316):
+                      with pts = 2^ceil(log2 deg) and the first hdeg
+                      coefficients of the small operand reversed, c[t] =
+                      sum_v a2[(v-t) mod pts] b[v] (small operand on the left)
+                      or sum_u a[u] b2[(u-t) mod pts]; cases include outputs past
+                      the middle product, where the correlation wraps.
 * fft_primes.json  -- primes m*2^k+1 in the visiting order of
                       RandomFFTPrime::generatePrimes (random-fftprime.h:107-133),
                       primality from sympy.
@@ -126,6 +133,60 @@ def mul_cases(rng):
     return cases
 
 
+def midproduct(p, m, k, n, a, sa, b, sb, sc, small_left, n0, n1):
+    hdeg = n0 or sc
+    deg = n1 or 2 * hdeg - 1
+    pts = 1
+    while pts < deg:
+        pts <<= 1
+
+    def padded(x, s, e):
+        return x[e * s:(e + 1) * s] + [0] * (pts - s)
+
+    def rev(v):  # the swaps of the first hdeg coefficients
+        v = list(v)
+        v[:hdeg] = v[:hdeg][::-1]
+        return v
+    c = []
+    for i in range(m):
+        for j in range(n):
+            out = [0] * sc
+            for q in range(k):
+                pa = padded(a, sa, i * k + q)
+                pb = padded(b, sb, q * n + j)
+                if small_left:
+                    pa = rev(pa)
+                    for t in range(sc):
+                        out[t] += sum(pa[(v - t) % pts] * pb[v] for v in range(pts) if pb[v])
+                else:
+                    pb = rev(pb)
+                    for t in range(sc):
+                        out[t] += sum(pa[u] * pb[(u - t) % pts] for u in range(pts) if pa[u])
+            c.extend(x % p for x in out)
+    return c
+
+
+def midproduct_cases(rng):
+    cases = []
+    # (m, k, n, sa, sb, sc, small_left, n0, n1): default windows, explicit windows,
+    # outputs longer than the middle product (wrapped terms), the big operand on the left
+    shapes = [(2, 3, 2, 4, 7, 4, True, 0, 0), (1, 2, 3, 5, 9, 5, True, 0, 0), (2, 2, 2, 3, 8, 6, True, 4, 9),
+              (3, 1, 2, 6, 3, 6, False, 0, 0), (2, 3, 1, 9, 5, 5, False, 5, 0), (2, 2, 2, 4, 12, 12, True, 4, 13),
+              (1, 1, 1, 1, 1, 1, True, 0, 0), (2, 2, 2, 4, 8, 8, True, 4, 8), (2, 1, 2, 8, 3, 8, False, 3, 7)]
+    for p in PRIMES:
+        for (m, k, n, sa, sb, sc, sl, n0, n1) in shapes:
+            a = [rng.randrange(p) for _ in range(m * k * sa)]
+            b = [rng.randrange(p) for _ in range(k * n * sb)]
+            deg = n1 or 2 * (n0 or sc) - 1
+            case = {"p": p, "m": m, "k": k, "n": n, "sa": sa, "sb": sb, "sc": sc, "small_left": sl,
+                    "n0": n0, "n1": n1, "a": a, "b": b, "status": 0,
+                    "c": midproduct(p, m, k, n, a, sa, b, sb, sc, sl, n0, n1)}
+            if deg <= 1:  # a 1-point transform: FFT's "k must be positive" (fft.h:136-139)
+                case["status"], case["c"] = 3, None
+            cases.append(case)
+    return cases
+
+
 def random_prime(bits, rng):
     while True:
         x = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
@@ -194,6 +255,8 @@ def main():
         json.dump(crt, f, separators=(",", ":"))
     with open(os.path.join(HERE, "fft_primes.json"), "w") as f:
         json.dump(prime_cases(crt), f, separators=(",", ":"))
+    with open(os.path.join(HERE, "midproduct.json"), "w") as f:
+        json.dump(midproduct_cases(random.Random(20192)), f, separators=(",", ":"))
 
 
 if __name__ == "__main__":

@@ -208,6 +208,55 @@ def test_crt_vs_bigint_random():
                 assert sum(int(got[t, z]) << (32 * z) for z in range(nl)) == ref[t] % P
 
 
+# ------------------------------------------------------------ midproduct -
+def test_midproduct_golden():
+    """FFT-prime midproduct against the cyclic-correlation fixtures (Python big
+    integers), including outputs where the correlation wraps."""
+    for c in load("midproduct.json"):
+        st, got = orc.midproduct_fft_prime(c["p"], c["m"], c["k"], c["n"], np.array(c["a"], dtype=np.uint32),
+                                           c["sa"], np.array(c["b"], dtype=np.uint32), c["sb"], c["sc"],
+                                           c["small_left"], c["n0"], c["n1"])
+        assert st == c["status"], c
+        if st == 0:
+            assert got.tolist() == c["c"]
+
+
+@pytest.mark.parametrize("small_left", [True, False])
+def test_midproduct_valid_window_is_the_middle_product(small_left):
+    """Inside the window [0, deg-hdeg] the FFT midproduct equals coefficients
+    hdeg-1.. of the plain product (naive convolution), as does the naive
+    midproduct, which leaves coefficients past min(sa+sb-1, deg) untouched."""
+    rng = np.random.default_rng(5 + small_left)
+    for p in PRIMES:
+        for (m, k, n, hdeg, big, n1) in [(2, 3, 4, 5, 9, 0), (3, 2, 2, 8, 16, 16), (1, 4, 2, 16, 31, 0),
+                                         (2, 2, 3, 7, 10, 13)]:
+            deg = n1 or 2 * hdeg - 1
+            sa, sb = (hdeg, big) if small_left else (big, hdeg)
+            a = rng.integers(0, p, m * k * sa, dtype=np.uint64).astype(np.uint32)
+            b = rng.integers(0, p, k * n * sb, dtype=np.uint64).astype(np.uint32)
+            sc = deg - hdeg + 1
+            st, c = orc.midproduct_fft_prime(p, m, k, n, a, sa, b, sb, sc, small_left, hdeg, n1)
+            assert st == 0
+            full = orc.mul_naive(p, m, k, n, a, sa, b, sb).reshape(m * n, sa + sb - 1)
+            want = np.zeros((m * n, sc), dtype=np.uint32)
+            top = min(sa + sb - 1, deg)
+            want[:, :top - hdeg + 1] = full[:, hdeg - 1:top]
+            assert np.array_equal(c.reshape(m * n, sc), want)
+            sentinel = np.full(m * n * sc, 7, dtype=np.uint32)
+            st, cn = orc.midproduct_naive(p, m, k, n, a, sa, b, sb, sentinel, small_left, hdeg, n1)
+            assert st == 0
+            cn = cn.reshape(m * n, sc)
+            assert np.array_equal(cn[:, :top - hdeg + 1], want[:, :top - hdeg + 1])
+            assert (cn[:, top - hdeg + 1:] == 7).all()
+
+
+def test_midproduct_errors():
+    p, a, b = 998244353, np.ones(4, dtype=np.uint32), np.ones(40, dtype=np.uint32)
+    assert orc.midproduct_fft_prime(p, 1, 1, 1, a, 4, b, 40, 4, True, 0, 0)[0] == 5      # b.size >= hdeg+deg
+    assert orc.midproduct_fft_prime(1000000007, 1, 1, 1, a, 4, b, 6, 4, True, 0, 0)[0] == 1  # not an FFT prime
+    assert orc.midproduct_fft_prime(p, 1, 1, 1, a, 4, b, 6, 2, True, 4, 9)[0] == 5       # c.size < deg-hdeg
+
+
 # ------------------------------------------------------------ mutations ---
 def test_checks_reject_planted_mistakes():
     p = 998244353
@@ -236,3 +285,14 @@ def test_checks_reject_planted_mistakes():
     c_perm[0, [1, 2]] = c_perm[0, [2, 1]]
     assert check_by_evaluation(a, b, c_perm, p, m, k, n, sa, sb, sa + sb - 1, [1])
     assert not check_by_evaluation(a, b, c_perm, p, m, k, n, sa, sb, sa + sb - 1, [12345, 777])
+    # midproduct with the roots of the two forward transforms exchanged computes
+    # a convolution instead of the correlation: the golden fixture rejects it
+    mc = [x for x in load("midproduct.json") if x["p"] == p and x["status"] == 0 and x["small_left"]][0]
+    st, good = orc.midproduct_fft_prime(p, mc["m"], mc["k"], mc["n"], np.array(mc["a"], dtype=np.uint32), mc["sa"],
+                                        np.array(mc["b"], dtype=np.uint32), mc["sb"], mc["sc"], True, mc["n0"],
+                                        mc["n1"])
+    assert good.tolist() == mc["c"]
+    st, swapped = orc.midproduct_fft_prime(p, mc["m"], mc["k"], mc["n"], np.array(mc["a"], dtype=np.uint32),
+                                           mc["sa"], np.array(mc["b"], dtype=np.uint32), mc["sb"], mc["sc"], False,
+                                           mc["n0"], mc["n1"])
+    assert swapped.tolist() != mc["c"]
