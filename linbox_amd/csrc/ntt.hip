@@ -1,6 +1,8 @@
 // Number-theoretic transform kernels and their launchers: whole transforms
 // for the plain path and the NTT entry points, and the split head / tail
 // transforms around the point-major pointwise stage (see ntt_device.h).
+#include <algorithm>
+
 #include "ntt_device.h"
 
 namespace lbx {
@@ -14,6 +16,25 @@ __global__ void __launch_bounds__(Cfg<L>::BLOCK) ntt_dif_kernel(DevView in, size
 {
     extern __shared__ uint32_t lds_all[];
     dif_body<L, SEND, HALFZERO>(in, entries, map, tw, p, out, (size_t)blockIdx.x * Cfg<L>::G, threadIdx.x, lds_all);
+}
+
+template <int L, int SEND, bool HALFZERO>
+__global__ void __launch_bounds__(Cfg<L>::BLOCK) dif_head_persistent_kernel(DevView in, size_t entries, EntryMap map,
+                                                                           const uint2 *__restrict__ tw, uint32_t p,
+                                                                           uint32_t *__restrict__ out)
+{
+    extern __shared__ uint32_t lds_all[];
+    dif_head_persistent<L, SEND, HALFZERO>(in, entries, map, tw, p, out, threadIdx.x, lds_all);
+}
+
+template <int L, int DSTART>
+__global__ void __launch_bounds__(Cfg<L>::BLOCK) dit_tail_persistent_kernel(const uint32_t *__restrict__ in,
+                                                                           size_t entries,
+                                                                           const uint2 *__restrict__ tw, uint32_t p,
+                                                                           DevView out, size_t out_first, size_t count)
+{
+    extern __shared__ uint32_t lds_all[];
+    dit_tail_persistent<L, DSTART>(in, entries, tw, p, out, out_first, count, threadIdx.x, lds_all);
 }
 
 template <int L, int DSTART, bool SCALE>
@@ -45,6 +66,66 @@ hipError_t fwd_launch(const DevView &in, size_t entries, const EntryMap &map, co
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(grid), dim3(C::BLOCK), C::LDS_BYTES, s, in, entries, map, tw, p, out);
     return hipGetLastError();
+}
+
+int device_cus()
+{
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                     hipSuccess || n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
+// Forward head: the persistent form (grid of two groups per CU) for polfirst
+// inputs of 2^12..2^14 points, the one-group-per-polynomial form otherwise (at
+// 2^15 the register twiddles of a 32-word pass would spill).
+template <int L, int SEND>
+hipError_t head_launch(const DevView &in, size_t entries, const EntryMap &map, const uint2 *tw, uint32_t p,
+                       uint32_t *out, hipStream_t s)
+{
+    if constexpr (L >= 12 && L <= 14) {
+        if (in.layout == 0) {
+            using H = HeadShape<L, SEND>;
+            const bool hz = in.size * 2 <= (size_t)H::N;
+            auto k = hz ? dif_head_persistent_kernel<L, SEND, true> : dif_head_persistent_kernel<L, SEND, false>;
+            hipError_t e = allow_lds(k, H::LDS_BYTES);
+            if (e != hipSuccess) return e;
+            const size_t grid = std::min<size_t>(entries, (size_t)device_cus() * 2);
+            hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(Cfg<L>::BLOCK), H::LDS_BYTES, s, in, entries, map, tw, p,
+                               out);
+            return hipGetLastError();
+        }
+    }
+    return fwd_launch<L, SEND>(in, entries, map, tw, p, out, s);
+}
+
+template <int L, int DSTART>
+hipError_t inv_launch(const uint32_t *in, size_t entries, const uint2 *tw, uint32_t p, const DevView &out,
+                      size_t out_first, size_t count, uint32_t scale, hipStream_t s);
+
+// Inverse tail: the persistent form for 2^12..2^14 points, polfirst output and
+// no final scale; the one-group-per-polynomial form otherwise.
+template <int L, int DSTART>
+hipError_t tail_inv_launch(const uint32_t *in, size_t entries, const uint2 *tw, uint32_t p, const DevView &out,
+                           size_t out_first, size_t count, uint32_t scale, hipStream_t s)
+{
+    if constexpr (L >= 12 && L <= 14) {
+        if (scale == 1 && out.layout == 0) {
+            using T = TailShape<L, DSTART>;
+            auto k = dit_tail_persistent_kernel<L, DSTART>;
+            hipError_t e = allow_lds(k, T::LDS_BYTES);
+            if (e != hipSuccess) return e;
+            const size_t grid = std::min<size_t>(entries, (size_t)device_cus() * 2);
+            hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(Cfg<L>::BLOCK), T::LDS_BYTES, s, in, entries, tw, p, out,
+                               out_first, count);
+            return hipGetLastError();
+        }
+    }
+    return inv_launch<L, DSTART>(in, entries, tw, p, out, out_first, count, scale, s);
 }
 
 template <int L, int DSTART>
@@ -379,7 +460,7 @@ hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const Entr
     if (!split_ok(L, BL)) return hipErrorInvalidValue;
     switch (L * 16 + BL) {
 #define LBX_CASE(l, bl) \
-    case l * 16 + bl: return fwd_launch<l, l - bl>(in, entries, map, tw, p, out, s);
+    case l * 16 + bl: return head_launch<l, l - bl>(in, entries, map, tw, p, out, s);
         LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
         LBX_CASE(15, 6)
 #undef LBX_CASE
@@ -395,7 +476,7 @@ hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned 
     if (!split_ok(L, BL)) return hipErrorInvalidValue;
     switch (L * 16 + BL) {
 #define LBX_CASE(l, bl) \
-    case l * 16 + bl: return inv_launch<l, bl>(in, entries, tw, p, out, out_first, count, scale, s);
+    case l * 16 + bl: return tail_inv_launch<l, bl>(in, entries, tw, p, out, out_first, count, scale, s);
         LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
         LBX_CASE(15, 6)
 #undef LBX_CASE

@@ -101,8 +101,11 @@ __device__ __forceinline__ void gather(uint32_t (&x)[Cfg<L>::R], const uint32_t 
 }
 
 // DIF stages S0..S0+RP-1 (span N>>(s+1) at stage s) on the registers.
+// tw_off: the twiddle table starts tw_off entries into the pow_w layout (an
+// LDS copy of the tail of the table).
 template <int L, int S0, int RP, bool HALFZERO>
-__device__ __forceinline__ void dif_pass(uint32_t (&x)[Cfg<L>::R], int tp, const uint2 *__restrict__ tw, uint32_t p)
+__device__ __forceinline__ void dif_pass(uint32_t (&x)[Cfg<L>::R], int tp, const uint2 *__restrict__ tw, uint32_t p,
+                                         int tw_off = 0)
 {
     constexpr int N = Cfg<L>::N, TP = Cfg<L>::TP, NG = Cfg<L>::R >> RP;
     constexpr int HL = L - S0 - RP;
@@ -113,7 +116,7 @@ __device__ __forceinline__ void dif_pass(uint32_t (&x)[Cfg<L>::R], int tp, const
         for (int st = 0; st < RP; ++st) {
             const int s = S0 + st;
             const int half = 1 << (RP - 1 - st);
-            const uint2 *tws = tw + (N - (N >> s));
+            const uint2 *tws = tw + (N - (N >> s) - tw_off);
 #pragma unroll
             for (int q = 0; q < (1 << RP); ++q) {
                 if (q & half) continue;
@@ -140,7 +143,8 @@ __device__ __forceinline__ void dif_pass(uint32_t (&x)[Cfg<L>::R], int tp, const
 
 // DIT stages D0..D0+RP-1 (span 2^d at stage d).
 template <int L, int D0, int RP>
-__device__ __forceinline__ void dit_pass(uint32_t (&x)[Cfg<L>::R], int tp, const uint2 *__restrict__ tw, uint32_t p)
+__device__ __forceinline__ void dit_pass(uint32_t (&x)[Cfg<L>::R], int tp, const uint2 *__restrict__ tw, uint32_t p,
+                                         int tw_off = 0)
 {
     constexpr int N = Cfg<L>::N, TP = Cfg<L>::TP, NG = Cfg<L>::R >> RP;
 #pragma unroll
@@ -150,7 +154,7 @@ __device__ __forceinline__ void dit_pass(uint32_t (&x)[Cfg<L>::R], int tp, const
         for (int st = 0; st < RP; ++st) {
             const int d = D0 + st;
             const int half = 1 << st;
-            const uint2 *tws = tw + (N - (2 << d));
+            const uint2 *tws = tw + (N - (2 << d) - tw_off);
 #pragma unroll
             for (int q = 0; q < (1 << RP); ++q) {
                 if (q & half) continue;
@@ -173,11 +177,11 @@ __device__ __forceinline__ void dit_pass(uint32_t (&x)[Cfg<L>::R], int tp, const
 // On entry x holds the mapping of pass S0.
 template <int L, int S0, int SEND, bool HALFZERO>
 __device__ __forceinline__ void dif_passes(uint32_t (&x)[Cfg<L>::R], uint32_t *lds, int tp, const uint2 *tw,
-                                           uint32_t p)
+                                           uint32_t p, int tw_off = 0)
 {
     constexpr int RLOG = Cfg<L>::RLOG;
     constexpr int RP = (SEND - S0) < RLOG ? (SEND - S0) : RLOG;
-    dif_pass<L, S0, RP, HALFZERO>(x, tp, tw, p);
+    dif_pass<L, S0, RP, HALFZERO>(x, tp, tw, p, tw_off);
     if constexpr (S0 + RP < SEND) {
         constexpr int S1 = S0 + RP;
         constexpr int NRP = (SEND - S1) < RLOG ? (SEND - S1) : RLOG;
@@ -185,7 +189,7 @@ __device__ __forceinline__ void dif_passes(uint32_t (&x)[Cfg<L>::R], uint32_t *l
         scatter<L, L - S0 - RP, RP>(x, lds, tp);
         __syncthreads();
         gather<L, L - S1 - NRP, NRP>(x, lds, tp);
-        dif_passes<L, S1, SEND, false>(x, lds, tp, tw, p);
+        dif_passes<L, S1, SEND, false>(x, lds, tp, tw, p, tw_off);
     }
 }
 
@@ -354,6 +358,178 @@ __device__ __forceinline__ void dit_body(const uint32_t *__restrict__ in, size_t
                     out.base[view_index(out, out_first + e, t)] = v;
                 }
             }
+    }
+}
+
+// ---- persistent forward head ------------------------------------------------
+// Stages [0, SEND) of many polynomials of N = 2^L >= 2^12 words (one per
+// workgroup at a time, polfirst input).  Each thread's twiddles of the first
+// pass (they depend only on the thread index) are loaded into registers once;
+// the twiddle tables of the later stages (N/2^RLOG - N/2^SEND entries) are
+// copied to LDS once.  Inside the loop the only global loads are then the
+// next polynomial's input, issued before the current one is transformed, so
+// they overlap its butterflies (an in-order wait for a twiddle load would also
+// wait for them).
+template <int RP>
+__device__ __forceinline__ constexpr int first_tw_base(int st)
+{
+    return (1 << RP) - (1 << (RP - st));  // offsets of stage st's 2^(RP-1-st) twiddles
+}
+
+template <int L, int SEND>
+struct HeadShape {
+    static constexpr int RLOG = Cfg<L>::RLOG, R = Cfg<L>::R, TP = Cfg<L>::TP, N = Cfg<L>::N;
+    static constexpr int RP0 = SEND < RLOG ? SEND : RLOG;  // stages of the first pass
+    static constexpr int HL0 = L - RP0;
+    static constexpr int NT = (1 << RP0) - 1;               // register twiddles per thread (one group)
+    static constexpr int LDS_TW = (N >> RP0) - (N >> SEND);  // later stages' table entries
+    static constexpr size_t LDS_BYTES = Cfg<L>::LDS_BYTES + (size_t)LDS_TW * 8;
+    static_assert(Cfg<L>::G == 1 && (R >> RP0) == 1, "one polynomial per group, one group per thread");
+};
+
+template <int L, int SEND, bool HALFZERO>
+__device__ __forceinline__ void dif_head_persistent(const DevView &in, size_t entries, const EntryMap &map,
+                                                    const uint2 *__restrict__ tw, uint32_t p,
+                                                    uint32_t *__restrict__ out, int tp, uint32_t *lds)
+{
+    using H = HeadShape<L, SEND>;
+    using C = Cfg<L>;
+    uint2 *tw_lds = reinterpret_cast<uint2 *>(lds + C::STRIDE);
+    const int tw_off = H::N - (H::N >> H::RP0);
+    for (int i = tp; i < H::LDS_TW; i += C::BLOCK) tw_lds[i] = tw[tw_off + i];
+    uint2 t0[H::NT];
+#pragma unroll
+    for (int st = 0; st < H::RP0; ++st)
+#pragma unroll
+        for (int j = 0; j < (1 << (H::RP0 - 1 - st)); ++j)
+            t0[first_tw_base<H::RP0>(st) + j] = tw[H::N - (H::N >> st) + tp + (j << H::HL0)];
+    // words tp + q*TP of the first pass; with HALFZERO only q < R/2 can be nonzero
+    constexpr int NLOAD = HALFZERO ? H::R / 2 : H::R;
+    uint32_t nxt[NLOAD];
+    auto fetch = [&](size_t e) {
+        const uint32_t *src = in.base + map(e) * in.stor + in.shift;
+#pragma unroll
+        for (int q = 0; q < NLOAD; ++q) {
+            const size_t t = (size_t)tp + (size_t)q * H::TP;
+            nxt[q] = t < in.size ? src[t] : 0u;
+        }
+    };
+    size_t e = blockIdx.x;
+    if (e < entries) fetch(e);
+    for (; e < entries; e += gridDim.x) {
+        uint32_t x[H::R];
+#pragma unroll
+        for (int q = 0; q < H::R; ++q) x[q] = q < NLOAD ? nxt[q] : 0u;
+        if (e + gridDim.x < entries) fetch(e + gridDim.x);
+        // first pass from the register twiddles
+#pragma unroll
+        for (int st = 0; st < H::RP0; ++st) {
+            const int half = 1 << (H::RP0 - 1 - st);
+#pragma unroll
+            for (int q = 0; q < H::R; ++q) {
+                if (q & half) continue;
+                const uint2 w = t0[first_tw_base<H::RP0>(st) + (q & (half - 1))];
+                if (HALFZERO && st == 0) {
+                    x[q + half] = shoup_mul(x[q], w.x, w.y, p);
+                } else {
+                    const uint32_t a = x[q], b = x[q + half];
+                    x[q] = add_mod(a, b, p);
+                    x[q + half] = shoup_mul(a + (p - b), w.x, w.y, p);
+                }
+            }
+        }
+        if constexpr (H::RP0 < SEND) {
+            constexpr int S1 = H::RP0;
+            constexpr int NRP = (SEND - S1) < H::RLOG ? (SEND - S1) : H::RLOG;
+            scatter<L, L - H::RP0, H::RP0>(x, lds, tp);
+            __syncthreads();  // also publishes the LDS twiddle table on the first entry
+            gather<L, L - S1 - NRP, NRP>(x, lds, tp);
+            dif_passes<L, S1, SEND, false>(x, lds, tp, tw_lds, p, tw_off);
+        }
+        constexpr int RP = dif_last_rp<L, SEND>();
+        constexpr int NG = H::R >> RP;
+        uint32_t *o = out + e * H::N;
+#pragma unroll
+        for (int u = 0; u < NG; ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << RP); ++q) o[elem<L, L - SEND, RP>(tp, u, q)] = x[u * (1 << RP) + q];
+        __syncthreads();  // LDS is rewritten by the next polynomial
+    }
+}
+
+// ---- persistent inverse tail ------------------------------------------------
+// DIT stages [DSTART, L) of many polynomials of N = 2^L >= 2^12 words in two
+// passes: the first (spans 2^DSTART .. 2^(L-RLOG-1)) reads an LDS copy of its
+// twiddle tables (2^(L-RLOG) - 2^DSTART entries), the second (the RLOG largest
+// spans) the thread's 2^RLOG - 1 twiddles held in registers.  The next
+// polynomial's words are loaded while the current one is transformed.
+template <int L, int DSTART>
+struct TailShape {
+    static constexpr int RLOG = Cfg<L>::RLOG, R = Cfg<L>::R, TP = Cfg<L>::TP, N = Cfg<L>::N;
+    static constexpr int D1 = L - RLOG;              // first stage of the last pass
+    static constexpr int RP0 = D1 - DSTART;          // stages of the first pass
+    static constexpr int LDS_TW = (1 << D1) - (1 << DSTART);
+    static constexpr size_t LDS_BYTES = Cfg<L>::LDS_BYTES + (size_t)LDS_TW * 8;
+    static_assert(Cfg<L>::G == 1 && RP0 >= 1 && RP0 <= RLOG, "one polynomial per group, two passes");
+};
+
+template <int L, int DSTART>
+__device__ __forceinline__ void dit_tail_persistent(const uint32_t *__restrict__ in, size_t entries,
+                                                    const uint2 *__restrict__ tw, uint32_t p, const DevView &out,
+                                                    size_t out_first, size_t count, int tp, uint32_t *lds)
+{
+    using T = TailShape<L, DSTART>;
+    using C = Cfg<L>;
+    uint2 *tw_lds = reinterpret_cast<uint2 *>(lds + C::STRIDE);
+    const int tw_off = T::N - (1 << T::D1);
+    for (int i = tp; i < T::LDS_TW; i += C::BLOCK) tw_lds[i] = tw[tw_off + i];
+    uint2 t1[(1 << T::RLOG) - 1];
+#pragma unroll
+    for (int st = 0; st < T::RLOG; ++st)
+#pragma unroll
+        for (int j = 0; j < (1 << st); ++j)
+            t1[(1 << st) - 1 + j] = tw[T::N - (2 << (T::D1 + st)) + tp + (j << T::D1)];
+    constexpr int NG0 = T::R >> T::RP0;
+    uint32_t nxt[T::R];
+    auto fetch = [&](size_t e) {
+        const uint32_t *src = in + e * T::N;
+#pragma unroll
+        for (int u = 0; u < NG0; ++u)
+#pragma unroll
+            for (int q = 0; q < (1 << T::RP0); ++q) nxt[u * (1 << T::RP0) + q] = src[elem<L, DSTART, T::RP0>(tp, u, q)];
+    };
+    size_t e = blockIdx.x;
+    if (e < entries) fetch(e);
+    for (; e < entries; e += gridDim.x) {
+        uint32_t x[T::R];
+#pragma unroll
+        for (int q = 0; q < T::R; ++q) x[q] = nxt[q];
+        if (e + gridDim.x < entries) fetch(e + gridDim.x);
+        dit_pass<L, DSTART, T::RP0>(x, tp, tw_lds, p, tw_off);
+        scatter<L, DSTART, T::RP0>(x, lds, tp);
+        __syncthreads();  // also publishes the LDS twiddle table on the first entry
+        gather<L, T::D1, T::RLOG>(x, lds, tp);
+        // last pass: spans 2^D1 .. 2^(L-1), one group per thread (offset tp)
+#pragma unroll
+        for (int st = 0; st < T::RLOG; ++st) {
+            const int half = 1 << st;
+#pragma unroll
+            for (int q = 0; q < T::R; ++q) {
+                if (q & half) continue;
+                const uint2 w = t1[half - 1 + (q & (half - 1))];
+                const uint32_t a = x[q];
+                const uint32_t t = shoup_mul(x[q + half], w.x, w.y, p);
+                x[q] = add_mod(a, t, p);
+                x[q + half] = sub_mod(a, t, p);
+            }
+        }
+        uint32_t *o = out.base + (out_first + e) * out.stor + out.shift;  // polfirst output
+#pragma unroll
+        for (int q = 0; q < T::R; ++q) {
+            const int t = tp + q * T::TP;
+            if ((size_t)t < count) o[t] = x[q];
+        }
+        __syncthreads();  // LDS is rewritten by the next polynomial
     }
 }
 
