@@ -483,6 +483,7 @@ __device__ __forceinline__ void dit_tail_persistent(const uint32_t *__restrict__
     uint2 *tw_lds = reinterpret_cast<uint2 *>(lds + C::STRIDE);
     const int tw_off = T::N - (1 << T::D1);
     for (int i = tp; i < T::LDS_TW; i += C::BLOCK) tw_lds[i] = tw[tw_off + i];
+    __syncthreads();  // the first pass reads the table before any other barrier
     uint2 t1[(1 << T::RLOG) - 1];
 #pragma unroll
     for (int st = 0; st < T::RLOG; ++st)
@@ -507,7 +508,7 @@ __device__ __forceinline__ void dit_tail_persistent(const uint32_t *__restrict__
         if (e + gridDim.x < entries) fetch(e + gridDim.x);
         dit_pass<L, DSTART, T::RP0>(x, tp, tw_lds, p, tw_off);
         scatter<L, DSTART, T::RP0>(x, lds, tp);
-        __syncthreads();  // also publishes the LDS twiddle table on the first entry
+        __syncthreads();
         gather<L, T::D1, T::RLOG>(x, lds, tp);
         // last pass: spans 2^D1 .. 2^(L-1), one group per thread (offset tp)
 #pragma unroll
