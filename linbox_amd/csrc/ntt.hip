@@ -261,19 +261,23 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
 #pragma unroll
         for (int w = 0; w < B::TPE; ++w) row[B::TPE * (sub + B::TPE * u) + w] = to_limbs(x[u * B::TPE + w], p);
     __syncthreads();
-    // point t = b*P + lam: word w of the run = plane w>>4, entries 4(w&15)..+3
-#pragma unroll 4
-    for (int it = 0; it < 16; ++it) {
+    // point t = b*P + lam: 16-byte chunk ch of the run = plane ch>>2, entries
+    // 16(ch&3)..+15 (byte j of word i holds entry 16(ch&3) + 4i + j)
+#pragma unroll 2
+    for (int it = 0; it < 4; ++it) {
         const int id = tid + T * it;
-        const int lam = id >> 6, w = id & 63;
-        const int sft = 8 * (w >> 4), q4 = (w & 15) * 4;
-        const uint32_t *col = sm + q4 * B::STRIDE + lam;
-        const uint32_t word = ((col[0] >> sft) & 0xffu) | (((col[B::STRIDE] >> sft) & 0xffu) << 8) |
-                              (((col[2 * B::STRIDE] >> sft) & 0xffu) << 16) |
-                              (((col[3 * B::STRIDE] >> sft) & 0xffu) << 24);
+        const int lam = id >> 4, ch = id & 15;
+        const int sft = 8 * (ch >> 2);
+        const uint32_t *col = sm + (16 * (ch & 3)) * B::STRIDE + lam;
+        uint32_t w4[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            w4[i] = ((col[(4 * i) * B::STRIDE] >> sft) & 0xffu) | (((col[(4 * i + 1) * B::STRIDE] >> sft) & 0xffu) << 8) |
+                    (((col[(4 * i + 2) * B::STRIDE] >> sft) & 0xffu) << 16) |
+                    (((col[(4 * i + 3) * B::STRIDE] >> sft) & 0xffu) << 24);
         const size_t t = (size_t)b * P + lam;
-        *reinterpret_cast<uint32_t *>(Y + t * block_bytes + ((size_t)(row0 + r) * kchunks + c) * 256 + (w >> 4) * 64 +
-                                      (w & 15) * 4) = word;
+        *reinterpret_cast<uint4 *>(Y + t * block_bytes + ((size_t)(row0 + r) * kchunks + c) * 256 + ch * 16) =
+            make_uint4(w4[0], w4[1], w4[2], w4[3]);
     }
 }
 
@@ -291,12 +295,27 @@ __global__ void __launch_bounds__(Blk<BL>::T) dit_head_kernel(const uint32_t *__
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
     const size_t e0 = (size_t)blockIdx.y * 64;  // relative to e_first
+    if (((entries | e_first) & 3) == 0 && e0 + 64 <= e_count) {
+        // 16 lanes read one point's 64 consecutive entries as 16-byte chunks
+#pragma unroll 2
+        for (int it = 0; it < 4; ++it) {
+            const int id = tid + T * it;
+            const int lam = id >> 4, q = (id & 15) * 4;
+            const size_t t = (size_t)b * P + lam;
+            const uint4 v = *reinterpret_cast<const uint4 *>(Cm + t * entries + e_first + e0 + q);
+            sm[q * B::STRIDE + lam] = v.x;
+            sm[(q + 1) * B::STRIDE + lam] = v.y;
+            sm[(q + 2) * B::STRIDE + lam] = v.z;
+            sm[(q + 3) * B::STRIDE + lam] = v.w;
+        }
+    } else {
 #pragma unroll 4
-    for (int it = 0; it < 16; ++it) {  // 64 lanes read 64 consecutive entries of one point
-        const int id = tid + T * it;
-        const int q = id & 63, lam = id >> 6;
-        const size_t t = (size_t)b * P + lam;
-        sm[q * B::STRIDE + lam] = (e0 + q < e_count) ? Cm[t * entries + e_first + e0 + q] : 0u;
+        for (int it = 0; it < 16; ++it) {  // 64 lanes read 64 consecutive entries of one point
+            const int id = tid + T * it;
+            const int q = id & 63, lam = id >> 6;
+            const size_t t = (size_t)b * P + lam;
+            sm[q * B::STRIDE + lam] = (e0 + q < e_count) ? Cm[t * entries + e_first + e0 + q] : 0u;
+        }
     }
     __syncthreads();
     const int q = tid / TPE, sub = tid % TPE;
