@@ -64,7 +64,9 @@ struct EntryMap {
 
 // Split transforms for the point-major pointwise form, with blocks of 2^BL
 // points (BL = kSplitBL, BL + 2 <= L <= 15):
-// forward head = DIF stages [0, L-BL) per entry, polfirst out (row l <- entry map(l));
+// forward head = DIF stages [0, L-BL) per entry, written block-major:
+//                word w of block j of output row l (input entry map(l)) at
+//                (j*entries + l)*2^BL + w, so a tail tile is one contiguous run;
 // forward tail = the last BL DIF stages on 2^BL-point blocks of 64 entries,
 //                written straight into rows [row0, row0+rows_chunk) of the
 //                per-point limb blocks (rows >= rows_valid are zero padding;

@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
                                                                     size_t block_bytes, int kchunks)
 {
     using B = Blk<BL>;
-    constexpr int N = 1 << L, P = B::P, T = B::T, CPR = P / 4;  // 16-byte chunks per block row
+    constexpr int P = B::P, T = B::T, CPR = P / 4;  // 16-byte chunks per block row
     extern __shared__ uint32_t sm[];
     const int tid = threadIdx.x;
     const int b = blockIdx.x, c = blockIdx.y, r = blockIdx.z;  // r: row within the chunk
@@ -242,7 +242,8 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
         uint4 v = make_uint4(0, 0, 0, 0);
         if (r < rows_valid && kk < kdim) {
             const size_t e = row_major ? (size_t)r * kdim + kk : (size_t)kk * rows_valid + r;
-            v = *reinterpret_cast<const uint4 *>(X + e * N + (size_t)b * P + ch * 4);
+            // block-major head output: [block][entry][P words]
+            v = *reinterpret_cast<const uint4 *>(X + ((size_t)b * rows_valid * kdim + e) * P + ch * 4);
         }
         uint32_t *d = sm + q * B::STRIDE + ch * 4;
         d[0] = v.x;

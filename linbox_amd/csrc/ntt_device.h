@@ -282,11 +282,15 @@ __device__ __forceinline__ void dif_body(const DevView &in, size_t entries, cons
                 for (int q = 0; q < (1 << RP); ++q) o[(g << RP) + q] = x[u * (1 << RP) + q];
             }
         }
-    } else {
+    } else {  // block-major: [block][entry][2^(L-SEND) words]
 #pragma unroll
         for (int u = 0; u < NG; ++u)
 #pragma unroll
-            for (int q = 0; q < (1 << RP); ++q) o[elem<L, L - SEND, RP>(tp, u, q)] = x[u * (1 << RP) + q];
+            for (int q = 0; q < (1 << RP); ++q) {
+                const size_t i = (size_t)elem<L, L - SEND, RP>(tp, u, q);
+                out[(((i >> (L - SEND)) * entries + e) << (L - SEND)) | (i & ((1u << (L - SEND)) - 1))] =
+                    x[u * (1 << RP) + q];
+            }
     }
 }
 
@@ -448,11 +452,15 @@ __device__ __forceinline__ void dif_head_persistent(const DevView &in, size_t en
         }
         constexpr int RP = dif_last_rp<L, SEND>();
         constexpr int NG = H::R >> RP;
-        uint32_t *o = out + e * H::N;
+        // block-major output: [block][entry][2^(L-SEND) words]
 #pragma unroll
         for (int u = 0; u < NG; ++u)
 #pragma unroll
-            for (int q = 0; q < (1 << RP); ++q) o[elem<L, L - SEND, RP>(tp, u, q)] = x[u * (1 << RP) + q];
+            for (int q = 0; q < (1 << RP); ++q) {
+                const size_t i = (size_t)elem<L, L - SEND, RP>(tp, u, q);
+                out[(((i >> (L - SEND)) * entries + e) << (L - SEND)) | (i & ((1u << (L - SEND)) - 1))] =
+                    x[u * (1 << RP) + q];
+            }
         __syncthreads();  // LDS is rewritten by the next polynomial
     }
 }
