@@ -111,8 +111,13 @@ hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries,
 // ---- batch scheduler (batch.hip, with kernels in ntt.hip / pointwise.hip) ----
 // Constants of the plain pointwise epilogue: V = hi 2^64 + ah 2^32 + al,
 // V*scale mod p = hi*c2 + ah*c1 + al*c0 mod p.
+// Plain-form epilogue: acc (< 2^63) * scale mod p = hi(acc)*c1 + lo(acc)*c0 with
+// c0 = scale, c1 = 2^32 scale; accumulators are folded every `fold` products
+// with f62 = 2^62 mod p so that they never reach 2^64.
 struct PlainConst {
-    uint32_t c0, c0p, c1, c1p, c2, c2p;
+    uint32_t c0, c0p, c1, c1p;
+    uint32_t f62;   // 2^62 mod p
+    uint32_t fold;  // products added between folds
 };
 PlainConst plain_const(uint32_t p, uint32_t scale);
 

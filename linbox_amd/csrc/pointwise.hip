@@ -36,33 +36,35 @@ __device__ __forceinline__ void pw_plain_body(const uint32_t *__restrict__ A, co
                                               uint32_t *__restrict__ C, int m, int k, int n, size_t npts,
                                               uint32_t p, const PlainConst &pc, size_t t, int i0, int j0)
 {
-
+    // acc += a*b is one v_mad_u64_u32; every pc.fold products the bits above
+    // 2^62 are folded back (acc = acc mod 2^62 + (acc >> 62) * (2^62 mod p)),
+    // which keeps acc < 2^62 + 2^33 between folds and below 2^64 inside them
+    constexpr uint64_t kLow62 = ((uint64_t)1 << 62) - 1;
     uint64_t acc[TI][TJ];
-    uint32_t hi[TI][TJ];
 #pragma unroll
     for (int ii = 0; ii < TI; ++ii)
 #pragma unroll
-        for (int jj = 0; jj < TJ; ++jj) {
-            acc[ii][jj] = 0;
-            hi[ii][jj] = 0;
+        for (int jj = 0; jj < TJ; ++jj) acc[ii][jj] = 0;
+    for (int q0 = 0; q0 < k; q0 += (int)pc.fold) {
+        const int q1 = min(k, q0 + (int)pc.fold);
+        for (int q = q0; q < q1; ++q) {
+            uint32_t a[TI], b[TJ];
+#pragma unroll
+            for (int ii = 0; ii < TI; ++ii)
+                a[ii] = (i0 + ii < m) ? A[((size_t)(i0 + ii) * k + q) * npts + t] : 0u;
+#pragma unroll
+            for (int jj = 0; jj < TJ; ++jj)
+                b[jj] = (j0 + jj < n) ? B[((size_t)q * n + j0 + jj) * npts + t] : 0u;
+#pragma unroll
+            for (int ii = 0; ii < TI; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < TJ; ++jj) acc[ii][jj] += (uint64_t)a[ii] * b[jj];
         }
-    for (int q = 0; q < k; ++q) {
-        uint32_t a[TI], b[TJ];
-#pragma unroll
-        for (int ii = 0; ii < TI; ++ii)
-            a[ii] = (i0 + ii < m) ? A[((size_t)(i0 + ii) * k + q) * npts + t] : 0u;
-#pragma unroll
-        for (int jj = 0; jj < TJ; ++jj)
-            b[jj] = (j0 + jj < n) ? B[((size_t)q * n + j0 + jj) * npts + t] : 0u;
 #pragma unroll
         for (int ii = 0; ii < TI; ++ii)
 #pragma unroll
-            for (int jj = 0; jj < TJ; ++jj) {
-                const uint64_t prod = (uint64_t)a[ii] * b[jj];
-                const uint64_t s = acc[ii][jj] + prod;
-                hi[ii][jj] += (s < prod) ? 1u : 0u;
-                acc[ii][jj] = s;
-            }
+            for (int jj = 0; jj < TJ; ++jj)
+                acc[ii][jj] = (acc[ii][jj] & kLow62) + (uint64_t)(uint32_t)(acc[ii][jj] >> 62) * pc.f62;
     }
 #pragma unroll
     for (int ii = 0; ii < TI; ++ii)
@@ -70,9 +72,7 @@ __device__ __forceinline__ void pw_plain_body(const uint32_t *__restrict__ A, co
         for (int jj = 0; jj < TJ; ++jj) {
             if (i0 + ii >= m || j0 + jj >= n) continue;
             const uint32_t al = (uint32_t)acc[ii][jj], ah = (uint32_t)(acc[ii][jj] >> 32);
-            uint32_t r = shoup_mul(hi[ii][jj], pc.c2, pc.c2p, p);
-            r = add_mod(r, shoup_mul(ah, pc.c1, pc.c1p, p), p);
-            r = add_mod(r, shoup_mul(al, pc.c0, pc.c0p, p), p);
+            const uint32_t r = add_mod(shoup_mul(ah, pc.c1, pc.c1p, p), shoup_mul(al, pc.c0, pc.c0p, p), p);
             C[((size_t)(i0 + ii) * n + j0 + jj) * npts + t] = r;
         }
 }
@@ -393,13 +393,16 @@ PlainConst plain_const(uint32_t p, uint32_t scale)
 {
     PlainConst pc;
     const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
-    const uint32_t r64 = (uint32_t)((uint64_t)r32 * r32 % p);
     pc.c0 = scale % p;
     pc.c1 = (uint32_t)((uint64_t)r32 * pc.c0 % p);
-    pc.c2 = (uint32_t)((uint64_t)r64 * pc.c0 % p);
     pc.c0p = shoup_pre(pc.c0, p);
     pc.c1p = shoup_pre(pc.c1, p);
-    pc.c2p = shoup_pre(pc.c2, p);
+    pc.f62 = (uint32_t)(((uint64_t)1 << 62) % p);
+    // largest n with (2^62 + 2^33) + n (p-1)^2 < 2^64 (p < 2^31 here: n >= 2)
+    const unsigned __int128 room = ((unsigned __int128)1 << 64) - ((uint64_t)1 << 62) - ((uint64_t)1 << 33) - 1;
+    const uint64_t sq = (uint64_t)(p - 1) * (p - 1);
+    const unsigned __int128 nmax = sq ? room / sq : 64;
+    pc.fold = (uint32_t)(nmax > 64 ? 64 : nmax);
     return pc;
 }
 
