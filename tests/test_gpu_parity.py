@@ -360,10 +360,18 @@ def full_size_check(ctx, p, m, sa, seed):
     xs = [int(x) for x in rng.integers(2, p, 2)]
     say("full size evaluation check", m, sa)
     assert check_by_evaluation(a, b, outs[0], p, m, m, m, sa, sa, sc, xs)
+    # one whole row of C word for word against the oracle (row m-1: the last
+    # row band, padding rows of the limb blocks next to it)
+    i = m - 1
+    say("full size oracle row", m, sa, i)
+    st, ref = orc.mul_fft_prime(p, 1, m, m, a[i * m * sa:(i + 1) * m * sa], sa, b, sa)
+    assert st == 0
+    assert np.array_equal(outs[0][i * m * sc:(i + 1) * m * sc], ref)
 
 
 def test_config2_full_size(ctx):
-    """64x64.64x64, degree 4095, p=2013265921: plain == MFMA and C(x)=A(x)B(x)."""
+    """64x64.64x64, degree 4095, p=2013265921: plain == MFMA, C(x)=A(x)B(x) at
+    random points, and the last row of C against the oracle."""
     full_size_check(ctx, 2013265921, 64, 4096, 0x4C494E424F58 + 1)
 
 
