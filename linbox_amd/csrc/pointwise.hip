@@ -22,8 +22,6 @@
 //    The digit split happens once per element in launch_to_limbs, which also
 //    transposes the NTT output into per-point blocks laid out for direct
 //    16-byte operand loads.
-#include <cstdlib>
-
 #include "lbx_internal.h"
 
 namespace lbx {
@@ -262,8 +260,8 @@ __global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict_
     }
 }
 
-// LDS-staged form, used when one point's A limb block fits in 64 KiB (m*k <= 16384
-// after padding): persistent workgroups, two per CU, each takes a point, copies
+// LDS-staged form, used for the other shapes whose A limb block fits in 64 KiB
+// (m*k <= 16384 after padding): persistent workgroups, two per CU, each takes a point, copies
 // its A block to LDS once, then streams B through LDS 32 columns at a time; every
 // wave computes the 32x32 tiles of one row band.  16-byte chunks are stored
 // XOR-swizzled within each row (chunk c of row r at c ^ (r & 15)) so the 16 rows
@@ -362,6 +360,88 @@ __global__ void __launch_bounds__(256, 2) pw_mfma_lds_kernel(const uint8_t *__re
                             if (i < m && j < n) Ct[(size_t)i * n + j] = mfma_combine(acc[ai][bj], r, p, mc);
                         }
             }
+        }
+    }
+}
+
+// A in registers, B in LDS: for mpad in {32, 64, 128}, k <= 128 and a B block
+// of at most 64 KiB.  Each wave multiplies one 32-row band of A, which it
+// loads straight into registers (64 VGPRs at k = 128); only B, which every
+// wave needs, goes through LDS, whole, with one LDS-direct copy.  A point
+// therefore costs one wait and one barrier, with A and B (128 KiB at C3) in
+// flight together; two groups per CU overlap one's loads with the other's
+// multiplies.  With nrt = mpad/32 < 4 bands, wave w takes band w % nrt and
+// every (4/nrt)-th column block from w / nrt.
+template <int KC>
+__global__ void __launch_bounds__(256, 2) pw_mfma_rega_kernel(const uint8_t *__restrict__ Al,
+                                                              const uint8_t *__restrict__ Bl,
+                                                              uint32_t *__restrict__ Cmat, int m, int n, int mpad,
+                                                              int npad, size_t npts, uint32_t p, MfmaConst mc)
+{
+    extern __shared__ uint4 smem4[];
+    constexpr int cpr = KC * 16;  // 16-byte chunks per row
+    uint4 *lB = smem4;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, l16 = lane & 15, lg = lane >> 4;
+    const int a_chunks = mpad * cpr, b_chunks = npad * cpr;
+    const int nrt = mpad / 32, ncb = npad / 32;
+    const int band = wave % nrt, cb0 = wave / nrt, cbs = 4 / nrt;
+    for (size_t t = blockIdx.x; t < npts; t += gridDim.x) {
+        const uint4 *gA = reinterpret_cast<const uint4 *>(Al) + t * (size_t)a_chunks;
+        const uint4 *gB = reinterpret_cast<const uint4 *>(Bl) + t * (size_t)b_chunks;
+        __syncthreads();  // the previous point is done with lB
+        copy_lds(gB, lB, b_chunks, cpr, wave, lane);
+        v4i a[2][KC][4];
+#pragma unroll
+        for (int sub = 0; sub < 2; ++sub)
+#pragma unroll
+            for (int kb = 0; kb < KC; ++kb)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const uint4 v = gA[(band * 32 + sub * 16 + l16) * cpr + kb * 16 + s * 4 + lg];
+                    a[sub][kb][s] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+                }
+        __builtin_amdgcn_s_waitcnt(0);  // A in registers, this thread's B chunks in LDS
+        __syncthreads();
+        uint32_t *Ct = Cmat + t * (size_t)m * n;
+        for (int cb = cb0; cb < ncb; cb += cbs) {
+            v4i acc[2][2][7];
+#pragma unroll
+            for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int d = 0; d < 7; ++d) acc[ai][bj][d] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int kb = 0; kb < KC; ++kb) {
+                v4i b[2][4];
+#pragma unroll
+                for (int sub = 0; sub < 2; ++sub)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int c = (kb * 16 + s * 4 + lg) ^ l16;
+                        const uint4 vb = lB[(cb * 32 + sub * 16 + l16) * cpr + c];
+                        b[sub][s] = v4i{(int)vb.x, (int)vb.y, (int)vb.z, (int)vb.w};
+                    }
+#pragma unroll
+                for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                acc[ai][bj][s + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ai][kb][s], b[bj][u],
+                                                                                          acc[ai][bj][s + u], 0, 0, 0);
+            }
+#pragma unroll
+            for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = band * 32 + ai * 16 + lg * 4 + r, j = cb * 32 + bj * 16 + l16;
+                        if (i < m && j < n) Ct[(size_t)i * n + j] = mfma_combine(acc[ai][bj], r, p, mc);
+                    }
         }
     }
 }
@@ -467,6 +547,16 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
+    }
+    if ((mpad == 32 || mpad == 64 || mpad == 128) && kchunks <= 2 && npad * kchunks * 256 <= 65536) {
+        const size_t lds = npad * kchunks * 256;
+        const size_t grid = npts < (size_t)cus * 2 ? npts : (size_t)cus * 2;
+        auto k = kchunks == 1 ? pw_mfma_rega_kernel<1> : pw_mfma_rega_kernel<2>;
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(256), lds, s, Al, Bl, Cmat, (int)m, (int)n, (int)mpad,
+                           (int)npad, npts, p, mc);
+        return hipGetLastError();
     }
     if (a_bytes <= (size_t)kLdsAMax) {
         const size_t lds = a_bytes + (size_t)32 * kchunks * 256;

@@ -190,9 +190,13 @@ def test_mul_split_all_lengths(ctx, monkeypatch):
 def test_mul_mfma_shapes(ctx):
     """INT8 pointwise stage on shapes with partial row bands and column blocks,
     several inner-dimension chunks, and fewer points than workgroups."""
+    # register-A form: row bands of 32/64/128, k <= 128, B block <= 64 KiB;
+    # LDS-A form: 96 rows, k in (128, ...] or wide B; global form: A block > 64 KiB
     for (p, m, k, n, sa, sb) in [(2013265921, 33, 100, 70, 200, 57), (998244353, 128, 128, 128, 40, 41),
                                  (469762049, 100, 64, 130, 300, 300), (2013265921, 16, 200, 16, 130, 127),
-                                 (998244353, 97, 17, 33, 5, 4)]:
+                                 (998244353, 97, 17, 33, 5, 4), (2013265921, 70, 40, 50, 90, 60),
+                                 (469762049, 20, 130, 40, 64, 64), (998244353, 32, 64, 300, 33, 32),
+                                 (2013265921, 150, 150, 20, 20, 20)]:
         rng = np.random.default_rng(m * k + n)
         a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
         st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
