@@ -166,6 +166,29 @@ lbx_status lbx_polymat_midproduct_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, c
 lbx_status lbx_polymat_midproduct_naive_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
                                             const lbx_polmat *b, size_t n0, size_t n1, void *stream);
 
+/* OrderBasis<Modular<uint32_t,uint64_t>>::PM_Basis(sigma, serie, order, shift)
+ * (linbox/algorithms/polynomial-matrix/order-basis.h:219-326): an order basis
+ * sigma (m x m) of the matrix power series serie (m x n, coefficients
+ * 0..order-1 used; serie->size >= order), i.e. sigma.serie = 0 mod x^order,
+ * with shifted row degrees tracked in shift (m entries, in/out, the
+ * reference's std::vector<size_t>&).  Below order 32 (MBASIS_THRESHOLD,
+ * :47-48,232) it is M_Basis; above, the halves are joined by a middle product
+ * (:274, lbx_polymat_midproduct_u32) and a product (:297, lbx_polymat_mul_u32)
+ * on device buffers.  sigma->size must be >= order+1 on entry; coefficients
+ * 0..*degree are written and sigma->size is set to *degree+1 (the
+ * reference's sigma.resize).  Limits: m <= 128, n <= 64, order < 2^15,
+ * shift entries < 2^48 (else LBX_ERR_UNSUPPORTED).  Any p < 2^32.  The call
+ * returns when the basis is complete (the recursion reads each half's degree).
+ * The discrepancy's PLUQ (:470-473) is taken in its rank-profile form: pivot
+ * rows = row rank profile, placed first, every group kept in row order. */
+lbx_status lbx_order_basis_pm_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_polmat *serie,
+                                  size_t order, size_t *shift, size_t *degree, int method, void *stream);
+/* OrderBasis<Modular<uint32_t,uint64_t>>::M_Basis(sigma, serie, order, shift)
+ * (order-basis.h:346-600) for any order < 2^15, in one device workgroup;
+ * arguments and results as lbx_order_basis_pm_u32. */
+lbx_status lbx_order_basis_m_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_polmat *serie,
+                                 size_t order, size_t *shift, size_t *degree, void *stream);
+
 /* FFT_PROFILER analogue (matpoly-mult-fft.h:43-77): with enable = 1, calls
  * record device events between their stages, labelled as in mul_fft
  * ("direct FFT_DIF", "Polfirst to Matfirst", "Pointwise mult",

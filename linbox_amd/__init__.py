@@ -26,7 +26,8 @@ METHOD_AUTO, METHOD_PLAIN_U64, METHOD_INT8_MFMA = 0, 1, 2
 EXPORTED = [
     "lbx_init", "lbx_destroy", "lbx_strerror", "lbx_trim", "lbx_polymat_mul_u32", "lbx_polymat_mul_fft_u32",
     "lbx_ntt_forward_u32", "lbx_ntt_inverse_u32", "lbx_polymat_mul_mp", "lbx_polymat_mul_batch",
-    "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_profile_enable", "lbx_profile_get",
+    "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_order_basis_pm_u32",
+    "lbx_order_basis_m_u32", "lbx_profile_enable", "lbx_profile_get",
 ]
 
 
@@ -81,6 +82,9 @@ def lib():
         L.lbx_polymat_mul_batch.argtypes = [vp, ctypes.POINTER(Job), sz, i32, vp]
         L.lbx_polymat_midproduct_u32.argtypes = [vp, u32, PM, PM, PM, i32, sz, sz, i32, vp]
         L.lbx_polymat_midproduct_naive_u32.argtypes = [vp, u32, PM, PM, PM, sz, sz, vp]
+        SZP = ctypes.POINTER(sz)
+        L.lbx_order_basis_pm_u32.argtypes = [vp, u32, PM, PM, sz, SZP, SZP, i32, vp]
+        L.lbx_order_basis_m_u32.argtypes = [vp, u32, PM, PM, sz, SZP, SZP, vp]
         L.lbx_profile_enable.argtypes = [vp, i32]
         L.lbx_profile_get.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]
         L.lbx_profile_get.restype = i32
@@ -137,9 +141,9 @@ def default_context():
 
 from .polymat import (PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain,  # noqa: E402
                       PolynomialMatrixFFTPrimeMulDomain, PolynomialMatrixNaiveMulDomain, FFT, ntt_forward,
-                      ntt_inverse, mul_mp, mul_batch)
+                      ntt_inverse, mul_mp, mul_batch, OrderBasis)
 
 __all__ = ["Context", "LinboxError", "PolynomialMatrix", "PolynomialMatrixMulDomain", "PolynomialMatrixFFTMulDomain",
-           "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch",
+           "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis",
            "POLFIRST", "MATFIRST", "MATROWFIRST", "METHOD_AUTO", "METHOD_PLAIN_U64", "METHOD_INT8_MFMA", "lib",
            "build", "default_context", "EXPORTED"]

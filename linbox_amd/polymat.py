@@ -210,6 +210,47 @@ class PolynomialMatrixMulDomain(PolynomialMatrixFFTMulDomain):
         return PolynomialMatrixFFTMulDomain.midproduct(self, c, a, b, small_left, n0, n1)
 
 
+class OrderBasis:
+    """OrderBasis<Field> (order-basis.h:181-600): PM_Basis and M_Basis on the device.
+
+    serie: rowdim m x coldim n PolynomialMatrix with at least `order`
+    coefficients; sigma: m x m PolynomialMatrix with at least order+1
+    coefficients; shift: list of m non-negative ints, updated in place like the
+    reference's std::vector<size_t>&.  Both return the degree d of sigma and
+    leave sigma.size() == d+1 (sigma.resize(d+1), :592-597 and :297-298)."""
+
+    def __init__(self, p, ctx=None, method=METHOD_AUTO, stream=None):
+        self.p = int(p)
+        self.ctx = ctx or default_context()
+        self.method = method
+        self.stream = stream
+
+    def _run(self, fn, sigma, serie, order, shift, *extra):
+        m = serie.rowdim()
+        if len(shift) != m:
+            raise LinboxError(5, "shift must have serie.rowdim() entries")
+        sh = (ctypes.c_size_t * m)(*[int(x) for x in shift])
+        deg = ctypes.c_size_t(0)
+        ps, pz = sigma.polmat(), serie.polmat()
+        check(fn(self.ctx.handle, self.p, ctypes.byref(ps), ctypes.byref(pz), int(order), sh, ctypes.byref(deg),
+                 *extra, self.stream))
+        d = int(deg.value)
+        shift[:] = [int(x) for x in sh]
+        if sigma._parent is None and isinstance(sigma.data, np.ndarray):
+            sigma.resize(d + 1)
+        else:
+            sigma._size = d + 1  # device storage keeps its allocation, like a view
+        return d
+
+    def PM_Basis(self, sigma, serie, order, shift):
+        """order-basis.h:219-326 (M_Basis below order 32, :232)."""
+        return self._run(lib().lbx_order_basis_pm_u32, sigma, serie, order, shift, self.method)
+
+    def M_Basis(self, sigma, serie, order, shift):
+        """order-basis.h:346-600."""
+        return self._run(lib().lbx_order_basis_m_u32, sigma, serie, order, shift)
+
+
 class PolynomialMatrixFFTPrimeMulDomain:
     """PolynomialMatrixFFTPrimeMulDomain (matpoly-mult-fft-wordsize-fast.inl:40-190)."""
 

@@ -655,3 +655,222 @@ int or_polymat_mul_crt(const uint32_t *P, int nlimbs, size_t m, size_t k, size_t
     free(ar); free(br); free(cr);
     return OR_OK;
 }
+
+/* ------------------------------------------------------------------------
+ * Order bases: OrderBasis::M_Basis (order-basis.h:346-600) and
+ * OrderBasis::PM_Basis (order-basis.h:219-326).
+ * ---------------------------------------------------------------------- */
+
+/* Row structure of a constant m x n matrix d (row-major) as a rank-profile
+ * revealing PLUQ exposes it (the FFPACK::PLUQ call of order-basis.h:470-473):
+ * the pivot rows are the row rank profile (row i is a pivot when it is not in
+ * the span of rows 0..i-1), and the row permutation puts the pivot rows first
+ * and the other rows after them, each group in increasing order.  rows[] gets
+ * that order, *rank the rank, and Y[(s-rank)*m + t] the coefficients of
+ * non-pivot row rows[s] in the pivot rows rows[0..rank):
+ *   d[rows[s]] = sum_t Y[(s-rank)*m+t] d[rows[t]],
+ * which is the L2 L1^-1 of the reference's PLUQ factors (:485-488 solve
+ * X L1 = -L2 for X = -L2 L1^-1).  Computed here from a reduced echelon form
+ * of the pivot rows and the inverse of their pivot-column submatrix. */
+static int pluq_rows(uint32_t p, size_t m, size_t n, const uint32_t *d, size_t *rows, size_t *rank_out, uint32_t *Y)
+{
+    size_t rmax = m < n ? m : n, rank = 0, npiv = 0, nnon = 0;
+    uint32_t *E = calloc(rmax * n + 1, 4), *v = malloc(n * 4 + 4);
+    uint32_t *M = malloc(rmax * rmax * 4 + 4), *Mi = malloc(rmax * rmax * 4 + 4);
+    size_t *pc = malloc(rmax * sizeof(size_t) + 8), *piv = malloc(m * sizeof(size_t)), *non = malloc(m * sizeof(size_t));
+    if (!E || !v || !M || !Mi || !pc || !piv || !non) return OR_ERR_INVALID_ARG;
+    for (size_t i = 0; i < m; ++i) {
+        memcpy(v, d + i * n, n * 4);
+        for (size_t t = 0; t < rank; ++t) {
+            uint64_t f = v[pc[t]];
+            if (!f) continue;
+            for (size_t c = 0; c < n; ++c) v[c] = (uint32_t)((v[c] + (uint64_t)(p - E[t * n + c]) * f) % p);
+        }
+        size_t c0 = 0;
+        while (c0 < n && !v[c0]) ++c0;
+        if (c0 == n) {
+            non[nnon++] = i;
+            continue;
+        }
+        uint64_t inv = or_invmod(v[c0], p);
+        for (size_t c = 0; c < n; ++c) E[rank * n + c] = (uint32_t)(v[c] * inv % p);
+        for (size_t t = 0; t < rank; ++t) { /* keep the form reduced */
+            uint64_t f = E[t * n + c0];
+            if (!f) continue;
+            for (size_t c = 0; c < n; ++c)
+                E[t * n + c] = (uint32_t)((E[t * n + c] + (uint64_t)(p - E[rank * n + c]) * f) % p);
+        }
+        pc[rank++] = c0;
+        piv[npiv++] = i;
+    }
+    /* M = d[pivot rows][pivot columns], inverted by Gauss-Jordan */
+    for (size_t a = 0; a < rank; ++a)
+        for (size_t b = 0; b < rank; ++b) {
+            M[a * rank + b] = d[piv[a] * n + pc[b]];
+            Mi[a * rank + b] = a == b;
+        }
+    for (size_t col = 0; col < rank; ++col) {
+        size_t r = col;
+        while (r < rank && !M[r * rank + col]) ++r;
+        if (r == rank) return OR_ERR_INVALID_ARG; /* cannot happen: pivot rows are independent */
+        for (size_t b = 0; b < rank; ++b) {
+            uint32_t t = M[col * rank + b]; M[col * rank + b] = M[r * rank + b]; M[r * rank + b] = t;
+            t = Mi[col * rank + b]; Mi[col * rank + b] = Mi[r * rank + b]; Mi[r * rank + b] = t;
+        }
+        uint64_t inv = or_invmod(M[col * rank + col], p);
+        for (size_t b = 0; b < rank; ++b) {
+            M[col * rank + b] = (uint32_t)(M[col * rank + b] * inv % p);
+            Mi[col * rank + b] = (uint32_t)(Mi[col * rank + b] * inv % p);
+        }
+        for (size_t a = 0; a < rank; ++a) {
+            uint64_t f = M[a * rank + col];
+            if (a == col || !f) continue;
+            for (size_t b = 0; b < rank; ++b) {
+                M[a * rank + b] = (uint32_t)((M[a * rank + b] + (uint64_t)(p - M[col * rank + b]) * f) % p);
+                Mi[a * rank + b] = (uint32_t)((Mi[a * rank + b] + (uint64_t)(p - Mi[col * rank + b]) * f) % p);
+            }
+        }
+    }
+    for (size_t s = 0; s < nnon; ++s)
+        for (size_t t = 0; t < rank; ++t) {
+            uint64_t acc = 0;
+            for (size_t b = 0; b < rank; ++b) acc = (acc + (uint64_t)d[non[s] * n + pc[b]] * Mi[b * rank + t]) % p;
+            Y[s * m + t] = (uint32_t)acc;
+        }
+    for (size_t t = 0; t < npiv; ++t) rows[t] = piv[t];
+    for (size_t s = 0; s < nnon; ++s) rows[npiv + s] = non[s];
+    *rank_out = rank;
+    free(E); free(v); free(M); free(Mi); free(pc); free(piv); free(non);
+    return OR_OK;
+}
+
+int or_m_basis(uint32_t p, size_t m, size_t n, const uint32_t *serie, size_t ssz, size_t order, uint64_t *shift,
+               uint32_t *sigma, size_t *deg_out)
+{
+    if (p < 2 || !m || !n || !order || ssz < order || !serie || !shift || !sigma) return OR_ERR_INVALID_ARG;
+    const size_t mm = m * m;
+    uint32_t *sig = calloc((order + 1) * mm, 4), *tmp = malloc(mm * 4), *delta = malloc(m * n * 4);
+    uint32_t *Y = malloc(m * m * 4 + 4);
+    uint64_t *degree = calloc(m, 8), *t64 = malloc(m * 8);
+    size_t *perm = malloc(m * sizeof(size_t)), *rows = malloc(m * sizeof(size_t));
+    if (!sig || !tmp || !delta || !Y || !degree || !t64 || !perm || !rows) return OR_ERR_INVALID_ARG;
+    for (size_t i = 0; i < m; ++i) sig[i * m + i] = 1; /* sigma = identity (:374-377) */
+    size_t maxdeg = 0;
+    for (size_t k = 0; k < order; ++k) {
+        /* selection sort of the shift, same swaps on degree (:388-399) */
+        for (size_t i = 0; i < m; ++i) {
+            size_t im = i;
+            for (size_t j = i + 1; j < m; ++j)
+                if (shift[j] < shift[im]) im = j;
+            uint64_t t = shift[i]; shift[i] = shift[im]; shift[im] = t;
+            t = degree[i]; degree[i] = degree[im]; degree[im] = t;
+            perm[i] = im;
+        }
+        const size_t lim = k < maxdeg ? k : maxdeg;
+        /* Bperm on the rows of sigma[0..lim] (:418-444) */
+        for (size_t l = 0; l <= lim; ++l)
+            for (size_t i = 0; i < m; ++i)
+                if (perm[i] != i)
+                    for (size_t j = 0; j < m; ++j) {
+                        uint32_t t = sig[l * mm + i * m + j];
+                        sig[l * mm + i * m + j] = sig[l * mm + perm[i] * m + j];
+                        sig[l * mm + perm[i] * m + j] = t;
+                    }
+        /* discrepancy: coefficient k of sigma.serie (:420-443) */
+        for (size_t i = 0; i < m; ++i)
+            for (size_t c = 0; c < n; ++c) {
+                uint64_t acc = 0;
+                for (size_t l = 0; l <= lim; ++l)
+                    for (size_t j = 0; j < m; ++j)
+                        acc = (acc + (uint64_t)sig[l * mm + i * m + j] * serie[(j * n + c) * ssz + (k - l)]) % p;
+                delta[i * n + c] = (uint32_t)acc;
+            }
+        size_t rank;
+        int st = pluq_rows(p, m, n, delta, rows, &rank, Y);
+        if (st) return st;
+        /* Qt on sigma[0..lim], then non-pivot rows -= (L2 L1^-1) pivot rows (:509-517) */
+        for (size_t l = 0; l <= lim; ++l) {
+            uint32_t *S = sig + l * mm;
+            for (size_t s = 0; s < m; ++s) memcpy(tmp + s * m, S + rows[s] * m, m * 4);
+            for (size_t s = rank; s < m; ++s)
+                for (size_t j = 0; j < m; ++j) {
+                    uint64_t acc = tmp[s * m + j];
+                    for (size_t t = 0; t < rank; ++t)
+                        acc = (acc + (uint64_t)(p - Y[(s - rank) * m + t]) % p * tmp[t * m + j]) % p;
+                    tmp[s * m + j] = (uint32_t)acc;
+                }
+            memcpy(S, tmp, mm * 4);
+        }
+        /* Qt on degree and shift, then the degree bookkeeping (:546-566) */
+        for (size_t s = 0; s < m; ++s) t64[s] = degree[rows[s]];
+        memcpy(degree, t64, m * 8);
+        for (size_t s = 0; s < m; ++s) t64[s] = shift[rows[s]];
+        memcpy(shift, t64, m * 8);
+        uint64_t dmax = 0;
+        for (size_t i = 0; i < rank; ++i) {
+            if (degree[i] > dmax) dmax = degree[i];
+            degree[i]++;
+            shift[i]++;
+        }
+        for (size_t i = rank; i < m; ++i)
+            if (degree[i] < dmax) degree[i] = dmax;
+        size_t md = maxdeg > dmax + 1 ? maxdeg : (size_t)dmax + 1;
+        maxdeg = md < order ? md : order;
+        /* pivot rows times x (:573-580) */
+        for (size_t l = maxdeg; l-- > 0;)
+            for (size_t i = 0; i < rank; ++i) memcpy(sig + (l + 1) * mm + i * m, sig + l * mm + i * m, m * 4);
+        for (size_t i = 0; i < rank; ++i) memset(sig + i * m, 0, m * 4);
+    }
+    const size_t so = order + 1;
+    for (size_t e = 0; e < mm; ++e)
+        for (size_t l = 0; l < so; ++l) sigma[e * so + l] = l <= maxdeg ? sig[l * mm + e] : 0;
+    *deg_out = maxdeg;
+    free(sig); free(tmp); free(delta); free(Y); free(degree); free(t64); free(perm); free(rows);
+    return OR_OK;
+}
+
+/* product of polfirst polynomial matrices (a m x k of size sa, b k x n of size sb):
+ * the FFT-prime route when 2^ceil(log2(sa+sb-1)) | p-1, the naive one otherwise
+ * (both give the unique product) */
+static int pm_mul(uint32_t p, size_t m, size_t k, size_t n, const uint32_t *a, size_t sa, const uint32_t *b,
+                  size_t sb, uint32_t *c)
+{
+    size_t pts = 1;
+    while (pts < sa + sb - 1) pts <<= 1;
+    if (sa + sb > 2 && p < (1u << 31) && (p - 1) % pts == 0)
+        return or_polymat_mul_fft_prime(p, m, k, n, a, sa, b, sb, c, 0);
+    return or_polymat_mul_naive(p, m, k, n, a, sa, b, sb, c);
+}
+
+int or_pm_basis(uint32_t p, size_t m, size_t n, const uint32_t *serie, size_t ssz, size_t order, uint64_t *shift,
+                uint32_t *sigma, size_t *deg_out)
+{
+    if (p < 2 || !m || !n || !order || ssz < order || !serie || !shift || !sigma) return OR_ERR_INVALID_ARG;
+    if (order <= OR_MBASIS_THRESHOLD) return or_m_basis(p, m, n, serie, ssz, order, shift, sigma, deg_out);
+    const size_t ord1 = order >> 1, ord2 = order - ord1, mm = m * m;
+    uint32_t *s1 = malloc(mm * (ord1 + 1) * 4), *s2 = malloc(mm * (ord2 + 1) * 4);
+    uint32_t *ser = malloc(m * n * order * 4), *ser2 = calloc(m * n * ord2, 4);
+    if (!s1 || !s2 || !ser || !ser2) return OR_ERR_INVALID_ARG;
+    size_t d1, d2;
+    int st = or_pm_basis(p, m, n, serie, ssz, ord1, shift, s1, &d1); /* first half (:249-261) */
+    if (st) return st;
+    /* sigma1 as a polfirst matrix of size d1+1, serie truncated to order */
+    uint32_t *s1c = malloc(mm * (d1 + 1) * 4);
+    for (size_t e = 0; e < mm; ++e) memcpy(s1c + e * (d1 + 1), s1 + e * (ord1 + 1), (d1 + 1) * 4);
+    for (size_t e = 0; e < m * n; ++e) memcpy(ser + e * order, serie + e * ssz, order * 4);
+    /* serie update: coefficients ord1..order-1 of sigma1.serie (:274, midproductgen) */
+    st = or_polymat_midproduct_naive(p, m, m, n, s1c, d1 + 1, ser, order, ser2, ord2, 1, ord1 + 1, order);
+    if (st) return st;
+    st = or_pm_basis(p, m, n, ser2, ord2, ord2, shift, s2, &d2); /* second half (:290) */
+    if (st) return st;
+    uint32_t *s2c = malloc(mm * (d2 + 1) * 4), *prod = malloc(mm * (d1 + d2 + 1) * 4);
+    for (size_t e = 0; e < mm; ++e) memcpy(s2c + e * (d2 + 1), s2 + e * (ord2 + 1), (d2 + 1) * 4);
+    st = pm_mul(p, m, m, m, s2c, d2 + 1, s1c, d1 + 1, prod); /* sigma = sigma2.sigma1 (:297) */
+    if (st) return st;
+    const size_t so = order + 1, d = d1 + d2;
+    for (size_t e = 0; e < mm; ++e)
+        for (size_t l = 0; l < so; ++l) sigma[e * so + l] = l <= d ? prod[e * (d + 1) + l] : 0;
+    *deg_out = d;
+    free(s1); free(s2); free(ser); free(ser2); free(s1c); free(s2c); free(prod);
+    return OR_OK;
+}

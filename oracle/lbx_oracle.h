@@ -20,6 +20,12 @@
  *   or_polymat_midproduct_fft_prime  matpoly-mult-fft-wordsize-fast.inl:209-316
  *                              pinned: cyclic-correlation brute force + golden
  *   or_polymat_midproduct_naive matpoly-mult-naive.h:85-125 pinned: coefficients of or_polymat_mul_naive
+ *   or_m_basis / or_pm_basis   order-basis.h:219-326,346-600
+ *                              pinned: order property, det(sigma) = c x^delta with delta the
+ *                              shift increase, every solution in the row module (brute force
+ *                              over tiny fields).  The choice among order bases
+ *                              rests on the PLUQ row order (fflas-ffpack, not vendored): parity
+ *                              unpinned beyond that restatement.
  *
  * Arithmetic is exact integer arithmetic in uint64_t / unsigned __int128.
  * The reference's lazy Harvey butterflies keep representatives in [0,2p) or
@@ -143,6 +149,23 @@ int or_polymat_midproduct_fft_prime(uint32_t p, size_t m, size_t k, size_t n,
 int or_polymat_midproduct_naive(uint32_t p, size_t m, size_t k, size_t n,
                                 const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
                                 uint32_t *c, size_t sc, int small_left, size_t n0, size_t n1);
+
+/* OrderBasis::M_Basis (order-basis.h:346-600), iterative order basis.
+ * serie: m x n polfirst, stride ssz >= order (coefficients 0..order-1 used).
+ * shift: m entries, updated in place (the reference's shift2).  sigma: m x m
+ * polfirst, stride order+1: coefficients 0..*deg of the basis, zeros after.
+ * Default early termination (EarlyTerm<(size_t)-1>, never).  The PLUQ of the
+ * discrepancy (:470-473) is taken in its rank-profile form: pivot rows =
+ * row rank profile, pivot rows first, every group in increasing row order. */
+int or_m_basis(uint32_t p, size_t m, size_t n, const uint32_t *serie, size_t ssz, size_t order, uint64_t *shift,
+               uint32_t *sigma, size_t *deg);
+/* OrderBasis::PM_Basis (order-basis.h:219-326): M_Basis for order <=
+ * MBASIS_THRESHOLD = 32 (:47-48,232), else halves order1 = order>>1 and
+ * order2 = order-order1, the serie update by middle product and
+ * sigma = sigma2.sigma1.  Same arguments as or_m_basis. */
+#define OR_MBASIS_THRESHOLD 32
+int or_pm_basis(uint32_t p, size_t m, size_t n, const uint32_t *serie, size_t ssz, size_t order, uint64_t *shift,
+                uint32_t *sigma, size_t *deg);
 
 #ifdef __cplusplus
 }

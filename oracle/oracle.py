@@ -49,6 +49,9 @@ def lib():
                                          ctypes.POINTER(i32)]
         L.or_polymat_midproduct_fft_prime.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz, i32, sz, sz]
         L.or_polymat_midproduct_naive.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz, i32, sz, sz]
+        u64p = ctypes.POINTER(u64)
+        L.or_m_basis.argtypes = [u32, sz, sz, _u32p, sz, sz, u64p, _u32p, ctypes.POINTER(sz)]
+        L.or_pm_basis.argtypes = [u32, sz, sz, _u32p, sz, sz, u64p, _u32p, ctypes.POINTER(sz)]
         L.or_stage_forward.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
         L.or_stage_inverse.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
         L.or_stage_pointwise.argtypes = [u32, sz, sz, sz, sz, _u32p, _u32p, _u32p, sz, sz]
@@ -143,6 +146,27 @@ def midproduct_naive(p, m, k, n, a, sa, b, sb, c, small_left=True, n0=0, n1=0):
     st = lib().or_polymat_midproduct_naive(p, m, k, n, _p(a), sa, _p(b), sb, _p(c), sc, 1 if small_left else 0,
                                            n0, n1)
     return st, c
+
+
+def _order_basis(fn, p, m, n, serie, ssz, order, shift):
+    serie = np.ascontiguousarray(serie, dtype=np.uint32)
+    assert serie.size == m * n * ssz
+    sh = np.ascontiguousarray(shift, dtype=np.uint64).copy()
+    sigma = np.zeros(m * m * (order + 1), dtype=np.uint32)
+    deg = ctypes.c_size_t(0)
+    st = fn(p, m, n, _p(serie), ssz, order, sh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), _p(sigma),
+            ctypes.byref(deg))
+    return st, sigma, int(deg.value), sh
+
+
+def m_basis(p, m, n, serie, ssz, order, shift):
+    """OrderBasis::M_Basis: (status, sigma polfirst m*m*(order+1), degree, shift out)."""
+    return _order_basis(lib().or_m_basis, p, m, n, serie, ssz, order, shift)
+
+
+def pm_basis(p, m, n, serie, ssz, order, shift):
+    """OrderBasis::PM_Basis: (status, sigma polfirst m*m*(order+1), degree, shift out)."""
+    return _order_basis(lib().or_pm_basis, p, m, n, serie, ssz, order, shift)
 
 
 def generate_fft_primes(pbound, prodbound, kmin, max_primes=64):

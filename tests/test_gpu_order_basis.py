@@ -1,0 +1,182 @@
+"""GPU parity of the order bases (lbx_order_basis_m_u32 / lbx_order_basis_pm_u32,
+OrderBasis::M_Basis / PM_Basis, order-basis.h:219-326,346-600).
+
+* Word-for-word against the oracle's restatement: the reference test's shape
+  (8 x 4, order 80, tests/test-order-basis.C:19-101) with an FFT prime and a
+  generic prime, shapes with shifts, rank-deficient and zero series, tiny
+  fields, primes >= 2^31 (products through the CRT route), the 128 x 64 limit,
+  layouts and device pointers.
+* At sizes the oracle cannot reach in seconds (32 x 16 at order 4096), the
+  properties: sigma.serie = 0 mod x^order through the device product (itself
+  compared with the oracle elsewhere) and det(sigma) = c x^delta at random points.
+* Error codes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import linbox_amd as lbx
+from linbox_amd import PolynomialMatrix, PolynomialMatrixMulDomain, OrderBasis, LinboxError
+from linbox_amd.hip import DeviceBuffer, synchronize
+from oracle import oracle as orc
+from basischeck import det_is_monomial
+
+pytestmark = pytest.mark.gpu
+FFT_P = 998244353
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lbx.Context(0)
+    yield c
+    c.close()
+
+
+def rand(rng, p, n):
+    return rng.integers(0, p, n, dtype=np.uint64).astype(np.uint32)
+
+
+def gpu_basis(ctx, which, p, m, n, serie, order, shift, layouts=("polfirst", "polfirst"), extra=0):
+    F = PolynomialMatrix.from_polfirst(np.asarray(serie).reshape(m * n, order), m, n, layouts[0])
+    S = PolynomialMatrix(m, m, order + 1 + extra, layouts[1])
+    sh = list(shift)
+    ob = OrderBasis(p, ctx)
+    d = (ob.PM_Basis if which == "pm" else ob.M_Basis)(S, F, order, sh)
+    assert S.size() == d + 1
+    out = np.zeros((m * m, order + 1), dtype=np.uint32)
+    out[:, :d + 1] = S.to_polfirst()
+    return d, out.reshape(-1), sh
+
+
+def compare(ctx, which, p, m, n, serie, order, shift, **kw):
+    say(which, p, m, n, order)
+    d, sig, sh = gpu_basis(ctx, which, p, m, n, serie, order, shift, **kw)
+    st, ref, dref, shref = (orc.pm_basis if which == "pm" else orc.m_basis)(p, m, n, serie, order, order, shift)
+    assert st == 0
+    assert d == dref
+    assert sh == [int(x) for x in shref]
+    assert np.array_equal(sig, ref)
+
+
+@pytest.mark.parametrize("p", [FFT_P, 1000000007])
+def test_reference_test_shape(ctx, p):
+    rng = np.random.default_rng(80)
+    serie = rand(rng, p, 8 * 4 * 80)
+    for which in ("m", "pm"):
+        compare(ctx, which, p, 8, 4, serie, 80, [0] * 8)
+
+
+@pytest.mark.parametrize("m,n,order", [(4, 2, 33), (6, 3, 64), (5, 5, 40), (3, 4, 70), (1, 1, 45), (16, 8, 130),
+                                       (65, 3, 6), (128, 64, 3), (20, 10, 257)])
+def test_shapes_and_shifts(ctx, m, n, order):
+    rng = np.random.default_rng(m * 1000 + n * 100 + order)
+    serie = rand(rng, FFT_P, m * n * order)
+    shift = [int(x) for x in rng.integers(0, 5, m)]
+    compare(ctx, "m", FFT_P, m, n, serie, order, shift)
+    if order > 32:
+        compare(ctx, "pm", FFT_P, m, n, serie, order, shift)
+
+
+def test_special_series(ctx):
+    rng = np.random.default_rng(5)
+    m, n, order = 6, 3, 50
+    F = rand(rng, FFT_P, m * n * order).reshape(m, n, order).astype(np.uint64)
+    F[:, 2] = (F[:, 0] * 3 + F[:, 1]) % FFT_P
+    F[4] = 0
+    for which in ("m", "pm"):
+        compare(ctx, which, FFT_P, m, n, F.astype(np.uint32).reshape(-1), order, [0] * m)
+        compare(ctx, which, FFT_P, 3, 2, np.zeros(3 * 2 * order, np.uint32), order, [1, 0, 2])
+    one = np.zeros(40, np.uint32)
+    one[0] = 1
+    compare(ctx, "pm", FFT_P, 1, 1, one, 40, [7])
+
+
+@pytest.mark.parametrize("p", [2, 3, 5, 97])
+def test_tiny_fields(ctx, p):
+    rng = np.random.default_rng(p)
+    for (m, n, order) in [(2, 1, 3), (3, 2, 5), (4, 2, 40)]:
+        compare(ctx, "m", p, m, n, rand(rng, p, m * n * order), order, [0] * m)
+    compare(ctx, "pm", p, 4, 2, rand(rng, p, 4 * 2 * 40), 40, [0] * 4)
+
+
+@pytest.mark.parametrize("p", [4294967291, 3221225473, 2013265921])
+def test_large_primes(ctx, p):
+    """p >= 2^31: the products of PM_Basis go through FFT primes and CRT."""
+    rng = np.random.default_rng(p % 1000)
+    serie = rand(rng, p, 6 * 3 * 70)
+    compare(ctx, "m", p, 6, 3, serie, 70, [0] * 6)
+    compare(ctx, "pm", p, 6, 3, serie, 70, [0] * 6)
+
+
+def test_layouts_and_device_pointers(ctx):
+    rng = np.random.default_rng(21)
+    m, n, order = 6, 4, 90
+    serie = rand(rng, FFT_P, m * n * order)
+    st, ref, dref, shref = orc.pm_basis(FFT_P, m, n, serie, order, order, [0] * m)
+    for lay in [("matfirst", "matrowfirst"), ("matrowfirst", "matfirst")]:
+        say("layouts", lay)
+        d, sig, sh = gpu_basis(ctx, "pm", FFT_P, m, n, serie, order, [0] * m, layouts=lay, extra=3)
+        assert d == dref and np.array_equal(sig, ref)
+    # device storage, polfirst
+    say("device pointers")
+    dser = DeviceBuffer.from_numpy(serie)
+    dsig = DeviceBuffer((m * m * (order + 1)) * 4)
+    dsig.zero()
+    F = PolynomialMatrix(m, n, order, data=dser)
+    S = PolynomialMatrix(m, m, order + 1, data=dsig)
+    sh = [0] * m
+    d = OrderBasis(FFT_P, ctx).PM_Basis(S, F, order, sh)
+    synchronize()
+    got = dsig.download().reshape(m * m, order + 1)
+    assert d == dref and sh == [int(x) for x in shref]
+    assert np.array_equal(got[:, :d + 1], ref.reshape(m * m, order + 1)[:, :d + 1])
+
+
+def test_full_size_properties(ctx):
+    """32 x 16 serie at order 4096 (a block Wiedemann shape): order property and det."""
+    p, m, n, order = 2013265921, 32, 16, 4096
+    rng = np.random.default_rng(4096)
+    serie = rand(rng, p, m * n * order)
+    say("pm full size", m, n, order)
+    d, sig, sh = gpu_basis(ctx, "pm", p, m, n, serie, order, [0] * m)
+    delta = sum(sh)
+    assert delta == n * order  # a generic serie has full column rank at every step
+    # sigma.serie mod x^order through the device product
+    S = PolynomialMatrix.from_polfirst(sig.reshape(m * m, order + 1)[:, :d + 1], m, m)
+    F = PolynomialMatrix.from_polfirst(serie.reshape(m * n, order), m, n)
+    T = PolynomialMatrix(m, n, d + order)
+    PolynomialMatrixMulDomain(p, ctx).mul(T, S, F)
+    assert not T.to_polfirst()[:, :order].any()
+    assert T.to_polfirst()[:, order:].any()
+    assert det_is_monomial(sig, order + 1, p, m, delta, [12345, 678901, 1357911])
+
+
+def test_errors(ctx):
+    rng = np.random.default_rng(1)
+    F = PolynomialMatrix.from_polfirst(rand(rng, FFT_P, 4 * 2 * 10).reshape(8, 10), 4, 2)
+    ob = OrderBasis(FFT_P, ctx)
+    cases = [
+        (PolynomialMatrix(4, 4, 10), 10, [0] * 4, 5),          # sigma smaller than order+1
+        (PolynomialMatrix(4, 4, 12), 11, [0] * 4, 5),          # serie shorter than order
+        (PolynomialMatrix(3, 3, 12), 10, [0] * 4, 5),          # sigma not m x m
+        (PolynomialMatrix(4, 4, 12), 10, [0, 0, 0, 1 << 50], 11),  # shift beyond 2^48
+        (PolynomialMatrix(4, 4, 12), 0, [0] * 4, 7),           # order 0
+    ]
+    for S, order, sh, code in cases:
+        with pytest.raises(LinboxError) as e:
+            ob.PM_Basis(S, F, order, sh)
+        assert e.value.status == code
+    big = PolynomialMatrix.from_polfirst(rand(rng, FFT_P, 129 * 2 * 4).reshape(258, 4), 129, 2)
+    with pytest.raises(LinboxError) as e:
+        ob.M_Basis(PolynomialMatrix(129, 129, 5), big, 4, [0] * 129)
+    assert e.value.status == 11
+    wide = PolynomialMatrix.from_polfirst(rand(rng, FFT_P, 2 * 65 * 4).reshape(130, 4), 2, 65)
+    with pytest.raises(LinboxError) as e:
+        ob.M_Basis(PolynomialMatrix(2, 2, 5), wide, 4, [0] * 2)
+    assert e.value.status == 11
