@@ -15,11 +15,13 @@ The JSON line also carries:
                 the committed rocprofv3 PMC summary (profiles/hbm_traffic_c3.txt);
   kernels       the same figures for every kernel of the product;
   batch_config5 1024 uneven products as one batch call vs one call per product;
+  order_basis   PM_Basis of a 64 x 32 matrix power series at order 4096;
   cpu_baseline  the oracle's restatement of the reference's single-prime
                 mul_fft timed on this host on a bounded sample of the same
                 workload and extrapolated to one product.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-batch]
+                       [--skip-order-basis]
 """
 import argparse
 import json
@@ -193,6 +195,62 @@ def run_batch(steps=5):
             "speedup": round(ms_single / ms_batch, 2)}
 
 
+def run_order_basis(skip_cpu, reps=3):
+    """PM_Basis of a 64 x 32 serie (the shape of benchmark-order-basis.C:340-346) at order
+    4096 over p = 998244353, device-resident; the call returns when the basis is done, so
+    wall time per call is the figure.  One M_Basis base case (order 32) is timed too, for
+    the share of the 128 base cases.  With the CPU leg, the oracle's PM_Basis and the
+    device's are timed at order 128, where the oracle finishes in seconds."""
+    import linbox_amd as lbx
+    from linbox_amd import hip, PolynomialMatrix, OrderBasis
+
+    p, m, n, order = 998244353, 64, 32, 4096
+    rng = np.random.default_rng(0x4C494E424F58 + 6)
+    serie = rng.integers(0, p, m * n * order, dtype=np.uint64).astype(np.uint32)
+    ctx = lbx.Context(0)
+    dser = hip.DeviceBuffer.from_numpy(serie)
+    dsig = hip.DeviceBuffer(m * m * (order + 1) * 4)
+    ob = OrderBasis(p, ctx)
+
+    def call(fn, o):
+        sh = [0] * m
+        t0 = time.perf_counter()
+        d = fn(PolynomialMatrix(m, m, o + 1, data=dsig), PolynomialMatrix(m, n, o, data=dser), o, sh)
+        return time.perf_counter() - t0, d, sh
+
+    def call_sub(ob, o, dsub):
+        t0 = time.perf_counter()
+        ob.PM_Basis(PolynomialMatrix(m, m, o + 1, data=dsig), PolynomialMatrix(m, n, o, data=dsub), o, [0] * m)
+        return time.perf_counter() - t0
+
+    call(ob.PM_Basis, order)  # workspace, twiddles
+    runs = [call(ob.PM_Basis, order) for _ in range(reps)]
+    t_pm = min(r[0] for r in runs)
+    d, sh = runs[0][1], runs[0][2]
+    call(ob.M_Basis, 32)
+    t_mb = min(call(ob.M_Basis, 32)[0] for _ in range(reps))
+    out = {"workload": "PM_Basis, 64x32 serie, order 4096, p=998244353 (M_Basis below order 32)",
+           "ms": round(t_pm * 1e3, 2), "bases_per_s": round(1.0 / t_pm, 2), "degree": d,
+           "shift_sum": int(sum(sh)), "m_basis_order32_ms": round(t_mb * 1e3, 3),
+           "base_cases": order // 32, "base_case_share": round(order // 32 * t_mb / t_pm, 3)}
+    if not skip_cpu:
+        from oracle import oracle as orc
+        o = 128
+        sub = serie.reshape(m * n, order)[:, :o].copy().reshape(-1)
+        dsub = hip.DeviceBuffer.from_numpy(sub)
+        t_gpu = min(call_sub(ob, o, dsub) for _ in range(reps))
+        t0 = time.perf_counter()
+        st = orc.pm_basis(p, m, n, sub, o, o, [0] * m)[0]
+        t_cpu = time.perf_counter() - t0
+        assert st == 0
+        out["order128"] = {"gpu_ms": round(t_gpu * 1e3, 2), "cpu_oracle_s": round(t_cpu, 2), "cores": 1,
+                           "kind": "port"}
+    ctx.close()
+    print("order basis: %.2f ms per PM_Basis (64x32, order 4096); M_Basis order 32: %.3f ms"
+          % (t_pm * 1e3, t_mb * 1e3), file=sys.stderr, flush=True)
+    return out
+
+
 class _Ptr:
     """A device address usable as PolynomialMatrix storage."""
 
@@ -267,6 +325,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="omit the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--skip-batch", action="store_true", help="omit the 1024-product batch measurement")
+    ap.add_argument("--skip-order-basis", action="store_true", help="omit the PM_Basis measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -283,6 +342,7 @@ def main():
         return
     ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
     batch = None if args.skip_batch else run_batch()
+    order_basis = None if args.skip_order_basis else run_order_basis(args.skip_cpu)
     pts = 1 << 14
     roofline, kernels = kernel_roofline(stages, kernel_bytes(M, K, N, SA, SB, pts))
     if args.skip_cpu:
@@ -317,6 +377,7 @@ def main():
         "wall_s_timed": round(wall, 4),
         "roofline": roofline,
         "batch_config5": batch,
+        "order_basis": order_basis,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

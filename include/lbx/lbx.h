@@ -176,7 +176,7 @@ lbx_status lbx_polymat_midproduct_naive_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat
  * (:274, lbx_polymat_midproduct_u32) and a product (:297, lbx_polymat_mul_u32)
  * on device buffers.  sigma->size must be >= order+1 on entry; coefficients
  * 0..*degree are written and sigma->size is set to *degree+1 (the
- * reference's sigma.resize).  Limits: m <= 128, n <= 64, order < 2^15,
+ * reference's sigma.resize).  Limits: m <= 128, n <= 64, m*n <= 4096, order < 2^15,
  * shift entries < 2^48 (else LBX_ERR_UNSUPPORTED).  Any p < 2^32.  The call
  * returns when the basis is complete (the recursion reads each half's degree).
  * The discrepancy's PLUQ (:470-473) is taken in its rank-profile form: pivot

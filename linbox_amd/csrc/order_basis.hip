@@ -51,13 +51,6 @@ __device__ __forceinline__ uint32_t mulm(uint32_t a, uint32_t b, uint32_t p, uin
     return red64((uint64_t)a * b, p, mu);
 }
 
-// a*x - b*y mod p
-__device__ __forceinline__ uint32_t axmby(uint32_t a, uint32_t x, uint32_t b, uint32_t y, uint32_t p, uint64_t mu)
-{
-    const uint32_t u = mulm(a, x, p, mu), v = mulm(b, y, p, mu);
-    return u >= v ? u - v : u + (p - v);
-}
-
 __device__ uint32_t invm(uint32_t a, uint32_t p, uint64_t mu)  // a^(p-2), a != 0
 {
     uint32_t r = 1, b = a;
@@ -66,6 +59,15 @@ __device__ uint32_t invm(uint32_t a, uint32_t p, uint64_t mu)  // a^(p-2), a != 
         b = mulm(b, b, p, mu);
     }
     return r;
+}
+
+// Montgomery product x y 2^-32 mod p for odd p < 2^31 (pinv = -p^-1 mod 2^32):
+// x y + q p < 2^63, so the quotient by 2^32 is below 2p.
+__device__ __forceinline__ uint32_t mont(uint32_t x, uint32_t y, uint32_t p, uint32_t pinv)
+{
+    const uint64_t t = (uint64_t)x * y;
+    const uint32_t q = (uint32_t)t * pinv;
+    return csub((uint32_t)((t + (uint64_t)q * p) >> 32), p);
 }
 
 // Sum of products kept as a 64-bit word plus a count of wraps of 2^64.
@@ -94,26 +96,118 @@ __global__ void __launch_bounds__(256) gather_coeff_major(DevView v, size_t coun
     }
 }
 
+// Sum of products for p < 2^31: products are below 2^62, and the word is
+// folded at 2^62 (2^62 = f62 mod p) after every second product, so it stays
+// below 2^62 + 2^33 + 2^63.
+struct AccSmall {
+    uint64_t v = 0;
+    __device__ __forceinline__ void add2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint64_t f62)
+    {
+        v += (uint64_t)a0 * b0;
+        v += (uint64_t)a1 * b1;
+        v = (v & (((uint64_t)1 << 62) - 1)) + (v >> 62) * f62;
+    }
+    __device__ __forceinline__ void add1(uint32_t a, uint32_t b, uint64_t f62)
+    {
+        v += (uint64_t)a * b;
+        v = (v & (((uint64_t)1 << 62) - 1)) + (v >> 62) * f62;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t p, uint64_t mu, uint32_t) const { return red64(v, p, mu); }
+};
+
+// Any p < 2^32: a 64-bit word and a count of its wraps.
+struct AccWide {
+    Acc a;
+    __device__ __forceinline__ void add2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint64_t)
+    {
+        a.add(a0, b0);
+        a.add(a1, b1);
+    }
+    __device__ __forceinline__ void add1(uint32_t x, uint32_t y, uint64_t) { a.add(x, y); }
+    __device__ __forceinline__ uint32_t get(uint32_t p, uint64_t mu, uint32_t r64) const { return a.get(p, mu, r64); }
+};
+
+// sum_j x[j] y[j*ystride], len terms
+template <typename A>
+__device__ __forceinline__ void dot(A &acc, const uint32_t *x, const uint32_t *y, int ystride, int len, uint64_t f62)
+{
+    int j = 0;
+    for (; j + 1 < len; j += 2) acc.add2(x[j], y[j * ystride], x[j + 1], y[(j + 1) * ystride], f62);
+    if (j < len) acc.add1(x[j], y[j * ystride], f62);
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) const void global_cvoid;
+
+// Copies rows of `len` words from global memory into LDS with
+// global_load_lds_dword (lane i of an instruction fills LDS word base + i):
+// one instruction per 64 words of a row, one row per wave in turn, and no
+// register staging, so every load of the copy can be in flight at once.  The
+// caller waits (s_waitcnt 0) and synchronises before reading the copy.
+template <typename Src, typename Dst>
+__device__ __forceinline__ void stage_rows(int nrows, int len, Src src, Dst dst, int wave, int lane)
+{
+    for (int r = wave; r < nrows; r += kMbWaves)
+        for (int j0 = 0; j0 < len; j0 += 64)
+            if (j0 + lane < len)
+                __builtin_amdgcn_global_load_lds((global_cvoid *)(src(r) + j0 + lane), (lds_void *)(dst(r) + j0), 4,
+                                                 0, 0);
+}
+
+constexpr int kMbStageWords = 24576;  // LDS words for staged coefficient matrices
+constexpr int kMbMaxOut = 4;          // m*n <= kMbMaxOut*kMbThreads
+constexpr unsigned kNone = 0xffffffffu;
+
 // One workgroup, kMbThreads threads.  S: serie, coefficient-major
-// S[(t*m + j)*n + c]; W: (order+1) coefficient matrices of m x m words,
-// W[(l*m + phys)*m + j], phys a physical row.  Dynamic LDS: R (m x n) and
-// Z (m x n) words.
+// S[(t*m + j)*n + c]; W: order+1 coefficient matrices of m x m words,
+// W[(l*m + phys)*m + j], phys a physical row.  Dynamic LDS: R, Z and D
+// (m x n words each) and `stage` words for staged coefficient matrices.
+//
+// Per step k:
+//  1. the shift sort (wave 0, keys in registers, one wave-wide minimum per position);
+//  2. the discrepancy: rows that were pivots at step k-1 reuse that step's
+//     discrepancy row (x.sigma_t has coefficient k+1 equal to coefficient k of
+//     sigma_t, order-basis.h:446-456), the others are summed over staged blocks
+//     of W[l] rows and S[k-l];
+//  3. elimination, one barrier per pivot: the pivot column is never written, so
+//     every thread reads its row's multiplier from it, and the next pivot is
+//     searched for in the same pass;
+//  4. non-pivot rows -= combination . pivot rows and pivot rows times x, on
+//     staged blocks of W taken from the highest coefficient down, so the
+//     shifted pivot rows never overwrite a coefficient still to be read.
+// MONT (odd p < 2^31): the elimination runs on Montgomery forms x 2^32 mod p,
+// whose zero pattern is the same and whose ratios Z/scale are the same.
+template <typename A, bool MONT>
 __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__restrict__ S, uint32_t *__restrict__ W,
                                                              DevView out, uint64_t *__restrict__ shift_io,
                                                              uint64_t *__restrict__ deg_out, int order, int m, int n,
-                                                             uint32_t p, uint64_t mu, uint32_t r64)
+                                                             int stage, uint32_t p, uint64_t mu, uint32_t r64,
+                                                             uint64_t f62, uint32_t pinv, uint32_t r2m, uint32_t one)
 {
+    // one = 1 in the elimination's representation (2^32 mod p with MONT)
+    auto mul = [&](uint32_t x, uint32_t y) -> uint32_t {
+        if constexpr (MONT) return mont(x, y, p, pinv);
+        else return mulm(x, y, p, mu);
+    };
+    auto amb = [&](uint32_t a, uint32_t x, uint32_t b, uint32_t y) -> uint32_t {
+        const uint32_t u = mul(a, x), v = mul(b, y);
+        return u >= v ? u - v : u + (p - v);
+    };
     extern __shared__ uint32_t dyn[];
-    uint32_t *R = dyn;          // discrepancy rows under elimination, logical row order
-    uint32_t *Z = dyn + m * n;  // coefficients over the pivot rows, then the combination
+    uint32_t *R = dyn;              // discrepancy rows under elimination, logical order
+    uint32_t *Z = dyn + m * n;      // coefficients over the pivot rows, then the combination
+    uint32_t *D = dyn + 2 * m * n;  // discrepancy before elimination, by physical row
+    uint32_t *ST = dyn + 3 * m * n; // staged coefficient matrices
     __shared__ uint64_t shift[kMbMaxM], degree[kMbMaxM], tmp64[kMbMaxM], tmp64b[kMbMaxM];
-    __shared__ int rowmap[kMbMaxM], tmpmap[kMbMaxM], pivot_of[kMbMaxM], newpos[kMbMaxM], oldrow[kMbMaxM];
+    __shared__ int rowmap[kMbMaxM], tmpmap[kMbMaxM], pivot_of[kMbMaxM], oldrow[kMbMaxM], comp[kMbMaxM];
+    __shared__ int was_piv[kMbMaxM];  // by physical row: pivot at the previous step
     __shared__ uint32_t scale[kMbMaxM];
-    __shared__ unsigned smin[2];
-    __shared__ int s_maxdeg;
+    __shared__ unsigned smin[3];
+    __shared__ int s_maxdeg, s_nc;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t mm = (size_t)m * m, nw = (size_t)(order + 1) * mm;
+    const int mn = m * n;
     for (size_t i = tid; i < nw; i += kMbThreads) W[i] = 0;
     __syncthreads();
     for (int i = tid; i < m; i += kMbThreads) {
@@ -121,20 +215,63 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
         rowmap[i] = i;
         shift[i] = shift_io[i];
         degree[i] = 0;
+        was_piv[i] = 0;
     }
     if (tid == 0) s_maxdeg = 0;
     __syncthreads();
 
     for (int k = 0; k < order; ++k) {
         const int maxdeg = s_maxdeg, lim = k < maxdeg ? k : maxdeg;
-        // ---- selection sort of the shift (order-basis.h:388-399), wave 0 ----
+        // ---- 1. selection sort of the shift (order-basis.h:388-399) ----
         if (wave == 0) {
-            // positions lane and lane+64; key = shift << 7 | position
             uint64_t sh0 = lane < m ? shift[lane] : 0, sh1 = lane + 64 < m ? shift[lane + 64] : 0;
             uint64_t dg0 = lane < m ? degree[lane] : 0, dg1 = lane + 64 < m ? degree[lane + 64] : 0;
             int rm0 = lane < m ? rowmap[lane] : 0, rm1 = lane + 64 < m ? rowmap[lane + 64] : 0;
-            for (int i = 0; i < m - 1; ++i) {
-                uint64_t best = ~(uint64_t)0;
+            // shifts spread over less than 2^24: 32-bit keys (shift - min) << 7 | position,
+            // and only the positions' origins move; the arrays are gathered once at the end
+            uint64_t lo = ~(uint64_t)0, hi = 0;
+            if (lane < m) { lo = sh0; hi = sh0; }
+            if (lane + 64 < m) { lo = sh1 < lo ? sh1 : lo; hi = sh1 > hi ? sh1 : hi; }
+            for (int off = 32; off; off >>= 1) {
+                const uint64_t a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+                lo = a < lo ? a : lo;
+                hi = b > hi ? b : hi;
+            }
+            const bool narrow = hi - lo < ((uint64_t)1 << 24);
+            if (narrow) {
+                uint32_t v0 = (uint32_t)(sh0 - lo), v1 = (uint32_t)(sh1 - lo);
+                int o0 = lane, o1 = lane + 64;
+                for (int i = 0; i < m - 1; ++i) {
+                    uint32_t best = 0xffffffffu;
+                    if (lane < m && lane >= i) best = v0 << 7 | (uint32_t)lane;
+                    if (lane + 64 < m && lane + 64 >= i) {
+                        const uint32_t c1 = v1 << 7 | (uint32_t)(lane + 64);
+                        best = c1 < best ? c1 : best;
+                    }
+                    for (int off = 32; off; off >>= 1) {
+                        const uint32_t o = __shfl_xor(best, off);
+                        best = o < best ? o : best;
+                    }
+                    const int im = (int)(best & 127);
+                    if (im != i) {
+                        const int li = i & 63, lm = im & 63;
+                        const bool hi_i = i >= 64, hi_m = im >= 64;
+                        const uint32_t vi = __shfl(hi_i ? v1 : v0, li), vm = best >> 7;
+                        const int oi = __shfl(hi_i ? o1 : o0, li), om = __shfl(hi_m ? o1 : o0, lm);
+                        if (lane == li) {
+                            if (hi_i) { v1 = vm; o1 = om; } else { v0 = vm; o0 = om; }
+                        }
+                        if (lane == lm) {
+                            if (hi_m) { v1 = vi; o1 = oi; } else { v0 = vi; o0 = oi; }
+                        }
+                    }
+                }
+                // gather by origin (all lanes read before any lane writes)
+                if (lane < m) { sh0 = shift[o0]; dg0 = degree[o0]; rm0 = rowmap[o0]; }
+                if (lane + 64 < m) { sh1 = shift[o1]; dg1 = degree[o1]; rm1 = rowmap[o1]; }
+            }
+            for (int i = 0; !narrow && i < m - 1; ++i) {
+                uint64_t best = ~(uint64_t)0;  // key = shift << 7 | position: first minimum
                 if (lane < m && lane >= i) best = sh0 << 7 | (uint64_t)lane;
                 if (lane + 64 < m && lane + 64 >= i) {
                     const uint64_t c1 = sh1 << 7 | (uint64_t)(lane + 64);
@@ -145,7 +282,7 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                     best = o < best ? o : best;
                 }
                 const int im = (int)(best & 127);
-                if (im != i) {  // swap positions i and im (uniform branch)
+                if (im != i) {
                     const int li = i & 63, lm = im & 63;
                     const bool hi_i = i >= 64, hi_m = im >= 64;
                     const uint64_t si = __shfl(hi_i ? sh1 : sh0, li), sm = __shfl(hi_m ? sh1 : sh0, lm);
@@ -161,68 +298,154 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
             }
             if (lane < m) { shift[lane] = sh0; degree[lane] = dg0; rowmap[lane] = rm0; }
             if (lane + 64 < m) { shift[lane + 64] = sh1; degree[lane + 64] = dg1; rowmap[lane + 64] = rm1; }
+            // rows to sum: those that were not pivots at the previous step
+            int base = 0;
+            for (int h = 0; h < m; h += 64) {
+                const int r = h + lane;
+                const bool need = r < m && !was_piv[h == 0 ? rm0 : rm1];
+                const uint64_t bal = __ballot(need);
+                if (need) comp[base + __popcll(bal & ((1ull << lane) - 1))] = r;
+                base += __popcll(bal);
+            }
+            if (lane == 0) s_nc = base;
         }
         __syncthreads();
-        // ---- discrepancy: coefficient k of sigma.serie (order-basis.h:416-444) ----
-        for (int o = tid; o < m * n; o += kMbThreads) {
-            const int r = o / n, c = o - r * n;
-            const uint32_t *wrow = W + (size_t)rowmap[r] * m;
-            Acc acc;
-            for (int l = 0; l <= lim; ++l) {
-                const uint32_t *wl = wrow + (size_t)l * mm;
-                const uint32_t *sl = S + ((size_t)(k - l) * m) * n + c;
-                for (int j = 0; j < m; ++j) acc.add(wl[j], sl[(size_t)j * n]);
+        // ---- 2. discrepancy: coefficient k of sigma.serie (order-basis.h:416-456) ----
+        const int nc = s_nc;
+        {
+            A acc[kMbMaxOut];
+            const int blk = nc * m + mn;  // staged words per coefficient
+            const int lb = stage / blk > 0 ? stage / blk : 1;
+            for (int l0 = 0; nc && l0 <= lim; l0 += lb) {
+                const int l1 = l0 + lb < lim + 1 ? l0 + lb : lim + 1;
+                stage_rows(
+                    (l1 - l0) * nc, m,
+                    [&](int pr) {
+                        const int lq = pr / nc, i = pr - lq * nc;
+                        return W + ((size_t)(l0 + lq) * m + rowmap[comp[i]]) * m;
+                    },
+                    [&](int pr) {
+                        const int lq = pr / nc, i = pr - lq * nc;
+                        return ST + lq * blk + i * m;
+                    },
+                    wave, lane);
+                // serie blocks S[k-l], rows of n words
+                stage_rows(
+                    (l1 - l0) * m, n,
+                    [&](int pr) {
+                        const int lq = pr / m, j = pr - lq * m;
+                        return S + (size_t)(k - l0 - lq) * mn + j * n;
+                    },
+                    [&](int pr) {
+                        const int lq = pr / m, j = pr - lq * m;
+                        return ST + lq * blk + nc * m + j * n;
+                    },
+                    wave, lane);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+#pragma unroll
+                for (int t = 0; t < kMbMaxOut; ++t) {
+                    const int o = tid + t * kMbThreads;
+                    if (o < nc * n) {
+                        const int i = o / n, c = o - i * n;
+                        for (int lq = 0; lq < l1 - l0; ++lq)
+                            dot(acc[t], ST + lq * blk + i * m, ST + lq * blk + nc * m + c, n, m, f62);
+                    }
+                }
+                __syncthreads();
             }
-            R[o] = acc.get(p, mu, r64);
+#pragma unroll
+            for (int t = 0; t < kMbMaxOut; ++t) {
+                const int o = tid + t * kMbThreads;
+                if (o < nc * n) {
+                    const int i = o / n, c = o - i * n;
+                    R[comp[i] * n + c] = acc[t].get(p, mu, r64);
+                }
+            }
+        }
+        for (int o = tid; o < mn; o += kMbThreads) {
+            const int r = o / n, c = o - r * n, ph = rowmap[r];
+            if (was_piv[ph]) R[o] = D[ph * n + c];
             Z[o] = 0;
         }
         for (int i = tid; i < m; i += kMbThreads) {
-            scale[i] = 1;
+            scale[i] = one;
             pivot_of[i] = -1;
         }
         if (tid == 0) {
-            smin[0] = 0xffffffffu;
-            smin[1] = 0xffffffffu;
+            smin[0] = kNone;
+            smin[1] = kNone;
+            smin[2] = kNone;
         }
         __syncthreads();
-        // ---- row rank profile and combinations (the PLUQ of :470-488) ----
-        int next = 0, rank = 0;
+        {
+            bool found = false;
+            for (int r = wave; r < m; r += kMbWaves) {  // one row per wave, one lane per column
+                const uint32_t v = lane < n ? R[r * n + lane] : 0u;
+                if (lane < n) {
+                    D[rowmap[r] * n + lane] = v;
+                    if constexpr (MONT) R[r * n + lane] = mont(v, r2m, p, pinv);
+                }
+                const uint64_t bal = __ballot(v != 0);
+                if (bal && !found) {
+                    found = true;
+                    if (lane == 0) atomicMin(&smin[0], (unsigned)(r * n + __builtin_ctzll(bal)));
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 3. row rank profile and combinations (the PLUQ of :470-488) ----
+        int rank = 0;
+        uint64_t pmask = 0;  // pivot columns so far
         for (;;) {
-            const int slot = rank & 1;
-            // first nonzero entry (row-major) among rows >= next
-            for (int o = next * n + tid; o < m * n; o += kMbThreads)
-                if (R[o]) atomicMin(&smin[slot], (unsigned)o);
-            __syncthreads();
-            const unsigned hit = smin[slot];
-            if (hit == 0xffffffffu) break;
+            const unsigned hit = smin[rank % 3];
+            if (hit == kNone) break;
             const int s = (int)(hit / n), c0 = (int)(hit - (unsigned)s * n);
-            const uint32_t a = R[s * n + c0];
-            // rows below s: row = a*row - b*pivot row; one wave per row, one lane per column
-            for (int r = s + 1 + wave; r < m; r += kMbWaves) {
+            const uint32_t a = R[s * n + c0], sc_s = scale[s];
+            pmask |= (uint64_t)1 << c0;
+            const int wdt = n + rank + 1;  // per row: n columns, rank+1 combination words
+            unsigned *nxt = &smin[(rank + 1) % 3];
+            bool found = false;  // this wave's rows come in increasing order: only its first hit counts
+            for (int r = s + 1 + wave; r < m; r += kMbWaves) {  // one row per wave
                 const uint32_t b = R[r * n + c0];
-                if (b == 0) continue;
-                uint32_t rv = 0, zv = 0;
-                if (lane < n) rv = axmby(a, R[r * n + lane], b, R[s * n + lane], p, mu);
-                if (lane < rank) zv = axmby(a, Z[r * n + lane], b, Z[s * n + lane], p, mu);
-                else if (lane == rank) zv = mulm(b, scale[s], p, mu);
-                const uint32_t sc = mulm(a, scale[r], p, mu);
-                if (lane < n) R[r * n + lane] = rv;
-                if (lane <= rank) Z[r * n + lane] = zv;
-                if (lane == 0) scale[r] = sc;
+                bool nz = false;  // lane's column (j = lane < n) nonzero after the update
+                for (int j = lane; j < wdt; j += 64) {
+                    if (j < n) {
+                        if ((pmask >> j) & 1) continue;
+                        uint32_t v = R[r * n + j];
+                        if (b) {
+                            v = amb(a, v, b, R[s * n + j]);
+                            R[r * n + j] = v;
+                        }
+                        nz = v != 0;
+                    } else if (b) {
+                        const int t = j - n;
+                        if (t < rank) Z[r * n + t] = amb(a, Z[r * n + t], b, Z[s * n + t]);
+                        else Z[r * n + t] = mul(b, sc_s);
+                    }
+                }
+                if (b && lane == 0) scale[r] = mul(a, scale[r]);
+                const uint64_t bal = __ballot(nz);
+                if (bal && !found) {
+                    found = true;
+                    if (lane == 0) atomicMin(nxt, (unsigned)(r * n + __builtin_ctzll(bal)));
+                }
             }
             if (tid == 0) {
                 pivot_of[s] = rank;
-                smin[slot ^ 1] = 0xffffffffu;
+                smin[(rank + 2) % 3] = kNone;
             }
-            next = s + 1;
             ++rank;
             __syncthreads();
         }
-        // non-pivot rows: combination = Z / scale
+        // non-pivot rows: combination = Z / scale (one inverse per row, one lane each)
+        for (int r = tid; r < m; r += kMbThreads)
+            if (pivot_of[r] < 0 && scale[r] != 1) scale[r] = invm(scale[r], p, mu);
+        __syncthreads();
         for (int r = wave; r < m; r += kMbWaves) {
-            if (pivot_of[r] >= 0) continue;
-            const uint32_t inv = scale[r] == 1 ? 1u : invm(scale[r], p, mu);
-            if (lane < rank) Z[r * n + lane] = mulm(Z[r * n + lane], inv, p, mu);
+            if (pivot_of[r] >= 0 || scale[r] == 1) continue;
+            const uint32_t inv = scale[r];
+            if (lane < rank) Z[r * n + lane] = mulm(Z[r * n + lane], inv, p, mu);  // plain product: Z~/scale~ = Z/scale
         }
         // Qt: pivot rows first, then the others, each in increasing order
         for (int r = tid; r < m; r += kMbThreads) {
@@ -232,11 +455,11 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                 for (int q = 0; q < r; ++q) before += pivot_of[q] < 0 ? 1 : 0;
                 pos = rank + before;
             }
-            newpos[r] = pos;
             oldrow[pos] = r;
             tmpmap[pos] = rowmap[r];
             tmp64[pos] = shift[r];
             tmp64b[pos] = degree[r];
+            was_piv[rowmap[r]] = pivot_of[r] >= 0;
         }
         __syncthreads();
         for (int i = tid; i < m; i += kMbThreads) {
@@ -245,23 +468,42 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
             degree[i] = tmp64b[i];
         }
         __syncthreads();
-        // ---- sigma rows rank.. -= combination . pivot rows, coefficients 0..lim (:509-517) ----
+        // ---- 4. non-pivot rows -= combination . pivot rows (:509-517), pivot rows times x (:573-580) ----
         {
-            const int nrows = m - rank;
-            const size_t total = (size_t)(lim + 1) * nrows * m;
-            for (size_t o = tid; o < total; o += kMbThreads) {
-                const int j = (int)(o % m);
-                const size_t q = o / m;
-                const int sidx = rank + (int)(q % nrows), l = (int)(q / nrows);
-                const uint32_t *Y = Z + (size_t)oldrow[sidx] * n;
-                const uint32_t *wl = W + (size_t)l * mm + j;
-                Acc acc;
-                for (int t = 0; t < rank; ++t) acc.add(Y[t], wl[(size_t)rowmap[t] * m]);
-                const uint32_t sub = acc.get(p, mu, r64);
-                uint32_t *dst = W + (size_t)l * mm + (size_t)rowmap[sidx] * m + j;
-                const uint32_t v = *dst;
-                *dst = v >= sub ? v - sub : v + (p - sub);
+            const int lb = stage / (int)mm > 0 ? stage / (int)mm : 1;
+            for (int l1 = lim + 1; l1 > 0;) {
+                const int l0 = l1 - lb > 0 ? l1 - lb : 0;
+                stage_rows(
+                    (l1 - l0) * m, m,
+                    [&](int pr) {
+                        const int lq = pr / m, i = pr - lq * m;
+                        return W + ((size_t)(l0 + lq) * m + rowmap[i]) * m;
+                    },
+                    [&](int pr) { return ST + pr * m; }, wave, lane);
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                const int nrows = m - rank;
+                for (int pr = wave; pr < (l1 - l0) * nrows; pr += kMbWaves) {
+                    const int lq = pr / nrows, si = rank + pr - lq * nrows;
+                    const uint32_t *st = ST + lq * (int)mm, *y = Z + oldrow[si] * n;
+                    uint32_t *dst = W + ((size_t)(l0 + lq) * m + rowmap[si]) * m;
+                    for (int j = lane; j < m; j += 64) {
+                        A acc;
+                        dot(acc, y, st + j, m, rank, f62);
+                        const uint32_t sub = acc.get(p, mu, r64), v = st[si * m + j];
+                        dst[j] = v >= sub ? v - sub : v + (p - sub);
+                    }
+                }
+                for (int pr = wave; pr < (l1 - l0) * rank; pr += kMbWaves) {
+                    const int lq = pr / rank, t = pr - lq * rank;
+                    const uint32_t *src = ST + lq * (int)mm + t * m;
+                    uint32_t *dst = W + ((size_t)(l0 + lq + 1) * m + rowmap[t]) * m;
+                    for (int j = lane; j < m; j += 64) dst[j] = src[j];
+                }
+                __syncthreads();
+                l1 = l0;
             }
+            for (int o = tid; o < rank * m; o += kMbThreads) W[(size_t)rowmap[o / m] * m + o % m] = 0;
         }
         // ---- degrees and shifts (:546-566) ----
         if (tid == 0) {
@@ -274,17 +516,6 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
             for (int i = rank; i < m; ++i) degree[i] = degree[i] < dmax ? dmax : degree[i];
             const uint64_t md = (uint64_t)maxdeg > dmax + 1 ? (uint64_t)maxdeg : dmax + 1;
             s_maxdeg = (int)(md < (uint64_t)order ? md : (uint64_t)order);
-        }
-        __syncthreads();
-        // ---- pivot rows times x (:573-580) ----
-        {
-            const int nmd = s_maxdeg;
-            for (int o = tid; o < rank * m; o += kMbThreads) {
-                const int t = o / m, j = o - t * m;
-                uint32_t *col = W + (size_t)rowmap[t] * m + j;
-                for (int l = nmd - 1; l >= 0; --l) col[(size_t)(l + 1) * mm] = col[(size_t)l * mm];
-                col[0] = 0;
-            }
         }
         __syncthreads();
     }
@@ -361,11 +592,25 @@ lbx_status run_m_basis(PmRun &R, const DevView &serie, const DevView &sigma, siz
     LBX_TRY_HIP(hipGetLastError());
     const uint64_t mu = ~(uint64_t)0 / R.p;
     const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % R.p), r64 = (uint32_t)((uint64_t)r32 * r32 % R.p);
-    const size_t lds = 2 * m * n * 4;
-    LBX_TRY_HIP(hipFuncSetAttribute((const void *)m_basis_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
-    hipLaunchKernelGGL(m_basis_kernel, dim3(1), dim3(kMbThreads), lds, R.s, R.mb.S, R.mb.W, sigma, R.mb.shift,
-                       R.mb.deg, (int)order, (int)m, (int)n, R.p, mu, r64);
+    const uint64_t f62 = (((uint64_t)1 << 62) % R.p);
+    // staged words: enough for every coefficient of a step, at most kMbStageWords
+    const size_t need = std::max((order + 1) * (m * m + m * n), m * m);
+    const int stage = (int)std::min(need, (size_t)kMbStageWords);
+    const size_t lds = (3 * m * n + (size_t)stage) * 4;
+    const bool mont_ok = R.p < (1u << 31) && (R.p & 1);
+    uint32_t pinv = 0, r2m = 0, one = 1;
+    if (mont_ok) {
+        uint32_t inv = R.p;  // Newton iteration for p^-1 mod 2^32
+        for (int i = 0; i < 5; ++i) inv *= 2u - R.p * inv;
+        pinv = 0u - inv;
+        one = r32;
+        r2m = (uint32_t)((uint64_t)r32 * r32 % R.p);
+    }
+    auto kern = mont_ok ? m_basis_kernel<AccSmall, true>
+                        : (R.p < (1u << 31) ? m_basis_kernel<AccSmall, false> : m_basis_kernel<AccWide, false>);
+    LBX_TRY_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kMbThreads), lds, R.s, R.mb.S, R.mb.W, sigma, R.mb.shift, R.mb.deg,
+                       (int)order, (int)m, (int)n, stage, R.p, mu, r64, f62, pinv, r2m, one);
     LBX_TRY_HIP(hipGetLastError());
     uint64_t d = 0;
     LBX_TRY_HIP(hipMemcpyAsync(&d, R.mb.deg, 8, hipMemcpyDeviceToHost, R.s));
@@ -436,7 +681,8 @@ lbx_status order_basis(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_po
     const size_t m = serie->rows, n = serie->cols;
     if (sigma->rows != m || sigma->cols != m) return LBX_ERR_SIZE_MISMATCH;
     if (serie->size < order || sigma->size < order + 1) return LBX_ERR_SIZE_MISMATCH;
-    if (m > (size_t)kMbMaxM || n > (size_t)kMbMaxN) return LBX_ERR_UNSUPPORTED;
+    if (m > (size_t)kMbMaxM || n > (size_t)kMbMaxN || m * n > (size_t)kMbMaxOut * kMbThreads)
+        return LBX_ERR_UNSUPPORTED;
     if (order > ((size_t)1 << kMaxLdsLog) - 1 || order > 0x7fffffff) return LBX_ERR_UNSUPPORTED;
     for (size_t i = 0; i < m; ++i)
         if (shift[i] >= kShiftLimit) return LBX_ERR_UNSUPPORTED;

@@ -73,7 +73,7 @@ def test_reference_test_shape(ctx, p):
 
 
 @pytest.mark.parametrize("m,n,order", [(4, 2, 33), (6, 3, 64), (5, 5, 40), (3, 4, 70), (1, 1, 45), (16, 8, 130),
-                                       (65, 3, 6), (128, 64, 3), (20, 10, 257)])
+                                       (65, 3, 6), (128, 32, 3), (64, 64, 5), (20, 10, 257)])
 def test_shapes_and_shifts(ctx, m, n, order):
     rng = np.random.default_rng(m * 1000 + n * 100 + order)
     serie = rand(rng, FFT_P, m * n * order)
@@ -95,6 +95,15 @@ def test_special_series(ctx):
     one = np.zeros(40, np.uint32)
     one[0] = 1
     compare(ctx, "pm", FFT_P, 1, 1, one, 40, [7])
+
+
+def test_wide_shift_range(ctx):
+    """Shifts spread over more than 2^24 take the 64-bit sort keys."""
+    rng = np.random.default_rng(24)
+    serie = rand(rng, FFT_P, 6 * 3 * 40)
+    shift = [0, 1 << 30, 5, 1 << 40, 7, 3]
+    compare(ctx, "m", FFT_P, 6, 3, serie, 40, shift)
+    compare(ctx, "pm", FFT_P, 6, 3, serie, 40, shift)
 
 
 @pytest.mark.parametrize("p", [2, 3, 5, 97])
@@ -179,4 +188,8 @@ def test_errors(ctx):
     wide = PolynomialMatrix.from_polfirst(rand(rng, FFT_P, 2 * 65 * 4).reshape(130, 4), 2, 65)
     with pytest.raises(LinboxError) as e:
         ob.M_Basis(PolynomialMatrix(2, 2, 5), wide, 4, [0] * 2)
+    assert e.value.status == 11
+    large = PolynomialMatrix.from_polfirst(rand(rng, FFT_P, 128 * 64 * 2).reshape(8192, 2), 128, 64)
+    with pytest.raises(LinboxError) as e:  # m*n above 4096
+        ob.M_Basis(PolynomialMatrix(128, 128, 3), large, 2, [0] * 128)
     assert e.value.status == 11
