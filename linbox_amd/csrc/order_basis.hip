@@ -396,33 +396,33 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
         __syncthreads();
         // ---- 3. row rank profile and combinations (the PLUQ of :470-488) ----
         int rank = 0;
-        uint64_t pmask = 0;  // pivot columns so far
         for (;;) {
             const unsigned hit = smin[rank % 3];
             if (hit == kNone) break;
             const int s = (int)(hit / n), c0 = (int)(hit - (unsigned)s * n);
             const uint32_t a = R[s * n + c0], sc_s = scale[s];
-            pmask |= (uint64_t)1 << c0;
-            const int wdt = n + rank + 1;  // per row: n columns, rank+1 combination words
+            // Each row below s is the word vector [R_r | Z_r], updated as a.x - b.y
+            // against [R_s | Z_s] with Z_s[rank] read as -scale_s (so that
+            // Z_r[rank] = b scale_s).  Columns of earlier pivots are zero in both
+            // rows and column c0 becomes a b - b a = 0; b is read by the whole wave
+            // before its lanes store, and no other wave touches row r.
+            const uint32_t neg_sc = p - sc_s;
+            const int wdt = n + rank + 1;
             unsigned *nxt = &smin[(rank + 1) % 3];
             bool found = false;  // this wave's rows come in increasing order: only its first hit counts
             for (int r = s + 1 + wave; r < m; r += kMbWaves) {  // one row per wave
                 const uint32_t b = R[r * n + c0];
                 bool nz = false;  // lane's column (j = lane < n) nonzero after the update
                 for (int j = lane; j < wdt; j += 64) {
-                    if (j < n) {
-                        if ((pmask >> j) & 1) continue;
-                        uint32_t v = R[r * n + j];
-                        if (b) {
-                            v = amb(a, v, b, R[s * n + j]);
-                            R[r * n + j] = v;
-                        }
-                        nz = v != 0;
-                    } else if (b) {
-                        const int t = j - n;
-                        if (t < rank) Z[r * n + t] = amb(a, Z[r * n + t], b, Z[s * n + t]);
-                        else Z[r * n + t] = mul(b, sc_s);
+                    const bool isr = j < n;
+                    uint32_t *xp = isr ? R + r * n + j : Z + r * n + (j - n);
+                    uint32_t v = *xp;
+                    if (b) {
+                        const uint32_t y = isr ? R[s * n + j] : (j - n == rank ? neg_sc : Z[s * n + (j - n)]);
+                        v = amb(a, v, b, y);
+                        *xp = v;
                     }
+                    nz = nz || (isr && v != 0);
                 }
                 if (b && lane == 0) scale[r] = mul(a, scale[r]);
                 const uint64_t bal = __ballot(nz);
