@@ -272,6 +272,50 @@ private:
     PolynomialMatrixNaiveMulDomain naive_;
 };
 
+// OrderBasis<Field> (order-basis.h:181-600): PM_Basis and M_Basis on the
+// device.  serie: m x n with at least `order` coefficients; sigma: m x m,
+// resized by the call to degree+1 (sigma.resize, :297-298, :592-597); shift:
+// m entries, updated in place.  Both return the degree of sigma.
+class OrderBasis {
+public:
+    typedef Modular Field;
+    explicit OrderBasis(const Field &F, lbx_ctx *ctx = nullptr, int method = LBX_METHOD_AUTO, void *stream = nullptr)
+        : fld_(&F), ctx_(ctx ? ctx : default_ctx()), method_(method), stream_(stream) {}
+    const Field &field() const { return *fld_; }
+
+    template <typename PMatrix1, typename PMatrix2>
+    size_t PM_Basis(PMatrix1 &sigma, const PMatrix2 &serie, size_t order, std::vector<size_t> &shift)
+    {
+        return run(true, sigma, serie, order, shift);
+    }
+    template <typename PMatrix1, typename PMatrix2>
+    size_t M_Basis(PMatrix1 &sigma, const PMatrix2 &serie, size_t order, std::vector<size_t> &shift)
+    {
+        return run(false, sigma, serie, order, shift);
+    }
+
+private:
+    template <typename PMatrix1, typename PMatrix2>
+    size_t run(bool divide, PMatrix1 &sigma, const PMatrix2 &serie, size_t order, std::vector<size_t> &shift)
+    {
+        if (shift.size() != serie.rowdim()) throw LinboxError(LBX_ERR_SIZE_MISMATCH, "shift size != serie.rowdim()");
+        if (sigma.size() < order + 1) sigma.resize(order + 1);
+        lbx_polmat ds = sigma.desc(), dz = serie.desc();
+        size_t deg = 0;
+        const uint32_t p = (uint32_t)fld_->cardinality();
+        if (divide)
+            check(lbx_order_basis_pm_u32(ctx_, p, &ds, &dz, order, shift.data(), &deg, method_, stream_));
+        else
+            check(lbx_order_basis_m_u32(ctx_, p, &ds, &dz, order, shift.data(), &deg, stream_));
+        sigma.resize(deg + 1);
+        return deg;
+    }
+    const Field *fld_;
+    lbx_ctx *ctx_;
+    int method_;
+    void *stream_;
+};
+
 class PolynomialMatrixFFTPrimeMulDomain {
 public:
     typedef Modular Field;

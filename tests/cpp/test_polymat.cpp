@@ -197,6 +197,31 @@ static void test_midproduct(uint32_t p, std::mt19937_64 &g)
     EXPECT(ok, "naive midproduct writes only its window");
 }
 
+// check_sigma of tests/test-order-basis.C:19-45: sigma.serie = 0 mod x^order
+// and sigma != 0, for M_Basis and PM_Basis on an 8 x 4 serie of order 80.
+static void test_order_basis(uint32_t p, std::mt19937_64 &g)
+{
+    Modular F(p);
+    const size_t m = 8, n = 4, d = 80;
+    PolynomialMatrix<matfirst> Serie(F, m, n, d);
+    fill(Serie, p, g);
+    PolynomialMatrixMulDomain PMD(F);
+    OrderBasis SB(F);
+    for (int variant = 0; variant < 2; ++variant) {
+        PolynomialMatrix<matfirst> Sigma(F, m, m, d + 1);
+        std::vector<size_t> shift(m, 0);
+        const size_t deg = variant ? SB.PM_Basis(Sigma, Serie, d, shift) : SB.M_Basis(Sigma, Serie, d, shift);
+        PolynomialMatrix<polfirst> T(F, m, n, 1);
+        PMD.mul(T, Sigma, Serie);
+        bool zero = Sigma.size() == deg + 1, nonzero = false;
+        for (size_t e = 0; e < m * n; ++e)
+            for (size_t t = 0; t < d; ++t) zero &= T.get(e, t) == 0;
+        for (size_t e = 0; e < m * m; ++e)
+            for (size_t t = 0; t <= deg; ++t) nonzero |= Sigma.get(e, t) != 0;
+        EXPECT(zero && nonzero, variant ? "PM_Basis: sigma.serie = 0 mod x^order" : "M_Basis: sigma.serie = 0 mod x^order");
+    }
+}
+
 int main(int argc, char **argv)
 {
     const uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 20191;
@@ -210,6 +235,7 @@ int main(int argc, char **argv)
         test_mul<matrowfirst, matfirst, matfirst>(p, 5, 33, LBX_METHOD_AUTO, g);
         test_views_and_errors(p, g);
         test_midproduct(p, g);
+        test_order_basis(p, g);
     }
     test_mul<polfirst, polfirst, polfirst>(1000000007u, 8, 64, LBX_METHOD_AUTO, g);  // CRT route
     if (failures) {
