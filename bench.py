@@ -16,12 +16,14 @@ The JSON line also carries:
   kernels       the same figures for every kernel of the product;
   batch_config5 1024 uneven products as one batch call vs one call per product;
   order_basis   PM_Basis of a 64 x 32 matrix power series at order 4096;
+  other_configs 16x16 deg 255, 64x64 deg 4095 (both forms), 32x32 deg 1023 over a
+                192-bit modulus, and a PM_Basis-level middle product;
   cpu_baseline  the oracle's restatement of the reference's single-prime
                 mul_fft timed on this host on a bounded sample of the same
                 workload and extrapolated to one product.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-batch]
-                       [--skip-order-basis]
+                       [--skip-order-basis] [--skip-configs]
 """
 import argparse
 import json
@@ -195,6 +197,94 @@ def run_batch(steps=5):
             "speedup": round(ms_single / ms_batch, 2)}
 
 
+def staged_bytes(m, k, n, sa, sb):
+    """Algorithmic HBM bytes of one staged product (forward, pointwise, inverse passes)."""
+    D = sa + sb - 2
+    pts = 1
+    while pts < D + 1:
+        pts <<= 1
+    return 4 * (m * k + k * n) * (sa + pts) + 4 * (m * k + k * n + m * n) * pts + 4 * m * n * (pts + D + 1)
+
+
+def run_configs(reps=10):
+    """The other configurations of the workload table, timed with HIP events on one stream:
+    16x16 deg 255 (p = 998244353), 64x64 deg 4095 (p = 2013265921, both pointwise forms),
+    32x32 deg 1023 over a 192-bit P (14 FFT primes and CRT), and the middle product of a
+    PM_Basis level (64x64 sigma of degree 2048 times a 64x32 serie of 4096 terms)."""
+    import linbox_amd as lbx
+    from linbox_amd import hip, PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain
+
+    ctx = lbx.Context(0)
+    stream = hip.Stream()
+    rng = np.random.default_rng(0x4C494E424F58 + 7)
+
+    def timed(fn):
+        fn()
+        stream.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record(stream.value)
+        for _ in range(reps):
+            fn()
+        e1.record(stream.value)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def dev(words):
+        return hip.DeviceBuffer.from_numpy(words)
+
+    out = {}
+    for name, p, dim, deg, methods in [("16x16_deg255", 998244353, 16, 255, [lbx.METHOD_AUTO]),
+                                        ("64x64_deg4095", 2013265921, 64, 4095,
+                                         [lbx.METHOD_INT8_MFMA, lbx.METHOD_PLAIN_U64])]:
+        sa = deg + 1
+        da = dev(rng.integers(0, p, dim * dim * sa, dtype=np.uint64).astype(np.uint32))
+        db = dev(rng.integers(0, p, dim * dim * sa, dtype=np.uint64).astype(np.uint32))
+        dc = hip.DeviceBuffer(dim * dim * (2 * sa - 1) * 4)
+        A, B = PolynomialMatrix(dim, dim, sa, data=da), PolynomialMatrix(dim, dim, sa, data=db)
+        C = PolynomialMatrix(dim, dim, 2 * sa - 1, data=dc)
+        for meth in methods:
+            dom = PolynomialMatrixMulDomain(p, ctx, meth, stream=stream.value)
+            ms = timed(lambda: dom.mul(C, A, B))
+            label = {lbx.METHOD_AUTO: "auto", lbx.METHOD_INT8_MFMA: "int8_mfma", lbx.METHOD_PLAIN_U64: "plain_u64"}[meth]
+            gbs = staged_bytes(dim, dim, dim, sa, sa) / (ms * 1e-3) / 1e9
+            out["%s_%s" % (name, label)] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1),
+                                            "staged_gbs": round(gbs, 1), "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 3)}
+    # 192-bit modulus through 14 FFT primes (fixture P of the CRT tests)
+    with open(os.path.join(ROOT, "tests", "golden", "crt_mul.json")) as f:
+        crt = json.load(f)
+    Pm, nl = int(crt["P_hex"], 16), crt["nlimbs"]
+    top = (Pm >> (32 * (nl - 1))) & 0xFFFFFFFF
+
+    def rand_mp(count):
+        w = rng.integers(0, 1 << 32, (count, nl), dtype=np.uint64)
+        w[:, nl - 1] = rng.integers(0, top, count, dtype=np.uint64)  # below P
+        return w.astype(np.uint32).reshape(-1)
+    dim, sa = 32, 1024
+    da, db = dev(rand_mp(dim * dim * sa)), dev(rand_mp(dim * dim * sa))
+    dc = hip.DeviceBuffer(dim * dim * (2 * sa - 1) * nl * 4)
+    A = PolynomialMatrix(dim, dim, sa, data=da, nlimbs=nl)
+    B = PolynomialMatrix(dim, dim, sa, data=db, nlimbs=nl)
+    C = PolynomialMatrix(dim, dim, 2 * sa - 1, data=dc, nlimbs=nl)
+    ms = timed(lambda: lbx.mul_mp(Pm, C, A, B, ctx=ctx, stream=stream.value))
+    out["32x32_deg1023_P192"] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1), "fft_primes": 14}
+    # middle product of a PM_Basis level: coefficients 2048..4095 of sigma1.serie
+    p, m, n, o1, order = 998244353, 64, 32, 2048, 4096
+    ds = dev(rng.integers(0, p, m * m * (o1 + 1), dtype=np.uint64).astype(np.uint32))
+    dz = dev(rng.integers(0, p, m * n * order, dtype=np.uint64).astype(np.uint32))
+    dc = hip.DeviceBuffer(m * n * (order - o1) * 4)
+    S, Z = PolynomialMatrix(m, m, o1 + 1, data=ds), PolynomialMatrix(m, n, order, data=dz)
+    C = PolynomialMatrix(m, n, order - o1, data=dc)
+    dom = PolynomialMatrixFFTMulDomain(p, ctx, lbx.METHOD_AUTO, stream=stream.value)
+    ms = timed(lambda: dom.midproduct(C, S, Z, True, o1 + 1, order))
+    pts = order
+    mbytes = (4 * (m * m * (o1 + 1 + pts) + m * n * (order + pts)) + 4 * (m * m + 2 * m * n) * pts +
+              4 * m * n * (pts + order - o1))
+    out["midproduct_64x64.64x32_order4096"] = {"ms": round(ms, 4), "staged_gbs": round(mbytes / (ms * 1e-3) / 1e9, 1)}
+    ctx.close()
+    print("configs: " + ", ".join("%s %.3f ms" % (k, v["ms"]) for k, v in out.items()), file=sys.stderr, flush=True)
+    return out
+
+
 def run_order_basis(skip_cpu, reps=3):
     """PM_Basis of a 64 x 32 serie (the shape of benchmark-order-basis.C:340-346) at order
     4096 over p = 998244353, device-resident; the call returns when the basis is done, so
@@ -326,6 +416,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--skip-batch", action="store_true", help="omit the 1024-product batch measurement")
     ap.add_argument("--skip-order-basis", action="store_true", help="omit the PM_Basis measurement")
+    ap.add_argument("--skip-configs", action="store_true", help="omit the other workload configurations")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -343,6 +434,7 @@ def main():
     ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
     batch = None if args.skip_batch else run_batch()
     order_basis = None if args.skip_order_basis else run_order_basis(args.skip_cpu)
+    configs = None if args.skip_configs else run_configs()
     pts = 1 << 14
     roofline, kernels = kernel_roofline(stages, kernel_bytes(M, K, N, SA, SB, pts))
     if args.skip_cpu:
@@ -378,6 +470,7 @@ def main():
         "roofline": roofline,
         "batch_config5": batch,
         "order_basis": order_basis,
+        "other_configs": configs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
