@@ -409,26 +409,46 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
             const uint32_t neg_sc = p - sc_s;
             const int wdt = n + rank + 1;
             unsigned *nxt = &smin[(rank + 1) % 3];
-            bool found = false;  // this wave's rows come in increasing order: only its first hit counts
-            for (int r = s + 1 + wave; r < m; r += kMbWaves) {  // one row per wave
-                const uint32_t b = R[r * n + c0];
-                bool nz = false;  // lane's column (j = lane < n) nonzero after the update
-                for (int j = lane; j < wdt; j += 64) {
-                    const bool isr = j < n;
-                    uint32_t *xp = isr ? R + r * n + j : Z + r * n + (j - n);
-                    uint32_t v = *xp;
-                    if (b) {
-                        const uint32_t y = isr ? R[s * n + j] : (j - n == rank ? neg_sc : Z[s * n + (j - n)]);
-                        v = amb(a, v, b, y);
-                        *xp = v;
-                    }
-                    nz = nz || (isr && v != 0);
+            // Every row is updated, b = 0 included (the row and its scale are then both
+            // multiplied by a, which keeps Z/scale and the zero pattern): no branch per
+            // row, and two rows per pass for independent chains.  Lane j holds word j
+            // and word j+64 of the pivot vector (n <= 64, so word j+64 is a combination word).
+            const int j1 = lane + 64;
+            const bool has0 = lane < wdt, has1 = j1 < wdt, isr0 = lane < n;
+            const int zo0 = lane - n, zo1 = j1 - n;
+            const uint32_t y0 = !has0 ? 0u : (isr0 ? R[s * n + lane] : (zo0 == rank ? neg_sc : Z[s * n + zo0]));
+            const uint32_t y1 = !has1 ? 0u : (zo1 == rank ? neg_sc : Z[s * n + zo1]);
+            auto update = [&](int r, uint32_t b) -> bool {  // returns: lane's column nonzero
+                bool nz = false;
+                if (has0) {
+                    uint32_t *xp = isr0 ? R + r * n + lane : Z + r * n + zo0;
+                    const uint32_t v = amb(a, *xp, b, y0);
+                    *xp = v;
+                    nz = isr0 && v != 0;
                 }
-                if (b && lane == 0) scale[r] = mul(a, scale[r]);
-                const uint64_t bal = __ballot(nz);
-                if (bal && !found) {
+                if (has1) {
+                    uint32_t *xp = Z + r * n + zo1;
+                    *xp = amb(a, *xp, b, y1);
+                }
+                return nz;
+            };
+            bool found = false;  // this wave's rows come in increasing order: only its first hit counts
+            for (int r = s + 1 + wave; r < m; r += 2 * kMbWaves) {
+                const int r2 = r + kMbWaves;
+                const bool two = r2 < m;
+                const uint32_t b = R[r * n + c0], b2 = two ? R[r2 * n + c0] : 0u;
+                const bool nz = update(r, b);
+                const bool nz2 = two && update(r2, b2);
+                if (lane == 0) {
+                    scale[r] = mul(a, scale[r]);
+                    if (two) scale[r2] = mul(a, scale[r2]);
+                }
+                const uint64_t bal = __ballot(nz), bal2 = __ballot(nz2);
+                if (!found && (bal || bal2)) {
                     found = true;
-                    if (lane == 0) atomicMin(nxt, (unsigned)(r * n + __builtin_ctzll(bal)));
+                    const unsigned hit = bal ? (unsigned)(r * n + __builtin_ctzll(bal))
+                                             : (unsigned)(r2 * n + __builtin_ctzll(bal2));
+                    if (lane == 0) atomicMin(nxt, hit);
                 }
             }
             if (tid == 0) {
