@@ -132,6 +132,18 @@ def test_layouts_and_device_pointers(ctx):
         say("layouts", lay)
         d, sig, sh = gpu_basis(ctx, "pm", FFT_P, m, n, serie, order, [0] * m, layouts=lay, extra=3)
         assert d == dref and np.array_equal(sig, ref)
+    # sigma as a view (SubPolynomialMatrix window) of a larger matrix
+    say("sigma view")
+    F = PolynomialMatrix.from_polfirst(serie.reshape(m * n, order), m, n)
+    big = PolynomialMatrix(m, m, order + 9, "matfirst")
+    big.data[:] = 7
+    view = big.at(4, 4 + order)
+    sh = [0] * m
+    d = OrderBasis(FFT_P, ctx).PM_Basis(view, F, order, sh)
+    assert d == dref and view.size() == d + 1
+    assert np.array_equal(view.to_polfirst(), ref.reshape(m * m, order + 1)[:, :d + 1])
+    outside = big.to_polfirst()[:, :4]
+    assert (outside == 7).all()
     # device storage, polfirst
     say("device pointers")
     dser = DeviceBuffer.from_numpy(serie)
