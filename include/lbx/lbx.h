@@ -58,7 +58,8 @@ typedef enum lbx_layout {
 } lbx_layout;
 
 typedef enum lbx_method {
-    LBX_METHOD_AUTO = 0,      /* INT8 MFMA pointwise when it applies, else PLAIN_U64 */
+    LBX_METHOD_AUTO = 0,      /* INT8 MFMA pointwise when m,k,n >= 16 and (m*k*n >= 2^18 or at most 2^9
+                                 points), else PLAIN_U64 (the faster form on gfx950); both are bit-exact */
     LBX_METHOD_PLAIN_U64 = 1, /* pointwise products on vector ALUs, 64-bit accumulation, delayed reduction */
     LBX_METHOD_INT8_MFMA = 2  /* pointwise products on INT8 MFMA over balanced signed 8-bit limbs */
 } lbx_method;

@@ -203,11 +203,19 @@ private:
     hipEvent_t start_ = nullptr;
 };
 
-int choose_method(int method, size_t m, size_t k, size_t n)
+// AUTO picks the INT8 form where it measured faster on gfx950 (both forms are
+// bit-exact): for large products (m*k*n >= 2^18) and for short transforms
+// (at most 2^9 points); in between, the split transforms the INT8 form needs
+// cost more than the VALU pointwise they replace (16^3 at 2^14 points: 0.25 ms
+// INT8, 0.08 ms plain; 32^3 at 2^13: 0.20 vs 0.13 ms; 64^3 at 2^11: 0.14 vs
+// 0.16 ms; 16^3 at 2^9: 0.032 vs 0.036 ms).  MFMA tiles are 32x32 with a
+// 64-deep inner step, so every dimension must be at least 16.
+int choose_method(int method, size_t m, size_t k, size_t n, unsigned L)
 {
     if (method == LBX_METHOD_PLAIN_U64 || method == LBX_METHOD_INT8_MFMA) return method;
-    // MFMA tiles are 32x32 with a 64-deep inner step; small products waste them.
-    return (m >= 16 && n >= 16 && k >= 16) ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
+    if (m < 16 || n < 16 || k < 16) return LBX_METHOD_PLAIN_U64;
+    const bool big = (double)m * k * n >= (double)(1u << 18) || L <= 9;
+    return big ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
 }
 
 // Bytes of head -> tail intermediate per chunk.  The default, 2 GiB, gives one
@@ -330,7 +338,7 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         (mid->small_left ? a : b) = rv;
     }
     const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
-    method = choose_method(method, m, k, n);
+    method = choose_method(method, m, k, n, L);
 
     Stages st(ctx, s);
     if (method == LBX_METHOD_INT8_MFMA && L >= 8) {
