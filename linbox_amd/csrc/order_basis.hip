@@ -248,16 +248,16 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                         const uint32_t c1 = v1 << 7 | (uint32_t)(lane + 64);
                         best = c1 < best ? c1 : best;
                     }
-                    for (int off = 32; off; off >>= 1) {
-                        const uint32_t o = __shfl_xor(best, off);
-                        best = o < best ? o : best;
-                    }
+                    best = __ockl_wfred_min_u32(best);  // wave-wide minimum (DPP), uniform
                     const int im = (int)(best & 127);
                     if (im != i) {
                         const int li = i & 63, lm = im & 63;
                         const bool hi_i = i >= 64, hi_m = im >= 64;
-                        const uint32_t vi = __shfl(hi_i ? v1 : v0, li), vm = best >> 7;
-                        const int oi = __shfl(hi_i ? o1 : o0, li), om = __shfl(hi_m ? o1 : o0, lm);
+                        // uniform lanes: read them with v_readlane
+                        const uint32_t vi = (uint32_t)__builtin_amdgcn_readlane((int)(hi_i ? v1 : v0), li);
+                        const uint32_t vm = best >> 7;
+                        const int oi = __builtin_amdgcn_readlane(hi_i ? o1 : o0, li);
+                        const int om = __builtin_amdgcn_readlane(hi_m ? o1 : o0, lm);
                         if (lane == li) {
                             if (hi_i) { v1 = vm; o1 = om; } else { v0 = vm; o0 = om; }
                         }
