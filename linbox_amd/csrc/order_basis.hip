@@ -101,16 +101,16 @@ __global__ void __launch_bounds__(256) gather_coeff_major(DevView v, size_t coun
 // below 2^62 + 2^33 + 2^63.
 struct AccSmall {
     uint64_t v = 0;
-    __device__ __forceinline__ void add2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint64_t f62)
+    __device__ __forceinline__ void add2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t f62)
     {
         v += (uint64_t)a0 * b0;
         v += (uint64_t)a1 * b1;
-        v = (v & (((uint64_t)1 << 62) - 1)) + (v >> 62) * f62;
+        v = (v & (((uint64_t)1 << 62) - 1)) + (uint64_t)(uint32_t)(v >> 62) * f62;
     }
-    __device__ __forceinline__ void add1(uint32_t a, uint32_t b, uint64_t f62)
+    __device__ __forceinline__ void add1(uint32_t a, uint32_t b, uint32_t f62)
     {
         v += (uint64_t)a * b;
-        v = (v & (((uint64_t)1 << 62) - 1)) + (v >> 62) * f62;
+        v = (v & (((uint64_t)1 << 62) - 1)) + (uint64_t)(uint32_t)(v >> 62) * f62;
     }
     __device__ __forceinline__ uint32_t get(uint32_t p, uint64_t mu, uint32_t) const { return red64(v, p, mu); }
 };
@@ -118,20 +118,30 @@ struct AccSmall {
 // Any p < 2^32: a 64-bit word and a count of its wraps.
 struct AccWide {
     Acc a;
-    __device__ __forceinline__ void add2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint64_t)
+    __device__ __forceinline__ void add2(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t)
     {
         a.add(a0, b0);
         a.add(a1, b1);
     }
-    __device__ __forceinline__ void add1(uint32_t x, uint32_t y, uint64_t) { a.add(x, y); }
+    __device__ __forceinline__ void add1(uint32_t x, uint32_t y, uint32_t) { a.add(x, y); }
     __device__ __forceinline__ uint32_t get(uint32_t p, uint64_t mu, uint32_t r64) const { return a.get(p, mu, r64); }
 };
 
 // sum_j x[j] y[j*ystride], len terms
 template <typename A>
-__device__ __forceinline__ void dot(A &acc, const uint32_t *x, const uint32_t *y, int ystride, int len, uint64_t f62)
+__device__ __forceinline__ void dot(A &acc, const uint32_t *x, const uint32_t *y, int ystride, int len, uint32_t f62)
 {
     int j = 0;
+    for (; j + 7 < len; j += 8) {  // eight terms' loads issued together
+        uint32_t xv[8], yv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            xv[u] = x[j + u];
+            yv[u] = y[(j + u) * ystride];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) acc.add2(xv[u], yv[u], xv[u + 1], yv[u + 1], f62);
+    }
     for (; j + 1 < len; j += 2) acc.add2(x[j], y[j * ystride], x[j + 1], y[(j + 1) * ystride], f62);
     if (j < len) acc.add1(x[j], y[j * ystride], f62);
 }
@@ -182,7 +192,7 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                                                              DevView out, uint64_t *__restrict__ shift_io,
                                                              uint64_t *__restrict__ deg_out, int order, int m, int n,
                                                              int stage, uint32_t p, uint64_t mu, uint32_t r64,
-                                                             uint64_t f62, uint32_t pinv, uint32_t r2m, uint32_t one)
+                                                             uint32_t f62, uint32_t pinv, uint32_t r2m, uint32_t one)
 {
     // one = 1 in the elimination's representation (2^32 mod p with MONT)
     auto mul = [&](uint32_t x, uint32_t y) -> uint32_t {
@@ -612,7 +622,7 @@ lbx_status run_m_basis(PmRun &R, const DevView &serie, const DevView &sigma, siz
     LBX_TRY_HIP(hipGetLastError());
     const uint64_t mu = ~(uint64_t)0 / R.p;
     const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % R.p), r64 = (uint32_t)((uint64_t)r32 * r32 % R.p);
-    const uint64_t f62 = (((uint64_t)1 << 62) % R.p);
+    const uint32_t f62 = (uint32_t)(((uint64_t)1 << 62) % R.p);
     // staged words: enough for every coefficient of a step, at most kMbStageWords
     const size_t need = std::max((order + 1) * (m * m + m * n), m * m);
     const int stage = (int)std::min(need, (size_t)kMbStageWords);
