@@ -289,6 +289,80 @@ __global__ void __launch_bounds__(128) mp_crt_kernel(const uint32_t *__restrict_
     }
 }
 
+// The same recombination with the prime count and the limb count bounded at
+// compile time (RM, NLM): every array index is then a constant, so v and the
+// accumulator stay in registers (the general kernel above indexes them at run
+// time and keeps them in scratch memory).
+template <int RM, int NLM>
+__global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__restrict__ res, size_t entries,
+                                                           size_t size, const uint32_t *__restrict__ K, int r, int nl,
+                                                           DevView out)
+{
+    const size_t total = entries * size;
+    const MpLayout lay{r, nl};
+    const uint32_t *P = K + lay.P();
+    for (size_t idx = (size_t)blockIdx.x * 128 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 128) {
+        uint32_t v[RM];
+        uint32_t acc[NLM + 2];
+#pragma unroll
+        for (int l = 0; l < NLM + 2; ++l) acc[l] = 0;
+#pragma unroll
+        for (int i = 0; i < RM; ++i) {
+            if (i < r) {
+                const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
+                uint32_t t = res[(size_t)i * total + idx];
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    const uint32_t vj = shoup_mul(v[j], 1u, onep, p);
+                    t = shoup_mul(sub_mod(t, vj, p), K[lay.inv() + 2 * (i * r + j)], K[lay.inv() + 2 * (i * r + j) + 1],
+                                  p);
+                }
+                v[i] = t;
+                // acc += v_i * (prod_{j<i} p_j mod P); limbs past nl only take the carry
+                const uint32_t *mi = K + lay.mi() + i * nl;
+                uint64_t carry = 0;
+#pragma unroll
+                for (int l = 0; l < NLM + 2; ++l) {
+                    const uint64_t sum = (uint64_t)(l < nl ? mi[l] : 0u) * t + acc[l] + carry;
+                    acc[l] = (uint32_t)sum;
+                    carry = sum >> 32;
+                }
+            }
+        }
+        // acc < r 2^31 P <= 2^37 P: binary long division by P << sh, sh = 40..0
+        for (int sh = 40; sh >= 0; --sh) {
+            const int ws = sh >> 5, bs = sh & 31;
+            uint32_t shifted[NLM + 2];
+#pragma unroll
+            for (int l = 0; l < NLM + 2; ++l) {
+                const int src = l - ws;
+                uint32_t w = 0;
+                if (src >= 0 && src < nl) w = P[src] << bs;
+                if (bs && src - 1 >= 0 && src - 1 < nl) w |= P[src - 1] >> (32 - bs);
+                shifted[l] = w;
+            }
+            int cmp = 0;  // sign of acc - (P << sh), from the top limb down
+#pragma unroll
+            for (int l = NLM + 1; l >= 0; --l)
+                if (cmp == 0) cmp = acc[l] > shifted[l] ? 1 : (acc[l] < shifted[l] ? -1 : 0);
+            if (cmp >= 0) {
+                uint64_t borrow = 0;
+#pragma unroll
+                for (int l = 0; l < NLM + 2; ++l) {
+                    const uint64_t d = (uint64_t)acc[l] - shifted[l] - borrow;
+                    acc[l] = (uint32_t)d;
+                    borrow = (d >> 32) & 1;
+                }
+            }
+        }
+        const size_t e = idx / size, t = idx - e * size;
+        uint32_t *dst = out.base + view_index(out, e, t) * nl;
+#pragma unroll
+        for (int l = 0; l < NLM; ++l)
+            if (l < nl) dst[l] = acc[l];
+    }
+}
+
 unsigned grid_of(size_t total, unsigned block)
 {
     const size_t g = (total + block - 1) / block;
@@ -399,8 +473,16 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
         else
             LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
     }
-    hipLaunchKernelGGL(mp_crt_kernel, dim3(grid_of(m * n * sc, 128)), dim3(128), 0, s, (const uint32_t *)rc, m * n,
-                       sc, plan->d_const, r, nl, c);
+    const dim3 grid(grid_of(m * n * sc, 128));
+    if (r <= 16 && nl <= 8)
+        hipLaunchKernelGGL((mp_crt_small_kernel<16, 8>), grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc,
+                           plan->d_const, r, nl, c);
+    else if (r <= 32 && nl <= 16)
+        hipLaunchKernelGGL((mp_crt_small_kernel<32, 16>), grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc,
+                           plan->d_const, r, nl, c);
+    else
+        hipLaunchKernelGGL(mp_crt_kernel, grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc, plan->d_const, r,
+                           nl, c);
     LBX_TRY_HIP(hipGetLastError());
     return LBX_OK;
 }

@@ -120,3 +120,28 @@ def test_fft_prime_too_short_for_length(ctx):
     C = PolynomialMatrix(m, n, 1)
     PolynomialMatrixMulDomain(p, ctx).mul(C, A, B)
     assert np.array_equal(C.to_polfirst().reshape(-1), ref)
+
+
+@pytest.mark.parametrize("bits", [320, 480, 512])
+def test_crt_wide_moduli(ctx, bits):
+    """Moduli of 10, 15 and 16 limbs (22 to 35 FFT primes, so every recombination kernel
+    shape runs) against the product computed with Python integers."""
+    rng = np.random.default_rng(bits)
+    P = (1 << (bits - 1)) | int(rng.integers(1, 1 << 62)) << 1 | 1
+    nl = (P.bit_length() + 31) // 32
+    m, k, n, sa, sb = 2, 3, 2, 6, 5
+    a = random_mp(rng, P, nl, m * k * sa)
+    b = random_mp(rng, P, nl, k * n * sb)
+
+    def val(words, i):
+        return sum(int(words[i * nl + l]) << (32 * l) for l in range(nl))
+    say("wide modulus", bits)
+    got = gpu_mp(ctx, P, nl, m, k, n, a, sa, b, sb, lbx.METHOD_AUTO)
+    for i in range(m):
+        for j in range(n):
+            for t in range(sa + sb - 1):
+                ref = 0
+                for q in range(k):
+                    for u in range(max(0, t - sb + 1), min(t, sa - 1) + 1):
+                        ref += val(a, (i * k + q) * sa + u) * val(b, (q * n + j) * sb + (t - u))
+                assert val(got, ((i * n + j) * (sa + sb - 1)) + t) == ref % P, (bits, i, j, t)
