@@ -23,7 +23,7 @@ The JSON line also carries:
                 workload and extrapolated to one product.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-batch]
-                       [--skip-order-basis] [--skip-configs]
+                       [--skip-order-basis] [--skip-configs] [--only order_basis|configs|batch]
 """
 import argparse
 import json
@@ -417,6 +417,8 @@ def main():
     ap.add_argument("--skip-batch", action="store_true", help="omit the 1024-product batch measurement")
     ap.add_argument("--skip-order-basis", action="store_true", help="omit the PM_Basis measurement")
     ap.add_argument("--skip-configs", action="store_true", help="omit the other workload configurations")
+    ap.add_argument("--only", choices=["order_basis", "configs", "batch"],
+                    help="run only this leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -430,6 +432,11 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.only:
+        leg = {"order_basis": lambda: run_order_basis(args.skip_cpu), "configs": run_configs,
+               "batch": run_batch}[args.only]
+        print(json.dumps({args.only: leg()}))
         return
     ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
     batch = None if args.skip_batch else run_batch()
