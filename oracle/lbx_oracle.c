@@ -290,6 +290,28 @@ void or_stage_pointwise(uint32_t p, size_t m, size_t k, size_t n, size_t pts,
     free(brow);
 }
 
+/* The same three stages over `threads` OpenMP threads (polynomials, points). */
+void or_stage_forward_mt(uint32_t p, unsigned L, const uint32_t *pow_w, uint32_t *polys, size_t count, int threads)
+{
+    size_t n = (size_t)1 << L;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (size_t e = 0; e < count; ++e) or_fft_dif(p, L, pow_w, polys + e * n);
+}
+
+void or_stage_inverse_mt(uint32_t p, unsigned L, const uint32_t *pow_winv, uint32_t *polys, size_t count, int threads)
+{
+    size_t n = (size_t)1 << L;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (size_t e = 0; e < count; ++e) or_fft_dit(p, L, pow_winv, polys + e * n);
+}
+
+void or_stage_pointwise_mt(uint32_t p, size_t m, size_t k, size_t n, size_t pts, const uint32_t *ahat,
+                           const uint32_t *bhat, uint32_t *chat, size_t pt0, size_t npts, int threads)
+{
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (size_t t = pt0; t < pt0 + npts; ++t) or_stage_pointwise(p, m, k, n, pts, ahat, bhat, chat, t, 1);
+}
+
 int or_polymat_mul_fft_prime(uint32_t p, size_t m, size_t k, size_t n,
                              const uint32_t *a, size_t sa, const uint32_t *b, size_t sb,
                              uint32_t *c, size_t max_rowdeg)

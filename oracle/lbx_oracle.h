@@ -104,6 +104,11 @@ void or_stage_pointwise(uint32_t p, size_t m, size_t k, size_t n, size_t pts,
                         const uint32_t *ahat, const uint32_t *bhat, uint32_t *chat,
                         size_t pt0, size_t npts);
 void or_stage_inverse(uint32_t p, unsigned L, const uint32_t *pow_winv, uint32_t *polys, size_t count);
+/* The same stages over `threads` OpenMP threads (over polynomials / points). */
+void or_stage_forward_mt(uint32_t p, unsigned L, const uint32_t *pow_w, uint32_t *polys, size_t count, int threads);
+void or_stage_inverse_mt(uint32_t p, unsigned L, const uint32_t *pow_winv, uint32_t *polys, size_t count, int threads);
+void or_stage_pointwise_mt(uint32_t p, size_t m, size_t k, size_t n, size_t pts, const uint32_t *ahat,
+                           const uint32_t *bhat, uint32_t *chat, size_t pt0, size_t npts, int threads);
 
 /* PolynomialMatrixNaiveMulDomain::mul (matpoly-mult-naive.h:56-82):
  * c[t] = sum_{i+j=t} a[i] b[j] mod p, t < sa+sb-1, polfirst, c stride sa+sb-1. */

@@ -55,6 +55,9 @@ def lib():
         L.or_stage_forward.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
         L.or_stage_inverse.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz]
         L.or_stage_pointwise.argtypes = [u32, sz, sz, sz, sz, _u32p, _u32p, _u32p, sz, sz]
+        L.or_stage_forward_mt.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz, i32]
+        L.or_stage_inverse_mt.argtypes = [u32, ctypes.c_uint, _u32p, _u32p, sz, i32]
+        L.or_stage_pointwise_mt.argtypes = [u32, sz, sz, sz, sz, _u32p, _u32p, _u32p, sz, sz, i32]
         _lib = L
     return _lib
 
@@ -188,6 +191,18 @@ def mul_crt(P_limbs, m, k, n, a, sa, b, sb):
     st = lib().or_polymat_mul_crt(_p(P_limbs), nl, m, k, n, _p(a), sa, _p(b), sb, _p(c), _p(primes),
                                   ctypes.byref(npr))
     return st, c, [int(x) for x in primes[:npr.value]]
+
+
+def stage_forward_mt(p, L, pow_w, polys, count, threads):
+    lib().or_stage_forward_mt(p, L, _p(pow_w), _p(polys), count, threads)
+
+
+def stage_inverse_mt(p, L, pow_w, polys, count, threads):
+    lib().or_stage_inverse_mt(p, L, _p(pow_w), _p(polys), count, threads)
+
+
+def stage_pointwise_mt(p, m, k, n, pts, ahat, bhat, chat, pt0, npts, threads):
+    lib().or_stage_pointwise_mt(p, m, k, n, pts, _p(ahat), _p(bhat), _p(chat), pt0, npts, threads)
 
 
 def stage_forward(p, L, pow_w, polys, count):
