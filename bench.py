@@ -17,7 +17,10 @@ The JSON line also carries:
   batch_config5 1024 uneven products as one batch call vs one call per product;
   order_basis   PM_Basis of a 64 x 32 matrix power series at order 4096;
   other_configs 16x16 deg 255, 64x64 deg 4095 (both forms), 32x32 deg 1023 over a
-                192-bit modulus, and a PM_Basis-level middle product;
+                192-bit modulus, a PM_Basis-level middle product, 64x64 deg 4095
+                over a non-FFT prime below 2^29 (few-primes route, and the
+                multiprecision route for comparison), and one 128x128 x 8192
+                layout conversion;
   cpu_baseline  the oracle's restatement of the reference's single-prime
                 mul_fft timed on this host on a bounded sample of the same
                 workload and extrapolated to one product.
@@ -209,8 +212,11 @@ def staged_bytes(m, k, n, sa, sb):
 def run_configs(reps=10):
     """The other configurations of the workload table, timed with HIP events on one stream:
     16x16 deg 255 (p = 998244353), 64x64 deg 4095 (p = 2013265921, both pointwise forms),
-    32x32 deg 1023 over a 192-bit P (14 FFT primes and CRT), and the middle product of a
-    PM_Basis level (64x64 sigma of degree 2048 times a 64x32 serie of 4096 terms)."""
+    32x32 deg 1023 over a 192-bit P (14 FFT primes and CRT), the middle product of a
+    PM_Basis level (64x64 sigma of degree 2048 times a 64x32 serie of 4096 terms), 64x64
+    deg 4095 over the non-FFT prime 268435399 through the few-primes route and, for
+    comparison, through the multiprecision route (the same modulus given as two limbs), and
+    the layout conversion of one 128x128 operand of 8192 coefficients (polfirst -> matfirst)."""
     import linbox_amd as lbx
     from linbox_amd import hip, PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain
 
@@ -280,6 +286,42 @@ def run_configs(reps=10):
     mbytes = (4 * (m * m * (o1 + 1 + pts) + m * n * (order + pts)) + 4 * (m * m + 2 * m * n) * pts +
               4 * m * n * (pts + order - o1))
     out["midproduct_64x64.64x32_order4096"] = {"ms": round(ms, 4), "staged_gbs": round(mbytes / (ms * 1e-3) / 1e9, 1)}
+    # few-primes route (p < 2^29, not an FFT prime): three 31-bit FFT primes, mixed radix mod p
+    p, dim, sa = 268435399, 64, 4096
+    ha = rng.integers(0, p, dim * dim * sa, dtype=np.uint64).astype(np.uint32)
+    hb = rng.integers(0, p, dim * dim * sa, dtype=np.uint64).astype(np.uint32)
+    da, db, dc = dev(ha), dev(hb), hip.DeviceBuffer(dim * dim * (2 * sa - 1) * 4)
+    A, B = PolynomialMatrix(dim, dim, sa, data=da), PolynomialMatrix(dim, dim, sa, data=db)
+    C = PolynomialMatrix(dim, dim, 2 * sa - 1, data=dc)
+    dom = PolynomialMatrixFFTMulDomain(p, ctx, lbx.METHOD_AUTO, stream=stream.value)
+    ms = timed(lambda: dom.mul(C, A, B))
+    out["64x64_deg4095_p268435399_few_primes"] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1),
+                                                  "fft_primes": 3}
+    del da, db, dc
+    two = np.zeros((dim * dim * sa, 2), np.uint32)
+    two[:, 0] = ha
+    da = dev(two.reshape(-1))
+    two[:, 0] = hb
+    db = dev(two.reshape(-1))
+    del two
+    dc = hip.DeviceBuffer(dim * dim * (2 * sa - 1) * 2 * 4)
+    A = PolynomialMatrix(dim, dim, sa, data=da, nlimbs=2)
+    B = PolynomialMatrix(dim, dim, sa, data=db, nlimbs=2)
+    C = PolynomialMatrix(dim, dim, 2 * sa - 1, data=dc, nlimbs=2)
+    ms = timed(lambda: lbx.mul_mp(p, C, A, B, ctx=ctx, stream=stream.value))
+    out["64x64_deg4095_p268435399_multiprecision_route"] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1)}
+    del da, db, dc, ha, hb
+    # layout conversion (PolynomialMatrix::copy) of one 128x128 operand of 8192 coefficients
+    dim, sa = 128, 8192
+    da = dev(rng.integers(0, 1 << 32, dim * dim * sa, dtype=np.uint64).astype(np.uint32))
+    db = hip.DeviceBuffer(dim * dim * sa * 4)
+    A = PolynomialMatrix(dim, dim, sa, data=da)
+    B = PolynomialMatrix(dim, dim, sa, "matfirst", data=db)
+    ms = timed(lambda: B.copy(A, ctx=ctx, stream=stream.value))
+    gbs = 2 * 4 * dim * dim * sa / (ms * 1e-3) / 1e9
+    out["copy_polfirst_to_matfirst_128x128x8192"] = {"ms": round(ms, 4), "gbs": round(gbs, 1),
+                                                     "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 3)}
+    del da, db
     ctx.close()
     print("configs: " + ", ".join("%s %.3f ms" % (k, v["ms"]) for k, v in out.items()), file=sys.stderr, flush=True)
     return out

@@ -91,7 +91,10 @@ lbx_status lbx_trim(lbx_ctx *ctx);
  * written and c->size is set to deg+1 (the reference's c.resize(deg+1));
  * c->size must be >= deg+1 on entry.  Any prime p < 2^32 is accepted: FFT
  * primes with 2^ceil(log2(deg+1)) | p-1 and p < 2^31 take the single-prime
- * NTT product; every other p goes through lbx_polymat_mul_mp with one limb. */
+ * NTT product; every other p goes through lbx_polymat_mul_mp with one limb,
+ * which is the few-primes route of PolynomialMatrixThreePrimesFFTMulDomain
+ * (three-primes.inl:67-191; the reference takes it for p < 2^29 and its
+ * multiprecision domain above, with the same result). */
 lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
                                const lbx_polmat *b, size_t max_rowdeg, int method, void *stream);
 
@@ -120,6 +123,10 @@ lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t
  * nlimbs words each, in [0,P).  The integer product is computed modulo FFT
  * primes below 2^31 (RandomFFTPrime::generatePrimes order, random-fftprime.h:107-133)
  * covering 2 P^2 k min(sa,sb), recombined by CRT on the device, and reduced mod P.
+ * A one-limb P takes the few-primes route instead (three-primes.inl:67-191):
+ * primes covering (P-1)^2 k min(sa,sb), the operands used as they are when
+ * every prime exceeds P (reduced first otherwise), and the mixed-radix
+ * reconstruction mod P (:164-181); the product is the same.
  * c->size must be >= a.size+b.size-1 and is set to that value. */
 lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nlimbs, lbx_polmat *c,
                               const lbx_polmat *a, const lbx_polmat *b, int method, void *stream);
@@ -189,6 +196,18 @@ lbx_status lbx_order_basis_pm_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, c
  * arguments and results as lbx_order_basis_pm_u32. */
 lbx_status lbx_order_basis_m_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_polmat *serie,
                                  size_t order, size_t *shift, size_t *degree, void *stream);
+
+/* PolynomialMatrix<Field,T>::copy(M, beg, end, start) for any pair of
+ * layouts and views (polynomial-matrix.h:206-243 into polfirst, :508-532 into
+ * matfirst, :763-790 into matrowfirst; SubPolynomialMatrix::copy :988-992
+ * writes at the view's shift, here dst->shift): for every entry e and i in
+ * [beg, end], coefficient start+i-beg of dst = coefficient i of src.  Same
+ * dimensions (else LBX_ERR_SIZE_MISMATCH); end < src->size and
+ * start+end-beg < dst->size; the rest of dst's storage is left as it is.
+ * src and dst must not share storage (LBX_ERR_INVALID_ARG when their data
+ * pointers are equal).  Host or device pointers; device work is stream-ordered. */
+lbx_status lbx_polymat_copy_u32(lbx_ctx *ctx, lbx_polmat *dst, const lbx_polmat *src, size_t beg, size_t end,
+                                size_t start, void *stream);
 
 /* FFT_PROFILER analogue (matpoly-mult-fft.h:43-77): with enable = 1, calls
  * record device events between their stages, labelled as in mul_fft

@@ -113,6 +113,21 @@ public:
 
     view at(size_t i, size_t j) { return view(*this, i, j); }
 
+    // copy(M, beg, end, start): coefficients beg..end of M into start.. of this
+    // matrix, any pair of layouts (polynomial-matrix.h:206-243, 508-532, 763-790),
+    // through the device (lbx_polymat_copy_u32); copy(M) copies all of M.
+    template <class Mat>
+    void copy(const Mat &M, size_t beg, size_t end, size_t start = 0)
+    {
+        lbx_polmat d = desc(), s = M.desc();
+        check(lbx_polymat_copy_u32(default_ctx(), &d, &s, beg, end, start, nullptr));
+    }
+    template <class Mat>
+    void copy(const Mat &M)
+    {
+        copy(M, 0, M.size() - 1);
+    }
+
     lbx_polmat desc() const
     {
         lbx_polmat d;
@@ -165,6 +180,18 @@ public:
         return d;
     }
     void after_mul(const lbx_polmat &d) { size_ = d.size; }
+    // SubPolynomialMatrix::copy(M, beg, end) writes at the view's shift (polynomial-matrix.h:988-992)
+    template <class Mat>
+    void copy(const Mat &M, size_t beg, size_t end)
+    {
+        lbx_polmat d = desc(), s = M.desc();
+        check(lbx_polymat_copy_u32(default_ctx(), &d, &s, beg, end, 0, nullptr));
+    }
+    template <class Mat>
+    void copy(const Mat &M)
+    {
+        copy(M, 0, M.size() - 1);
+    }
 
 private:
     MatPoly *ptr_;
@@ -342,6 +369,40 @@ private:
     lbx_ctx *ctx_;
     int method_;
     void *stream_;
+};
+
+// PolynomialMatrixThreePrimesFFTMulDomain (matpoly-mult-fft-wordsize-three-primes.inl:40-328):
+// word primes below 2^29.  The constructor throws for larger primes (:57-64);
+// mul and midproduct take the single-prime route when (p-1) % pts == 0
+// (:107-111, :238-243) and the few-primes route otherwise, which is the route
+// lbx_polymat_mul_u32 / lbx_polymat_midproduct_u32 choose for such primes.
+class PolynomialMatrixThreePrimesFFTMulDomain {
+public:
+    typedef Modular Field;
+    explicit PolynomialMatrixThreePrimesFFTMulDomain(const Field &F, lbx_ctx *ctx = nullptr,
+                                                     int method = LBX_METHOD_AUTO, void *stream = nullptr)
+        : dom_(F, ctx, method, stream)
+    {
+        if (F.cardinality() >= (1ull << 29))
+            throw LinboxError(LBX_ERR_PRIME_TOO_LARGE, "too large FFT Prime (more than 29 bits)");
+    }
+    const Field &field() const { return dom_.field(); }
+
+    template <typename Matrix1, typename Matrix2, typename Matrix3>
+    void mul(Matrix1 &c, const Matrix2 &a, const Matrix3 &b, size_t max_rowdeg = 0) const
+    {
+        dom_.mul(c, a, b, max_rowdeg);
+    }
+
+    template <typename Matrix1, typename Matrix2, typename Matrix3>
+    void midproduct(Matrix1 &c, const Matrix2 &a, const Matrix3 &b, bool smallLeft = true, size_t n0 = 0,
+                    size_t n1 = 0) const
+    {
+        dom_.midproduct(c, a, b, smallLeft, n0, n1);
+    }
+
+private:
+    PolynomialMatrixFFTMulDomain dom_;
 };
 
 // FFT<Modular<uint32_t,uint64_t>> (fft.h:57-144).

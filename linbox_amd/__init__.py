@@ -27,7 +27,7 @@ EXPORTED = [
     "lbx_init", "lbx_destroy", "lbx_strerror", "lbx_trim", "lbx_polymat_mul_u32", "lbx_polymat_mul_fft_u32",
     "lbx_ntt_forward_u32", "lbx_ntt_inverse_u32", "lbx_polymat_mul_mp", "lbx_polymat_mul_batch",
     "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_order_basis_pm_u32",
-    "lbx_order_basis_m_u32", "lbx_profile_enable", "lbx_profile_get",
+    "lbx_order_basis_m_u32", "lbx_polymat_copy_u32", "lbx_profile_enable", "lbx_profile_get",
 ]
 
 
@@ -85,6 +85,7 @@ def lib():
         SZP = ctypes.POINTER(sz)
         L.lbx_order_basis_pm_u32.argtypes = [vp, u32, PM, PM, sz, SZP, SZP, i32, vp]
         L.lbx_order_basis_m_u32.argtypes = [vp, u32, PM, PM, sz, SZP, SZP, vp]
+        L.lbx_polymat_copy_u32.argtypes = [vp, PM, PM, sz, sz, sz, vp]
         L.lbx_profile_enable.argtypes = [vp, i32]
         L.lbx_profile_get.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]
         L.lbx_profile_get.restype = i32
@@ -140,10 +141,12 @@ def default_context():
 
 
 from .polymat import (PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain,  # noqa: E402
-                      PolynomialMatrixFFTPrimeMulDomain, PolynomialMatrixNaiveMulDomain, FFT, ntt_forward,
-                      ntt_inverse, mul_mp, mul_batch, OrderBasis)
+                      PolynomialMatrixFFTPrimeMulDomain, PolynomialMatrixNaiveMulDomain,
+                      PolynomialMatrixThreePrimesFFTMulDomain, FFT, ntt_forward, ntt_inverse, mul_mp, mul_batch,
+                      OrderBasis)
 
 __all__ = ["Context", "LinboxError", "PolynomialMatrix", "PolynomialMatrixMulDomain", "PolynomialMatrixFFTMulDomain",
-           "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis",
+           "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain",
+           "PolynomialMatrixThreePrimesFFTMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis",
            "POLFIRST", "MATFIRST", "MATROWFIRST", "METHOD_AUTO", "METHOD_PLAIN_U64", "METHOD_INT8_MFMA", "lib",
            "build", "default_context", "EXPORTED"]

@@ -1,7 +1,11 @@
 // Multiprecision modulus route: PolynomialMatrixFFTMulDomain<Modular<integer>>::mul
 // (matpoly-mult-fft-multiprecision.inl:611-630) over ZRing<integer>::mul_crtla
-// (:141-277), and the same route for word-size primes that are not FFT primes
-// (the reference's few-primes domain, three-primes.inl:67-191).
+// (:141-277), and the few-primes route of PolynomialMatrixThreePrimesFFTMulDomain
+// (three-primes.inl:67-191) for one-word moduli that are not FFT primes for the
+// length.  The reference takes the few-primes domain for p < 2^29 and the
+// multiprecision one above (wordsize.inl:62-83); the product is the same either
+// way, and here every one-word modulus takes the few-primes route, which needs
+// no multiword arithmetic (five times faster at 64x64, degree 4095).
 //
 // The integer product of the [0,P) representatives has coefficients below
 // bound = 2 P^2 k min(sa,sb) (:616).  It is computed modulo FFT primes
@@ -32,7 +36,8 @@ struct MpPlan {
     std::vector<uint32_t> primes;
     unsigned lpts = 0;
     int nlimbs = 0;
-    uint32_t *d_const = nullptr;  // device constant block, layout described at MpLayout
+    bool reduce = false;          // few-primes plans: some prime is below the modulus
+    uint32_t *d_const = nullptr;  // device constant block, layout described at MpLayout / FpLayout
 };
 
 namespace {
@@ -363,6 +368,81 @@ __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__res
     }
 }
 
+// ---- few-primes route (three-primes.inl:67-191) ----------------------------
+// Constants of a few-primes plan, r primes, modulus p < 2^32.
+struct FpLayout {
+    int r;
+    __host__ __device__ int primes() const { return 0; }
+    __host__ __device__ int onep() const { return r; }            // Shoup constant of 1 mod p_i
+    __host__ __device__ int inv() const { return 2 * r; }         // p_j^-1 mod p_i and its Shoup constant
+    __host__ __device__ int beta() const { return 2 * r + 2 * r * r; }  // prod_{j<i} p_j mod p, Shoup constant
+    __host__ __device__ int words() const { return beta() + 2 * r; }
+};
+
+// Mixed-radix recombination (three-primes.inl:164-181): the Garner digits v_i
+// of each coefficient from its residues res[i][idx] (c_i <- (c_i - c_j) p_j^-1
+// mod p_i for j < i), then c = sum_i v_i beta_i mod p with beta_i = prod_{j<i}
+// p_j mod p, written to the output view.  Every value is a 32-bit word: v_i <
+// p_i < 2^31 and Shoup products take any 32-bit multiplicand.  WIDE (p >= 2^31):
+// the Shoup remainder x*w - q*p < 2p and the sum are held in 64 bits.
+template <int RM, bool WIDE>
+__global__ void __launch_bounds__(256) few_primes_crt_kernel(const uint32_t *__restrict__ res, size_t entries,
+                                                             size_t size, const uint32_t *__restrict__ K, int r,
+                                                             uint32_t p, DevView out)
+{
+    const size_t total = entries * size;
+    const FpLayout lay{r};
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        uint32_t v[RM];
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < RM; ++i) {
+            if (i < r) {
+                const uint32_t pi = K[lay.primes() + i], onep = K[lay.onep() + i];
+                uint32_t t = res[(size_t)i * total + idx];
+#pragma unroll
+                for (int j = 0; j < i; ++j) {
+                    const uint32_t vj = shoup_mul(v[j], 1u, onep, pi);
+                    t = shoup_mul(sub_mod(t, vj, pi), K[lay.inv() + 2 * (i * r + j)],
+                                  K[lay.inv() + 2 * (i * r + j) + 1], pi);
+                }
+                v[i] = t;
+                const uint32_t w = K[lay.beta() + 2 * i], wp = K[lay.beta() + 2 * i + 1];
+                if constexpr (!WIDE) {
+                    acc = add_mod(acc, shoup_mul(t, w, wp, p), p);
+                } else {
+                    uint64_t y = (uint64_t)t * w - (uint64_t)mulhi32(t, wp) * p;
+                    if (y >= p) y -= p;
+                    y += acc;
+                    if (y >= p) y -= p;
+                    acc = (uint32_t)y;
+                }
+            }
+        }
+        const size_t e = idx / size, t = idx - e * size;
+        out.base[view_index(out, e, t)] = acc;
+    }
+}
+
+// Residues of one-word coefficients modulo every prime of a few-primes plan,
+// out[i][e*size + t] (the finit of three-primes.inl:143-146), for plans with a
+// prime below the modulus.
+__global__ void __launch_bounds__(256) fp_residue_kernel(DevView in, size_t entries, size_t size,
+                                                         const uint32_t *__restrict__ K, int r,
+                                                         uint32_t *__restrict__ out)
+{
+    const size_t total = entries * size;
+    const FpLayout lay{r};
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        const size_t e = idx / size, t = idx - e * size;
+        const uint32_t x = in.base[view_index(in, e, t)];
+        for (int i = 0; i < r; ++i)
+            out[(size_t)i * total + idx] = shoup_mul(x, 1u, K[lay.onep() + i], K[lay.primes() + i]);
+    }
+}
+
+constexpr int kFewMaxPrimes = 8;
+
 unsigned grid_of(size_t total, unsigned block)
 {
     const size_t g = (total + block - 1) / block;
@@ -424,6 +504,63 @@ lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t m
     return LBX_OK;
 }
 
+// Plan of the few-primes route: bound = (p-1)^2 k min(sa,sb) (three-primes.inl:75-76,
+// 180-181 of mul, :225-226 of midproduct), FFT primes below 2^31 with
+// 2^lpts | p_i - 1 in the order of generatePrimes until their product exceeds
+// the bound (:116-121).  r = 0 (the caller then takes the multiprecision
+// route) when more than kFewMaxPrimes would be needed; `reduce` when some
+// p_i <= p, whose operands need the reduction of :143-146.
+lbx_status get_few_plan(lbx_ctx *ctx, uint32_t p, size_t k, size_t minsz, unsigned lpts, MpPlan **out)
+{
+    std::string key = "fp|" + std::to_string(p) + "|" + std::to_string(k) + "|" + std::to_string(minsz) + "|" +
+                      std::to_string(lpts);
+    auto it = ctx->mp_plans.find(key);
+    if (it != ctx->mp_plans.end()) {
+        *out = it->second;
+        return LBX_OK;
+    }
+    Limbs bound{p - 1};
+    limbs_mul_small(bound, p - 1);
+    limbs_mul_small(bound, (uint32_t)k);
+    limbs_mul_small(bound, (uint32_t)minsz);
+    std::vector<uint32_t> primes;
+    if (!generate_primes((uint64_t)1 << 31, bound, lpts, primes)) return LBX_ERR_NOT_ENOUGH_PRIMES;
+    MpPlan *plan = new MpPlan();
+    plan->lpts = lpts;
+    plan->nlimbs = 1;
+    for (uint32_t q : primes) plan->reduce = plan->reduce || q <= p;
+    if ((int)primes.size() <= kFewMaxPrimes) {
+        const int r = (int)primes.size();
+        const FpLayout lay{r};
+        std::vector<uint32_t> K((size_t)lay.words(), 0);
+        uint32_t beta = 1 % p;
+        for (int i = 0; i < r; ++i) {
+            const uint32_t pi = primes[i];
+            K[lay.primes() + i] = pi;
+            K[lay.onep() + i] = shoup_pre(1, pi);
+            for (int j = 0; j < i; ++j) {
+                const uint32_t inv = invmod_h(primes[j] % pi, pi);
+                K[lay.inv() + 2 * (i * r + j)] = inv;
+                K[lay.inv() + 2 * (i * r + j) + 1] = shoup_pre(inv, pi);
+            }
+            K[lay.beta() + 2 * i] = beta;
+            K[lay.beta() + 2 * i + 1] = shoup_pre(beta, p);
+            beta = (uint32_t)((uint64_t)beta * (pi % p) % p);  // beta *= p_i (:179-180)
+        }
+        hipError_t e = hipMalloc(&plan->d_const, K.size() * 4);
+        if (e == hipSuccess) e = hipMemcpy(plan->d_const, K.data(), K.size() * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (plan->d_const) (void)hipFree(plan->d_const);
+            delete plan;
+            return from_hip(e);
+        }
+        plan->primes = primes;
+    }
+    ctx->mp_plans[key] = plan;
+    *out = plan;
+    return LBX_OK;
+}
+
 }  // namespace
 
 void mp_release(lbx_ctx *ctx)
@@ -449,8 +586,55 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
         lpts = mid_L;
         pts = (size_t)1 << lpts;
     }
+    const size_t minsz = sa < sb ? sa : sb;
+    if (nl == 1 && Pw[0] >= 2) {
+        // few-primes route (wordsize.inl:66-68): with every p_i > p the operands
+        // are already residues mod p_i (three-primes.inl:138-142), so the
+        // per-prime products read them where they are; else one pass writes
+        // their residues first (:143-146)
+        MpPlan *fp;
+        LBX_TRY(get_few_plan(ctx, Pw[0], k, minsz, lpts, &fp));
+        const int r = (int)fp->primes.size();
+        if (r > 0) {
+            void *ra = nullptr, *rb = nullptr, *rc;
+            if (fp->reduce) {
+                LBX_TRY(ensure(ctx, "mp_ra", (size_t)r * m * k * sa * 4, &ra));
+                LBX_TRY(ensure(ctx, "mp_rb", (size_t)r * k * n * sb * 4, &rb));
+                hipLaunchKernelGGL(fp_residue_kernel, dim3(grid_of(m * k * sa, 256)), dim3(256), 0, s, a, m * k, sa,
+                                   fp->d_const, r, (uint32_t *)ra);
+                LBX_TRY_HIP(hipGetLastError());
+                hipLaunchKernelGGL(fp_residue_kernel, dim3(grid_of(k * n * sb, 256)), dim3(256), 0, s, b, k * n, sb,
+                                   fp->d_const, r, (uint32_t *)rb);
+                LBX_TRY_HIP(hipGetLastError());
+            }
+            LBX_TRY(ensure(ctx, "mp_rc", (size_t)r * m * n * sc * 4, &rc));
+            for (int i = 0; i < r; ++i) {
+                DevView va = a, vb = b;
+                if (fp->reduce) {
+                    va = DevView{(uint32_t *)ra + (size_t)i * m * k * sa, LBX_POLFIRST, m, k, sa, 0, sa};
+                    vb = DevView{(uint32_t *)rb + (size_t)i * k * n * sb, LBX_POLFIRST, k, n, sb, 0, sb};
+                }
+                DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
+                if (mid)
+                    LBX_TRY(fft_prime_product(ctx, fp->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s, mid));
+                else if (sc == 1)
+                    LBX_TRY(constant_product(ctx, fp->primes[i], va, vb, vc, m, k, n, s));
+                else
+                    LBX_TRY(fft_prime_product(ctx, fp->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
+            }
+            const dim3 grid(grid_of(m * n * sc, 256));
+            if (Pw[0] >= (1u << 31))
+                hipLaunchKernelGGL((few_primes_crt_kernel<kFewMaxPrimes, true>), grid, dim3(256), 0, s,
+                                   (const uint32_t *)rc, m * n, sc, fp->d_const, r, Pw[0], c);
+            else
+                hipLaunchKernelGGL((few_primes_crt_kernel<kFewMaxPrimes, false>), grid, dim3(256), 0, s,
+                                   (const uint32_t *)rc, m * n, sc, fp->d_const, r, Pw[0], c);
+            LBX_TRY_HIP(hipGetLastError());
+            return LBX_OK;
+        }
+    }
     MpPlan *plan;
-    LBX_TRY(get_plan(ctx, Pw, nl, k, sa < sb ? sa : sb, lpts, &plan));
+    LBX_TRY(get_plan(ctx, Pw, nl, k, minsz, lpts, &plan));
     const int r = (int)plan->primes.size();
     void *ra, *rb, *rc;
     LBX_TRY(ensure(ctx, "mp_ra", (size_t)r * m * k * sa * 4, &ra));

@@ -8,6 +8,8 @@
 * PolynomialMatrixFFTMulDomain / PolynomialMatrixFFTPrimeMulDomain
                               -- the FFT domains (matpoly-mult-fft-wordsize.inl:57-88,
                                  matpoly-mult-fft-wordsize-fast.inl:62-190).
+* PolynomialMatrixThreePrimesFFTMulDomain
+                              -- the few-primes domain for p < 2^29 (three-primes.inl:40-328).
 * FFT                         -- FFT<Field> with FFT_direct / FFT_inverse (fft.h:57-144).
 
 Storage is a numpy uint32 array (host) or any device tensor exposing
@@ -137,6 +139,17 @@ class PolynomialMatrix:
 
     def polmat(self):
         return Polmat(_ptr(self.data), self.layout, self.rows, self.cols, self._size, self.storage_size, self.shift)
+
+    def copy(self, M, beg=0, end=None, start=0, ctx=None, stream=None):
+        """copy(M, beg, end, start): coefficients beg..end of every entry of M into
+        start.. of this matrix or view, any layouts (polynomial-matrix.h:206-243,508-532,
+        763-790; a view writes at its shift, :988-992); copy(M) copies all of M.  Runs on
+        the device (lbx_polymat_copy_u32); the rest of this matrix is left as it is."""
+        end = M.size() - 1 if end is None else end
+        ctx = ctx or default_context()
+        pd, ps = self.polmat(), M.polmat()
+        check(lib().lbx_polymat_copy_u32(ctx.handle, ctypes.byref(pd), ctypes.byref(ps), beg, end, start, stream))
+        return self
 
 
 class PolynomialMatrixFFTMulDomain:
@@ -269,6 +282,19 @@ class PolynomialMatrixFFTPrimeMulDomain:
         check(lib().lbx_polymat_mul_fft_u32(self.ctx.handle, self.p, lpts, a.rowdim(), a.coldim(), b.coldim(),
                                             _ptr(a.data), _ptr(b.data), _ptr(c.data), self.method, self.stream))
         return c
+
+
+class PolynomialMatrixThreePrimesFFTMulDomain(PolynomialMatrixFFTMulDomain):
+    """PolynomialMatrixThreePrimesFFTMulDomain (matpoly-mult-fft-wordsize-three-primes.inl:40-328):
+    word primes below 2^29 (the constructor raises for larger ones, :57-64).  mul and
+    midproduct take the single-prime route when (p-1) % pts == 0 (:107-111, :238-243),
+    else the few-primes route: products modulo 31-bit FFT primes whose product exceeds
+    (p-1)^2 k min(sa,sb), and the mixed-radix reconstruction mod p (:164-181)."""
+
+    def __init__(self, p, ctx=None, method=METHOD_AUTO, stream=None):
+        if int(p).bit_length() > 29:
+            raise LinboxError(8, "too large FFT Prime (more than 29 bits)")
+        super().__init__(p, ctx, method, stream)
 
 
 def _ntt(fn, p, lpts, data, count, root, ctx, stream):

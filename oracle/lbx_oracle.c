@@ -678,6 +678,57 @@ int or_polymat_mul_crt(const uint32_t *P, int nlimbs, size_t m, size_t k, size_t
     return OR_OK;
 }
 
+/* PolynomialMatrixThreePrimesFFTMulDomain (three-primes.inl:57-191). */
+int or_polymat_mul_three_primes(uint32_t p, size_t m, size_t k, size_t n,
+                                const uint32_t *a, size_t sa, const uint32_t *b, size_t sb, uint32_t *c,
+                                uint32_t *primes_used, int *nprimes_out)
+{
+    if (p >= (1u << 29)) return OR_ERR_PRIME_TOO_LARGE; /* :59-63 */
+    if (p < 2 || !sa || !sb) return OR_ERR_INVALID_ARG;
+    if (nprimes_out) *nprimes_out = 0;
+    size_t s = sa + sb - 1;
+    if (s == 1) return or_polymat_mul_naive(p, m, k, n, a, sa, b, sb, c);
+    unsigned lpts = 0;
+    size_t pts = 1;
+    while (pts < s) { pts <<= 1; ++lpts; } /* :71-72, pts > deg */
+    if ((p - 1) % pts == 0) return or_polymat_mul_fft_prime(p, m, k, n, a, sa, b, sb, c, 0); /* :107-111 */
+    uint32_t bound[4] = {p - 1, 0, 0, 0}; /* (p-1)^2 k min(sa,sb) < 2^122 */
+    mp_mul_small(bound, 4, p - 1);
+    mp_mul_small(bound, 4, (uint32_t)k);
+    mp_mul_small(bound, 4, (uint32_t)(sa < sb ? sa : sb));
+    uint32_t primes[64];
+    int r = or_generate_fft_primes((uint64_t)1 << 31, bound, 4, lpts, primes, 64);
+    if (r < 0) return OR_ERR_NOT_ENOUGH_PRIMES; /* :118-123 */
+    if (primes_used) memcpy(primes_used, primes, sizeof(uint32_t) * r);
+    if (nprimes_out) *nprimes_out = r;
+    size_t na = m * k * sa, nbb = k * n * sb, nc = m * n * s;
+    uint32_t *ar = malloc(sizeof(uint32_t) * na), *br = malloc(sizeof(uint32_t) * nbb);
+    uint32_t *cr = malloc(sizeof(uint32_t) * nc * r);
+    if (!ar || !br || !cr) { free(ar); free(br); free(cr); return OR_ERR_INVALID_ARG; }
+    for (int l = 0; l < r; ++l) {
+        for (size_t i = 0; i < na; ++i) ar[i] = a[i] % primes[l]; /* copy when p_l > p, finit otherwise */
+        for (size_t i = 0; i < nbb; ++i) br[i] = b[i] % primes[l];
+        int st = or_polymat_mul_fft_prime(primes[l], m, k, n, ar, sa, br, sb, cr + (size_t)l * nc, 0);
+        if (st) { free(ar); free(br); free(cr); return st; }
+    }
+    /* mixed-radix reconstruction (:164-181), on whole arrays as the reference does */
+    for (size_t e = 0; e < nc; ++e) c[e] = cr[e] % p; /* freduce */
+    uint64_t beta = 1 % p;
+    for (int i = 1; i < r; ++i) {
+        uint64_t pi = primes[i];
+        uint32_t *ci = cr + (size_t)i * nc;
+        for (int j = 0; j < i; ++j) {
+            uint64_t alpha = or_invmod(primes[j] % primes[i], primes[i]);
+            const uint32_t *cj = cr + (size_t)j * nc;
+            for (size_t e = 0; e < nc; ++e) ci[e] = (uint32_t)((ci[e] + pi - cj[e] % pi) % pi * alpha % pi);
+        }
+        beta = beta * (primes[i - 1] % p) % p;
+        for (size_t e = 0; e < nc; ++e) c[e] = (uint32_t)((c[e] + beta * (ci[e] % p)) % p);
+    }
+    free(ar); free(br); free(cr);
+    return OR_OK;
+}
+
 /* ------------------------------------------------------------------------
  * Order bases: OrderBasis::M_Basis (order-basis.h:346-600) and
  * OrderBasis::PM_Basis (order-basis.h:219-326).

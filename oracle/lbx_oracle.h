@@ -17,6 +17,8 @@
  *   or_polymat_mul_naive       matpoly-mult-naive.h:56-82  pinned: Python big-int golden
  *   or_generate_fft_primes     random-fftprime.h:107-166   pinned: sympy-free MR + golden list
  *   or_polymat_mul_crt         matpoly-mult-fft-multiprecision.inl:141-277,611-630 pinned: golden
+ *   or_polymat_mul_three_primes matpoly-mult-fft-wordsize-three-primes.inl:67-191
+ *                              pinned: naive product (unique) + prime count from the bound
  *   or_polymat_midproduct_fft_prime  matpoly-mult-fft-wordsize-fast.inl:209-316
  *                              pinned: cyclic-correlation brute force + golden
  *   or_polymat_midproduct_naive matpoly-mult-naive.h:85-125 pinned: coefficients of or_polymat_mul_naive
@@ -52,6 +54,7 @@ extern "C" {
 #define OR_ERR_SIZE_MISMATCH 5
 #define OR_ERR_NOT_ENOUGH_PRIMES 6
 #define OR_ERR_INVALID_ARG 7
+#define OR_ERR_PRIME_TOO_LARGE 8
 
 /* ---- deterministic input generator (shared with the bench input spec) ---- */
 uint64_t or_splitmix64(uint64_t *state);
@@ -133,6 +136,21 @@ int or_generate_fft_primes(uint64_t pbound, const uint32_t *prodbound, int nlimb
 int or_polymat_mul_crt(const uint32_t *P, int nlimbs, size_t m, size_t k, size_t n,
                        const uint32_t *a, size_t sa, const uint32_t *b, size_t sb, uint32_t *c,
                        uint32_t *primes_used, int *nprimes);
+
+/* PolynomialMatrixThreePrimesFFTMulDomain::mul + mul_fft (three-primes.inl:57-191)
+ * for p < 2^29 (:59-63, else OR_ERR_PRIME_TOO_LARGE).  pts = the least power of
+ * two > sa+sb-2.  If (p-1) % pts == 0: the single-prime product (:107-111).
+ * Else bound = (p-1)^2 k min(sa,sb) (:75-76), FFT primes from
+ * or_generate_fft_primes with pbound 2^31 and kmin = lpts (the reference's
+ * maxFFTPrimeValue bounds primes for products in doubles; 32-bit integer
+ * arithmetic allows 31 bits, and the product is unique), one product mod each
+ * p_i of the operands reduced mod p_i (:138-146), and the mixed-radix
+ * reconstruction of :164-181: c = c_0 mod p; for i >= 1, c_i <- (c_i - c_j)
+ * p_j^-1 mod p_i for j < i, beta *= p_{i-1} mod p, c += beta c_i mod p.
+ * c polfirst, stride sa+sb-1; primes_used (optional, >= 64 entries). */
+int or_polymat_mul_three_primes(uint32_t p, size_t m, size_t k, size_t n,
+                                const uint32_t *a, size_t sa, const uint32_t *b, size_t sb, uint32_t *c,
+                                uint32_t *primes_used, int *nprimes);
 
 /* PolynomialMatrixFFTPrimeMulDomain::midproduct (matpoly-mult-fft-wordsize-fast.inl:209-245)
  * + midproduct_fft (:249-316).  hdeg = n0 ? n0 : sc, deg = n1 ? n1 : 2*hdeg-1,

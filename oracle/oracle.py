@@ -47,6 +47,8 @@ def lib():
         L.or_generate_fft_primes.argtypes = [u64, _u32p, i32, ctypes.c_uint, _u32p, i32]
         L.or_polymat_mul_crt.argtypes = [_u32p, i32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, _u32p,
                                          ctypes.POINTER(i32)]
+        L.or_polymat_mul_three_primes.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, _u32p,
+                                                  ctypes.POINTER(ctypes.c_int)]
         L.or_polymat_midproduct_fft_prime.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz, i32, sz, sz]
         L.or_polymat_midproduct_naive.argtypes = [u32, sz, sz, sz, _u32p, sz, _u32p, sz, _u32p, sz, i32, sz, sz]
         u64p = ctypes.POINTER(u64)
@@ -190,6 +192,18 @@ def mul_crt(P_limbs, m, k, n, a, sa, b, sb):
     npr = ctypes.c_int(0)
     st = lib().or_polymat_mul_crt(_p(P_limbs), nl, m, k, n, _p(a), sa, _p(b), sb, _p(c), _p(primes),
                                   ctypes.byref(npr))
+    return st, c, [int(x) for x in primes[:npr.value]]
+
+
+def mul_three_primes(p, m, k, n, a, sa, b, sb):
+    """PolynomialMatrixThreePrimesFFTMulDomain::mul (three-primes.inl:57-191); returns
+    (status, c polfirst of stride sa+sb-1, primes used)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    c = np.zeros(m * n * (sa + sb - 1), dtype=np.uint32)
+    primes = np.zeros(64, dtype=np.uint32)
+    npr = ctypes.c_int(0)
+    st = lib().or_polymat_mul_three_primes(p, m, k, n, _p(a), sa, _p(b), sb, _p(c), _p(primes), ctypes.byref(npr))
     return st, c, [int(x) for x in primes[:npr.value]]
 
 

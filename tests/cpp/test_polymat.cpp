@@ -222,6 +222,59 @@ static void test_order_basis(uint32_t p, std::mt19937_64 &g)
     }
 }
 
+// PolynomialMatrixThreePrimesFFTMulDomain: primes above 29 bits are refused
+// (three-primes.inl:57-64); a non-FFT prime takes the few-primes route.
+static void test_three_primes(std::mt19937_64 &g)
+{
+    bool threw = false;
+    try {
+        Modular big(1000000007u);
+        PolynomialMatrixThreePrimesFFTMulDomain D(big);
+    } catch (const LinboxError &e) {
+        threw = e.status == LBX_ERR_PRIME_TOO_LARGE;
+    }
+    EXPECT(threw, "three-primes domain refuses a 30-bit prime");
+    const uint32_t p = 1048573u;  // p-1 = 2^2 * 262143
+    Modular F(p);
+    PolynomialMatrix<matfirst> A(F, 6, 5, 300);
+    PolynomialMatrix<polfirst> B(F, 5, 4, 200);
+    PolynomialMatrix<matrowfirst> C(F, 6, 4, 1);
+    fill(A, p, g);
+    fill(B, p, g);
+    PolynomialMatrixThreePrimesFFTMulDomain D(F);
+    D.mul(C, A, B);
+    EXPECT(C.size() == 499, "c resized to deg+1");
+    EXPECT(check_at(C, A, B, 1, p), "few primes: check_mul at x=1");
+    EXPECT(check_at(C, A, B, g() % p, p), "few primes: check at a random point");
+}
+
+// PolynomialMatrix::copy across layouts and through a view (polynomial-matrix.h:206-243,
+// 508-532, 763-790, 988-992): coefficient by coefficient against the source.
+static void test_copy(std::mt19937_64 &g)
+{
+    const uint32_t p = 998244353u;
+    Modular F(p);
+    PolynomialMatrix<polfirst> A(F, 5, 7, 90);
+    fill(A, p, g);
+    PolynomialMatrix<matfirst> B(F, 5, 7, 90);
+    B.copy(A);
+    PolynomialMatrix<matrowfirst> C(F, 5, 7, 60);
+    C.copy(B, 20, 69, 5);  // coefficients 20..69 of B into 5..54 of C
+    bool ok = true;
+    for (size_t e = 0; e < 35; ++e) {
+        for (size_t t = 0; t < 90; ++t) ok &= B.get(e, t) == A.get(e, t);
+        for (size_t t = 0; t < 60; ++t) ok &= C.get(e, t) == ((t >= 5 && t < 55) ? A.get(e, t + 15) : 0u);
+    }
+    EXPECT(ok, "copy polfirst -> matfirst -> matrowfirst window");
+    PolynomialMatrix<polfirst> D(F, 5, 7, 40);
+    auto V = D.at(10, 29);
+    V.copy(C, 5, 24);  // into coefficients 10..29 of D
+    ok = true;
+    for (size_t e = 0; e < 35; ++e)
+        for (size_t t = 0; t < 40; ++t) ok &= D.get(e, t) == ((t >= 10 && t < 30) ? A.get(e, t + 10) : 0u);
+    EXPECT(ok, "copy into a view");
+}
+
 int main(int argc, char **argv)
 {
     const uint64_t seed = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 20191;
@@ -238,6 +291,8 @@ int main(int argc, char **argv)
         test_order_basis(p, g);
     }
     test_mul<polfirst, polfirst, polfirst>(1000000007u, 8, 64, LBX_METHOD_AUTO, g);  // CRT route
+    test_three_primes(g);
+    test_copy(g);
     if (failures) {
         std::printf("%d failures\n", failures);
         return 1;

@@ -54,6 +54,7 @@ def test_null_arguments_rejected_without_device():
     assert L.lbx_polymat_mul_u32(None, 7, None, None, None, 0, 0, None) == 7
     assert L.lbx_order_basis_pm_u32(None, 7, None, None, 4, None, None, 0, None) == 7
     assert L.lbx_order_basis_m_u32(None, 7, None, None, 4, None, None, None) == 7
+    assert L.lbx_polymat_copy_u32(None, None, None, 0, 0, 0, None) == 7
 
 
 def test_layout_index_formulas():
