@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "lbx_ctx.h"
+#include "mp_div.h"
 
 namespace lbx {
 
@@ -190,9 +191,12 @@ bool generate_primes(uint64_t pbound, const Limbs &prodbound, unsigned kmin, std
 //   [2r..3r)       2^32 mod p_i
 //   [3r..4r)       Shoup constant of 2^32 mod p_i
 //   [4r..4r+2r^2)  Garner inverses (p_j mod p_i)^-1 and their Shoup constants, i*r+j
-//   then r*nlimbs  prod_{j<i} p_j mod P, then nlimbs words of P.
+//   then r*nlimbs  prod_{j<i} p_j mod P, then nlimbs words of P;
+//   for the register kernels (nq = their word count NLM, else 0): off and s,
+//   Q = P 2^(32 off + s) in nq words, and r*nq words of
+//   (prod_{j<i} p_j mod P) 2^(32 off + s) (mp_div.h).
 struct MpLayout {
-    int r, nl;
+    int r, nl, nq = 0;
     __host__ __device__ int primes() const { return 0; }
     __host__ __device__ int onep() const { return r; }
     __host__ __device__ int r32() const { return 2 * r; }
@@ -200,7 +204,10 @@ struct MpLayout {
     __host__ __device__ int inv() const { return 4 * r; }
     __host__ __device__ int mi() const { return 4 * r + 2 * r * r; }
     __host__ __device__ int P() const { return mi() + r * nl; }
-    __host__ __device__ int words() const { return P() + nl; }
+    __host__ __device__ int sh() const { return P() + nl; }
+    __host__ __device__ int Q() const { return sh() + 2; }
+    __host__ __device__ int mq() const { return Q() + nq; }
+    __host__ __device__ int words() const { return mq() + r * nq; }
 };
 
 // Residues of nlimbs-word coefficients modulo every prime:
@@ -297,15 +304,21 @@ __global__ void __launch_bounds__(128) mp_crt_kernel(const uint32_t *__restrict_
 // The same recombination with the prime count and the limb count bounded at
 // compile time (RM, NLM): every array index is then a constant, so v and the
 // accumulator stay in registers (the general kernel above indexes them at run
-// time and keeps them in scratch memory).
+// time and keeps them in scratch memory).  The accumulator is formed against
+// P moved to the top of NLM words (Q = P 2^(32 off + s), normalized), reduced
+// by two quotient digits of Knuth's division (mp_div.h) instead of 41 steps of
+// shift-and-subtract, and shifted back into nl words.
 template <int RM, int NLM>
 __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__restrict__ res, size_t entries,
                                                            size_t size, const uint32_t *__restrict__ K, int r, int nl,
                                                            DevView out)
 {
     const size_t total = entries * size;
-    const MpLayout lay{r, nl};
-    const uint32_t *P = K + lay.P();
+    const MpLayout lay{r, nl, NLM};
+    const int off = (int)K[lay.sh()], s = (int)K[lay.sh() + 1];
+    uint32_t q[NLM];
+#pragma unroll
+    for (int l = 0; l < NLM; ++l) q[l] = K[lay.Q() + l];
     for (size_t idx = (size_t)blockIdx.x * 128 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 128) {
         uint32_t v[RM];
         uint32_t acc[NLM + 2];
@@ -323,48 +336,28 @@ __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__res
                                   p);
                 }
                 v[i] = t;
-                // acc += v_i * (prod_{j<i} p_j mod P); limbs past nl only take the carry
-                const uint32_t *mi = K + lay.mi() + i * nl;
+                // acc += v_i * (prod_{j<i} p_j mod P) 2^(32 off + s)
+                const uint32_t *mi = K + lay.mq() + i * NLM;
                 uint64_t carry = 0;
 #pragma unroll
-                for (int l = 0; l < NLM + 2; ++l) {
-                    const uint64_t sum = (uint64_t)(l < nl ? mi[l] : 0u) * t + acc[l] + carry;
+                for (int l = 0; l < NLM; ++l) {
+                    const uint64_t sum = (uint64_t)mi[l] * t + acc[l] + carry;
                     acc[l] = (uint32_t)sum;
                     carry = sum >> 32;
                 }
+                const uint64_t top = (uint64_t)acc[NLM] + carry;
+                acc[NLM] = (uint32_t)top;
+                acc[NLM + 1] += (uint32_t)(top >> 32);
             }
         }
-        // acc < r 2^31 P <= 2^37 P: binary long division by P << sh, sh = 40..0
-        for (int sh = 40; sh >= 0; --sh) {
-            const int ws = sh >> 5, bs = sh & 31;
-            uint32_t shifted[NLM + 2];
-#pragma unroll
-            for (int l = 0; l < NLM + 2; ++l) {
-                const int src = l - ws;
-                uint32_t w = 0;
-                if (src >= 0 && src < nl) w = P[src] << bs;
-                if (bs && src - 1 >= 0 && src - 1 < nl) w |= P[src - 1] >> (32 - bs);
-                shifted[l] = w;
-            }
-            int cmp = 0;  // sign of acc - (P << sh), from the top limb down
-#pragma unroll
-            for (int l = NLM + 1; l >= 0; --l)
-                if (cmp == 0) cmp = acc[l] > shifted[l] ? 1 : (acc[l] < shifted[l] ? -1 : 0);
-            if (cmp >= 0) {
-                uint64_t borrow = 0;
-#pragma unroll
-                for (int l = 0; l < NLM + 2; ++l) {
-                    const uint64_t d = (uint64_t)acc[l] - shifted[l] - borrow;
-                    acc[l] = (uint32_t)d;
-                    borrow = (d >> 32) & 1;
-                }
-            }
-        }
+        // acc < r 2^31 Q <= 2^36 Q: acc mod Q = (X mod P) 2^(32 off + s) in acc[0..NLM)
+        knuth_rem2<NLM>(acc, q);
         const size_t e = idx / size, t = idx - e * size;
         uint32_t *dst = out.base + view_index(out, e, t) * nl;
 #pragma unroll
         for (int l = 0; l < NLM; ++l)
-            if (l < nl) dst[l] = acc[l];
+            if (l >= off) dst[l - off] = s ? (acc[l] >> s) | (acc[l + 1] << (32 - s)) : acc[l];
+        for (int l = NLM - off; l < nl; ++l) dst[l] = 0;  // words of P's leading zero limbs
     }
 }
 
@@ -460,6 +453,7 @@ lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t m
     }
     Limbs P(Pw, Pw + nl);
     limbs_trim(P);
+    if (P.size() == 1 && P[0] < 2) return LBX_ERR_INVALID_ARG;
     Limbs bound = limbs_mul(P, P);  // :616 bound = 2 P^2 k min(sa,sb)
     limbs_mul_small(bound, 2);
     limbs_mul_small(bound, (uint32_t)k);
@@ -467,8 +461,25 @@ lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t m
     std::vector<uint32_t> primes;
     if (!generate_primes((uint64_t)1 << 31, bound, lpts, primes)) return LBX_ERR_NOT_ENOUGH_PRIMES;
     const int r = (int)primes.size();
-    const MpLayout lay{r, nl};
+    // word count of the register kernel mp_product launches for (r, nl), 0 for the general one
+    const int nq = (r <= 16 && nl <= 8) ? 8 : ((r <= 32 && nl <= 16) ? 16 : 0);
+    const MpLayout lay{r, nl, nq};
     std::vector<uint32_t> K((size_t)lay.words(), 0);
+    // P moved to the top of nq words with its top bit set: 2^(32 off + s)
+    const int np = (int)P.size(), off = nq - np, sbits = __builtin_clz(P.back());
+    auto place = [&](const Limbs &x, uint32_t *dst) {  // x < P: x 2^(32 off + s) in nq words
+        uint32_t prev = 0;
+        for (int l = 0; l < np; ++l) {
+            const uint32_t w = l < (int)x.size() ? x[l] : 0u;
+            dst[off + l] = sbits ? (w << sbits) | prev : w;
+            prev = sbits ? w >> (32 - sbits) : 0u;
+        }
+    };
+    if (nq) {
+        K[lay.sh()] = (uint32_t)off;
+        K[lay.sh() + 1] = (uint32_t)sbits;
+        place(P, &K[lay.Q()]);
+    }
     Limbs M{1};
     for (int i = 0; i < r; ++i) {
         const uint32_t p = primes[i];
@@ -484,6 +495,7 @@ lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t m
         }
         const Limbs mi = limbs_mod(M, P);
         for (int l = 0; l < nl; ++l) K[lay.mi() + i * nl + l] = l < (int)mi.size() ? mi[l] : 0;
+        if (nq) place(mi, &K[lay.mq() + i * nq]);
         limbs_mul_small(M, p);
         M = limbs_mod(M, P);
     }
