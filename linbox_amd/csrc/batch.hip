@@ -350,6 +350,40 @@ void batch_release(lbx_ctx *ctx)
     ctx->batch_plans.clear();
 }
 
+// The batch kernels on jobs that are all batchable (Ls their lengths), with
+// device operands; the context lock is held by the caller.
+lbx_status run_batch_plan(lbx_ctx *ctx, const std::vector<lbx_job> &bj, const std::vector<unsigned> &Ls,
+                          hipStream_t s)
+{
+    const std::string sig = signature(bj);
+    BatchPlan *plan = nullptr;
+    auto it = ctx->batch_plans.find(sig);
+    if (it != ctx->batch_plans.end()) plan = it->second;
+    if (!plan) {
+        if (ctx->batch_plans.size() >= 8) batch_release(ctx);  // bounded cache
+        LBX_TRY(build_plan(ctx, bj, Ls, s, &plan));
+        ctx->batch_plans[sig] = plan;
+    }
+    for (size_t j = 0; j < bj.size(); ++j) {  // this call's operands
+        plan->jobs[j].a = bj[j].a;
+        plan->jobs[j].b = bj[j].b;
+        plan->jobs[j].c = bj[j].c;
+    }
+    LBX_TRY_HIP(hipMemcpyAsync(plan->jobs_d, plan->jobs.data(), plan->jobs.size() * sizeof(BatchJob),
+                               hipMemcpyHostToDevice, s));
+    LBX_TRY_HIP(hipGraphLaunch(plan->exec, s));
+    return LBX_OK;
+}
+
+lbx_status batch_products(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, hipStream_t s)
+{
+    std::vector<lbx_job> bj(jobs, jobs + njobs);
+    std::vector<unsigned> Ls(njobs);
+    for (size_t i = 0; i < njobs; ++i)
+        if (!batchable(jobs[i], &Ls[i])) return LBX_ERR_UNSUPPORTED;
+    return run_batch_plan(ctx, bj, Ls, s);
+}
+
 }  // namespace lbx
 
 using namespace lbx;
@@ -388,24 +422,11 @@ extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, s
     }
     if (!bj.empty()) {
         const std::string sig = signature(bj);
-        BatchPlan *plan = nullptr;
         auto it = ctx->batch_plans.find(sig);
-        if (it != ctx->batch_plans.end()) plan = it->second;
+        const BatchPlan *plan = it != ctx->batch_plans.end() ? it->second : nullptr;
         for (size_t j = 0; j < bj.size(); ++j)
             if (!pointers_ok(plan, j, bj[j])) return LBX_ERR_INVALID_ARG;
-        if (!plan) {
-            if (ctx->batch_plans.size() >= 8) batch_release(ctx);  // bounded cache
-            LBX_TRY(build_plan(ctx, bj, Ls, s, &plan));
-            ctx->batch_plans[sig] = plan;
-        }
-        for (size_t j = 0; j < bj.size(); ++j) {  // this call's operands
-            plan->jobs[j].a = bj[j].a;
-            plan->jobs[j].b = bj[j].b;
-            plan->jobs[j].c = bj[j].c;
-        }
-        LBX_TRY_HIP(hipMemcpyAsync(plan->jobs_d, plan->jobs.data(), plan->jobs.size() * sizeof(BatchJob),
-                                   hipMemcpyHostToDevice, s));
-        LBX_TRY_HIP(hipGraphLaunch(plan->exec, s));
+        LBX_TRY(run_batch_plan(ctx, bj, Ls, s));
     }
     for (size_t i : rest) {  // outside the batch kernels' range: single-product routes
         const lbx_job &j = jobs[i];

@@ -114,6 +114,12 @@ lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const De
 void mp_release(lbx_ctx *ctx);
 // Frees the batch plans (graphs, descriptors) cached in a context.
 void batch_release(lbx_ctx *ctx);
+// Independent polfirst products with device operands through the batch
+// kernels, the context lock held; LBX_ERR_UNSUPPORTED (nothing launched) when a
+// job is outside their range.
+lbx_status batch_products(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, hipStream_t s);
+// The pointwise form LBX_METHOD_AUTO resolves to for m x k x n at 2^L points.
+int choose_method(int method, size_t m, size_t k, size_t n, unsigned L);
 
 // Root of unity of order 2^k (fft-utils.h:83-104), its inverse (fft.h:119-130)
 // and the cached device twiddle table of a root (fft-integral.inl:1614-1650).

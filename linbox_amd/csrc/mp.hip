@@ -442,6 +442,23 @@ unsigned grid_of(size_t total, unsigned block)
     return (unsigned)(g < 8192 ? (g ? g : 1) : 8192);
 }
 
+// The r per-prime products c_i = a_i b_i (polfirst, contiguous; operand i at
+// a + i*astride) as one call of the batch kernels, when the pointwise form
+// this shape takes is their plain one: a few launches in place of r separate
+// products.  LBX_ERR_UNSUPPORTED (nothing launched) otherwise.
+lbx_status primes_as_batch(lbx_ctx *ctx, const std::vector<uint32_t> &primes, const uint32_t *a, size_t astride,
+                           const uint32_t *b, size_t bstride, uint32_t *c, size_t m, size_t k, size_t n, size_t sa,
+                           size_t sb, unsigned lpts, int method, hipStream_t s)
+{
+    if (sa + sb <= 2 || choose_method(method, m, k, n, lpts) != LBX_METHOD_PLAIN_U64) return LBX_ERR_UNSUPPORTED;
+    std::vector<lbx_job> jobs(primes.size());
+    for (size_t i = 0; i < primes.size(); ++i)
+        jobs[i] = lbx_job{primes[i], m, k, n, sa, sb, a + i * astride, b + i * bstride, c + i * m * n * (sa + sb - 1)};
+    return batch_products(ctx, jobs.data(), jobs.size(), s);
+}
+
+constexpr int kBatchMinPrimes = 6;
+
 lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t minsz, unsigned lpts, MpPlan **out)
 {
     std::string key((const char *)Pw, (size_t)nl * 4);
@@ -658,7 +675,14 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
     hipLaunchKernelGGL(mp_residue_kernel, dim3(grid_of(k * n * sb, 256)), dim3(256), 0, s, b, nl, k * n, sb,
                        plan->d_const, r, (uint32_t *)rb);
     LBX_TRY_HIP(hipGetLastError());
-    for (int i = 0; i < r; ++i) {
+    // many primes (multiword moduli): the products as one batch; a new shape
+    // builds and caches a batch plan (graph capture), which pays off here
+    lbx_status bst = LBX_ERR_UNSUPPORTED;
+    if (!mid && r >= kBatchMinPrimes)
+        bst = primes_as_batch(ctx, plan->primes, (const uint32_t *)ra, m * k * sa, (const uint32_t *)rb, k * n * sb,
+                              (uint32_t *)rc, m, k, n, sa, sb, lpts, method, s);
+    if (bst != LBX_ERR_UNSUPPORTED) LBX_TRY(bst);
+    for (int i = 0; i < r && bst == LBX_ERR_UNSUPPORTED; ++i) {
         DevView va{(uint32_t *)ra + (size_t)i * m * k * sa, LBX_POLFIRST, m, k, sa, 0, sa};
         DevView vb{(uint32_t *)rb + (size_t)i * k * n * sb, LBX_POLFIRST, k, n, sb, 0, sb};
         DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
