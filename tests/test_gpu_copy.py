@@ -40,7 +40,11 @@ def test_all_layout_pairs(ctx, src_lay, dst_lay):
     rng = np.random.default_rng(LAYOUTS.index(src_lay) * 3 + LAYOUTS.index(dst_lay))
     for (r, c, ss, ds, beg, end, start) in [(3, 5, 100, 100, 0, 99, 0), (7, 2, 130, 70, 17, 80, 3),
                                             (1, 1, 5, 5, 2, 2, 4), (65, 66, 70, 200, 0, 69, 130),
-                                            (9, 130, 64, 64, 0, 63, 0)]:
+                                            (9, 130, 64, 64, 0, 63, 0),
+                                            # entry counts and polfirst offsets that are multiples of four take the
+                                            # 16-byte kernel for polfirst <-> matfirst (with a coefficient tail here)
+                                            (4, 6, 100, 120, 8, 97, 4), (8, 8, 64, 64, 0, 63, 0),
+                                            (16, 20, 200, 136, 4, 134, 0), (4, 4, 70, 70, 2, 40, 6)]:
         say("copy", src_lay, dst_lay, r, c, ss, ds, beg, end, start)
         src = PolynomialMatrix.from_polfirst(rand(rng, r * c * ss).reshape(r * c, ss), r, c, src_lay)
         fill = rand(rng, r * c * ds).reshape(r * c, ds)
@@ -86,6 +90,12 @@ def test_device_pointers_and_round_trip_at_operand_size(ctx):
     mrf = bufs[2].download()
     i, j = e // c, e % c
     assert np.array_equal(mrf[i * c * s + j + t * c], host[idx])
+    assert np.array_equal(bufs[3].download(), host)
+    # matfirst -> polfirst directly (the 16-byte kernel's other direction)
+    back = PolynomialMatrix(r, c, s, "polfirst", data=bufs[3])
+    bufs[3].zero()
+    back.copy(mats[1], ctx=ctx)
+    synchronize()
     assert np.array_equal(bufs[3].download(), host)
 
 
