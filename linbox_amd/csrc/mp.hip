@@ -210,6 +210,16 @@ struct MpLayout {
     __host__ __device__ int words() const { return mq() + r * nq; }
 };
 
+// Word index of element idx (entry idx / size, coefficient idx % size) of a
+// view.  A polfirst view without window (stride = size, no shift) maps idx to
+// itself, which spares the per-element 64-bit division.
+__device__ __forceinline__ size_t elem_index(const DevView &v, size_t idx, size_t size)
+{
+    if (v.layout == 0 && v.stor == size && v.shift == 0) return idx;
+    const size_t e = idx / size, t = idx - e * size;
+    return view_index(v, e, t);
+}
+
 // Residues of nlimbs-word coefficients modulo every prime:
 // out[i][e*size + t] for e < entries, t < size.
 __global__ void __launch_bounds__(256) mp_residue_kernel(DevView in, int nl, size_t entries, size_t size,
@@ -219,8 +229,7 @@ __global__ void __launch_bounds__(256) mp_residue_kernel(DevView in, int nl, siz
     const size_t total = entries * size;
     const MpLayout lay{r, nl};
     for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
-        const size_t e = idx / size, t = idx - e * size;
-        const uint32_t *src = in.base + view_index(in, e, t) * nl;
+        const uint32_t *src = in.base + elem_index(in, idx, size) * nl;
         uint32_t limb[kMaxLimbs];
         for (int l = 0; l < nl; ++l) limb[l] = src[l];
         for (int i = 0; i < r; ++i) {
@@ -295,8 +304,7 @@ __global__ void __launch_bounds__(128) mp_crt_kernel(const uint32_t *__restrict_
                 }
             }
         }
-        const size_t e = idx / size, t = idx - e * size;
-        uint32_t *dst = out.base + view_index(out, e, t) * nl;
+        uint32_t *dst = out.base + elem_index(out, idx, size) * nl;
         for (int l = 0; l < nl; ++l) dst[l] = acc[l];
     }
 }
@@ -352,8 +360,7 @@ __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__res
         }
         // acc < r 2^31 Q <= 2^36 Q: acc mod Q = (X mod P) 2^(32 off + s) in acc[0..NLM)
         knuth_rem2<NLM>(acc, q);
-        const size_t e = idx / size, t = idx - e * size;
-        uint32_t *dst = out.base + view_index(out, e, t) * nl;
+        uint32_t *dst = out.base + elem_index(out, idx, size) * nl;
 #pragma unroll
         for (int l = 0; l < NLM; ++l)
             if (l >= off) dst[l - off] = s ? (acc[l] >> s) | (acc[l + 1] << (32 - s)) : acc[l];
@@ -412,8 +419,7 @@ __global__ void __launch_bounds__(256) few_primes_crt_kernel(const uint32_t *__r
                 }
             }
         }
-        const size_t e = idx / size, t = idx - e * size;
-        out.base[view_index(out, e, t)] = acc;
+        out.base[elem_index(out, idx, size)] = acc;
     }
 }
 
@@ -427,8 +433,7 @@ __global__ void __launch_bounds__(256) fp_residue_kernel(DevView in, size_t entr
     const size_t total = entries * size;
     const FpLayout lay{r};
     for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
-        const size_t e = idx / size, t = idx - e * size;
-        const uint32_t x = in.base[view_index(in, e, t)];
+        const uint32_t x = in.base[elem_index(in, idx, size)];
         for (int i = 0; i < r; ++i)
             out[(size_t)i * total + idx] = shoup_mul(x, 1u, K[lay.onep() + i], K[lay.primes() + i]);
     }
