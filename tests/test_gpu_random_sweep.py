@@ -5,7 +5,8 @@ so the sweep is reproducible; every drawn case prints its parameters before it
 runs.  Products: any prime below 2^32 (FFT primes, word primes that take the CRT
 route, primes of too small 2-valuation), both forms, all layouts.  Middle
 products: both operand orders and explicit windows.  Order bases: M_Basis and
-PM_Basis with random shifts.
+PM_Basis with random shifts.  Layout conversions: random layouts, windows,
+offsets and views on either side.
 """
 import sys
 
@@ -120,3 +121,35 @@ def test_random_order_bases(ctx):
         st, ref, dref, shref = (orc.pm_basis if which == "pm" else orc.m_basis)(p, m, n, serie, order, order, shift)
         assert st == 0 and d == dref and sh == [int(x) for x in shref], case
         assert np.array_equal(S.to_polfirst(), ref.reshape(m * m, order + 1)[:, :d + 1]), case
+
+
+def test_random_copies(ctx):
+    rng = np.random.default_rng(20261205)
+    for case in range(100):
+        r, c = (int(x) for x in rng.integers(1, 24, 2))
+        if rng.integers(0, 3) == 0:  # entry counts that are multiples of four (the 16-byte kernel)
+            r = 4 * int(rng.integers(1, 8))
+        ss = int(rng.integers(1, 300))
+        beg = int(rng.integers(0, ss))
+        end = int(rng.integers(beg, ss))
+        cnt = end - beg + 1
+        ds = cnt + int(rng.integers(0, 40))
+        start = int(rng.integers(0, ds - cnt + 1))
+        ls, ld = str(rng.choice(LAYOUTS)), str(rng.choice(LAYOUTS))
+        sview, dview = bool(rng.integers(0, 2)), bool(rng.integers(0, 2))
+        say("copy", case, r, c, ss, ds, beg, end, start, ls, ld, sview, dview)
+        s0 = int(rng.integers(0, 9)) if sview else 0  # views: extra coefficients before the window
+        d0 = int(rng.integers(0, 9)) if dview else 0
+        sdata = rng.integers(0, 1 << 32, (r * c, ss + s0 + 3), dtype=np.uint64).astype(np.uint32)
+        S = PolynomialMatrix.from_polfirst(sdata, r, c, ls)
+        src = S.at(s0, s0 + ss - 1) if sview else S
+        if not sview:
+            S = PolynomialMatrix.from_polfirst(sdata[:, :ss], r, c, ls)
+            src = S
+        ddata = rng.integers(0, 1 << 32, (r * c, ds + d0 + 2), dtype=np.uint64).astype(np.uint32)
+        D = PolynomialMatrix.from_polfirst(ddata if dview else ddata[:, :ds], r, c, ld)
+        dst = D.at(d0, d0 + ds - 1) if dview else D
+        dst.copy(src, beg, end, start, ctx=ctx)
+        want = (ddata if dview else ddata[:, :ds]).copy()
+        want[:, d0 + start:d0 + start + cnt] = sdata[:, s0 + beg:s0 + end + 1]
+        assert np.array_equal(D.to_polfirst(), want), case
