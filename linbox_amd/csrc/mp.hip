@@ -328,15 +328,17 @@ __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__res
 #pragma unroll
     for (int l = 0; l < NLM; ++l) q[l] = K[lay.Q() + l];
     for (size_t idx = (size_t)blockIdx.x * 128 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 128) {
-        uint32_t v[RM];
+        uint32_t v[RM], rv[RM];
         uint32_t acc[NLM + 2];
 #pragma unroll
         for (int l = 0; l < NLM + 2; ++l) acc[l] = 0;
 #pragma unroll
+        for (int i = 0; i < RM; ++i) rv[i] = i < r ? res[(size_t)i * total + idx] : 0u;  // loads issued together
+#pragma unroll
         for (int i = 0; i < RM; ++i) {
             if (i < r) {
                 const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
-                uint32_t t = res[(size_t)i * total + idx];
+                uint32_t t = rv[i];
 #pragma unroll
                 for (int j = 0; j < i; ++j) {
                     const uint32_t vj = shoup_mul(v[j], 1u, onep, p);
@@ -393,13 +395,15 @@ __global__ void __launch_bounds__(256) few_primes_crt_kernel(const uint32_t *__r
     const size_t total = entries * size;
     const FpLayout lay{r};
     for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
-        uint32_t v[RM];
+        uint32_t v[RM], rv[RM];
         uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < RM; ++i) rv[i] = i < r ? res[(size_t)i * total + idx] : 0u;  // loads issued together
 #pragma unroll
         for (int i = 0; i < RM; ++i) {
             if (i < r) {
                 const uint32_t pi = K[lay.primes() + i], onep = K[lay.onep() + i];
-                uint32_t t = res[(size_t)i * total + idx];
+                uint32_t t = rv[i];
 #pragma unroll
                 for (int j = 0; j < i; ++j) {
                     const uint32_t vj = shoup_mul(v[j], 1u, onep, pi);
