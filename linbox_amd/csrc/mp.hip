@@ -222,6 +222,9 @@ __device__ __forceinline__ size_t elem_index(const DevView &v, size_t idx, size_
 
 // Residues of nlimbs-word coefficients modulo every prime:
 // out[i][e*size + t] for e < entries, t < size.
+// The limb count is bounded at compile time (NLM >= nl), so the limbs stay in
+// registers and the Horner steps unroll.
+template <int NLM>
 __global__ void __launch_bounds__(256) mp_residue_kernel(DevView in, int nl, size_t entries, size_t size,
                                                          const uint32_t *__restrict__ K, int r,
                                                          uint32_t *__restrict__ out)
@@ -230,14 +233,16 @@ __global__ void __launch_bounds__(256) mp_residue_kernel(DevView in, int nl, siz
     const MpLayout lay{r, nl};
     for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
         const uint32_t *src = in.base + elem_index(in, idx, size) * nl;
-        uint32_t limb[kMaxLimbs];
-        for (int l = 0; l < nl; ++l) limb[l] = src[l];
+        uint32_t limb[NLM];
+#pragma unroll
+        for (int l = 0; l < NLM; ++l) limb[l] = l < nl ? src[l] : 0u;
         for (int i = 0; i < r; ++i) {
             const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
             const uint32_t c = K[lay.r32() + i], cp = K[lay.r32p() + i];
             uint32_t v = 0;
-            for (int l = nl - 1; l >= 0; --l)
-                v = add_mod(shoup_mul(v, c, cp, p), shoup_mul(limb[l], 1u, onep, p), p);
+#pragma unroll
+            for (int l = NLM - 1; l >= 0; --l)
+                if (l < nl) v = add_mod(shoup_mul(v, c, cp, p), shoup_mul(limb[l], 1u, onep, p), p);
             out[(size_t)i * total + idx] = v;
         }
     }
@@ -678,11 +683,12 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
     LBX_TRY(ensure(ctx, "mp_ra", (size_t)r * m * k * sa * 4, &ra));
     LBX_TRY(ensure(ctx, "mp_rb", (size_t)r * k * n * sb * 4, &rb));
     LBX_TRY(ensure(ctx, "mp_rc", (size_t)r * m * n * sc * 4, &rc));
-    hipLaunchKernelGGL(mp_residue_kernel, dim3(grid_of(m * k * sa, 256)), dim3(256), 0, s, a, nl, m * k, sa,
-                       plan->d_const, r, (uint32_t *)ra);
+    auto residues = nl <= 8 ? mp_residue_kernel<8> : mp_residue_kernel<kMaxLimbs>;
+    hipLaunchKernelGGL(residues, dim3(grid_of(m * k * sa, 256)), dim3(256), 0, s, a, nl, m * k, sa, plan->d_const, r,
+                       (uint32_t *)ra);
     LBX_TRY_HIP(hipGetLastError());
-    hipLaunchKernelGGL(mp_residue_kernel, dim3(grid_of(k * n * sb, 256)), dim3(256), 0, s, b, nl, k * n, sb,
-                       plan->d_const, r, (uint32_t *)rb);
+    hipLaunchKernelGGL(residues, dim3(grid_of(k * n * sb, 256)), dim3(256), 0, s, b, nl, k * n, sb, plan->d_const, r,
+                       (uint32_t *)rb);
     LBX_TRY_HIP(hipGetLastError());
     // many primes (multiword moduli): the products as one batch; a new shape
     // builds and caches a batch plan (graph capture), which pays off here
