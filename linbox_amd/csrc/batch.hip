@@ -360,7 +360,10 @@ lbx_status run_batch_plan(lbx_ctx *ctx, const std::vector<lbx_job> &bj, const st
     auto it = ctx->batch_plans.find(sig);
     if (it != ctx->batch_plans.end()) plan = it->second;
     if (!plan) {
-        if (ctx->batch_plans.size() >= 8) batch_release(ctx);  // bounded cache
+        if (ctx->batch_plans.size() >= 8) {  // bounded cache
+            (void)hipDeviceSynchronize();      // earlier launches of these graphs may still run
+            batch_release(ctx);
+        }
         LBX_TRY(build_plan(ctx, bj, Ls, s, &plan));
         ctx->batch_plans[sig] = plan;
     }

@@ -4,9 +4,12 @@
   convolution fixtures).
 * Configuration 4: 32x32 . 32x32, degree 1023, coefficients uniform in [0,P),
   14 NTT primes, word-for-word against the oracle's CRT product.
-* Word-size moduli that are not FFT primes (lbx_polymat_mul_u32 routes them
-  through the same CRT path): 1000000007 and the largest 32-bit prime, against
-  the oracle's naive product.
+* Word-size moduli that are not FFT primes: 1000000007 and the largest 32-bit
+  prime, against the oracle's naive product.
+* Ten multiword products of distinct shapes queued on device buffers with no
+  synchronisation between them: their per-prime products go through the batch
+  kernels, and the tenth shape evicts the cached batch plans while earlier
+  launches may still be queued; every result against the oracle.
 """
 import json
 import os
@@ -145,3 +148,33 @@ def test_crt_wide_moduli(ctx, bits):
                     for u in range(max(0, t - sb + 1), min(t, sa - 1) + 1):
                         ref += val(a, (i * k + q) * sa + u) * val(b, (q * n + j) * sb + (t - u))
                 assert val(got, ((i * n + j) * (sa + sb - 1)) + t) == ref % P, (bits, i, j, t)
+
+
+def test_batched_primes_with_plan_eviction(ctx, crt):
+    from linbox_amd.hip import DeviceBuffer, synchronize
+    P, nl = int(crt["P_hex"], 16), crt["nlimbs"]
+    top = (P >> (32 * (nl - 1))) & 0xFFFFFFFF
+    rng = np.random.default_rng(10)
+    shapes = [(3 + i % 3, 4 + i, 2 + i % 4, 40 + 7 * i, 33 + 5 * i) for i in range(10)]
+    cases, keep = [], []
+    for (m, k, n, sa, sb) in shapes:
+        def rand_mp(count):
+            w = rng.integers(0, 1 << 32, (count, nl), dtype=np.uint64)
+            w[:, nl - 1] = rng.integers(0, top, count, dtype=np.uint64)
+            return w.astype(np.uint32).reshape(-1)
+        a, b = rand_mp(m * k * sa), rand_mp(k * n * sb)
+        da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+        dc = DeviceBuffer(m * n * (sa + sb - 1) * nl * 4)
+        A = PolynomialMatrix(m, k, sa, nlimbs=nl, data=da)
+        B = PolynomialMatrix(k, n, sb, nlimbs=nl, data=db)
+        C = PolynomialMatrix(m, n, sa + sb - 1, nlimbs=nl, data=dc)
+        say("queued multiword product", m, k, n, sa, sb)
+        lbx.mul_mp(P, C, A, B, ctx=ctx, method=lbx.METHOD_PLAIN_U64)
+        cases.append((m, k, n, sa, sb, a, b, dc))
+        keep.append((da, db))
+    synchronize()
+    for (m, k, n, sa, sb, a, b, dc) in cases:
+        st, ref, primes = orc.mul_crt(np.array([(P >> (32 * i)) & 0xFFFFFFFF for i in range(nl)], np.uint32),
+                                      m, k, n, a, sa, b, sb)
+        assert st == 0 and len(primes) >= 6
+        assert np.array_equal(dc.download(), ref), (m, k, n, sa, sb)
