@@ -12,9 +12,9 @@ The JSON line also carries:
                 HBM bytes per launch divided by its average launch time, measured
                 live with HIP events recorded on the launch stream during the
                 timed steps; `traffic` is the measured HBM bytes per launch from
-                the committed rocprofv3 PMC summary (profiles/hbm_traffic_c3.txt);
+                the committed rocprofv3 PMC summary (profiles/hbm_traffic_128x128_deg8191.txt);
   kernels       the same figures for every kernel of the product;
-  batch_config5 1024 uneven products as one batch call vs one call per product;
+  batch_1024_products 1024 uneven products as one batch call vs one call per product;
   order_basis   PM_Basis of a 64 x 32 matrix power series at order 4096;
   other_configs 16x16 deg 255, 64x64 deg 4095 (both forms), 32x32 deg 1023 over a
                 192-bit modulus, a PM_Basis-level middle product, 64x64 deg 4095
@@ -65,7 +65,7 @@ def kernel_bytes(m, k, n, sa, sb, pts):
 def traffic_table():
     """Measured HBM bytes per launch, from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a device-to-device copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_c3.txt")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_128x128_deg8191.txt")
     table = {}
     if os.path.exists(path):
         for line in open(path):  # rows "| kernel <label> | ... | <bytes per launch> |"
@@ -144,7 +144,7 @@ def run_gpu(args):
 
 
 def run_batch(steps=5):
-    """Configuration 5: 1024 independent products (m,k,n in {2,...,64}, degree log-uniform in
+    """The 1024-product batch: independent products (m,k,n in {2,...,64}, degree log-uniform in
     [8,4096], prime uniform from four FFT primes) as one lbx_polymat_mul_batch call
     (replayed hipGraph of persistent work-queue kernels) vs one lbx_polymat_mul_u32 per product."""
     import linbox_amd as lbx
@@ -210,7 +210,7 @@ def staged_bytes(m, k, n, sa, sb):
 
 
 def run_configs(reps=10):
-    """The other configurations of the workload table, timed with HIP events on one stream:
+    """The other workloads, timed with HIP events on one stream:
     16x16 deg 255 (p = 998244353), 64x64 deg 4095 (p = 2013265921, both pointwise forms),
     32x32 deg 1023 over a 192-bit P (14 FFT primes and CRT), the middle product of a
     PM_Basis level (64x64 sigma of degree 2048 times a 64x32 serie of 4096 terms), 64x64
@@ -543,7 +543,7 @@ def main():
         "gmulmod_per_s": round(40265318400 / (ms_int8 * 1e-3) / 1e9, 2),
         "wall_s_timed": round(wall, 4),
         "roofline": roofline,
-        "batch_config5": batch,
+        "batch_1024_products": batch,
         "order_basis": order_basis,
         "other_configs": configs,
         "cpu_baseline": cpu,

@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256, 2) pw_mfma_lds_kernel(const uint8_t *__re
 // of at most 64 KiB.  Each wave multiplies one 32-row band of A, which it
 // loads straight into registers (64 VGPRs at k = 128); only B, which every
 // wave needs, goes through LDS, whole, with one LDS-direct copy.  A point
-// therefore costs one wait and one barrier, with A and B (128 KiB at C3) in
+// therefore costs one wait and one barrier, with A and B (128 KiB for 128x128 operands) in
 // flight together; two groups per CU overlap one's loads with the other's
 // multiplies.  With nrt = mpad/32 < 4 bands, wave w takes band w % nrt and
 // every (4/nrt)-th column block from w / nrt.

@@ -5,7 +5,7 @@
   prime, a 2^14-point product) against the oracle, job by job.
 * Graph replays: the same batch shape with new data, with the program's own
   eager products and synchronisations between replays; every replay is checked.
-* Configuration 5: 1024 products, m,k,n in {2,...,64}, degree log-uniform in
+* The 1024-product batch: 1024 products, m,k,n in {2,...,64}, degree log-uniform in
   [8,4096], prime uniform from the four; every job against one
   lbx_polymat_mul_u32 call (itself checked against the oracle elsewhere), and a
   sample of jobs against the oracle.
@@ -110,8 +110,8 @@ def test_batch_graph_replays(ctx):
             assert np.array_equal(j["dc"].download(), oracle_ref(j)), (rep, i)
 
 
-def test_batch_config5(ctx):
-    """1024 products of the configuration-5 mix against one product call per job."""
+def test_batch_1024_products(ctx):
+    """The 1024-product mix as one batch against one product call per job."""
     rng = np.random.default_rng(0x4C494E424F58 + 4)
     dims = [2, 4, 8, 16, 32, 64]
     specs = []
@@ -120,11 +120,11 @@ def test_batch_config5(ctx):
         d = int(round(np.exp(rng.uniform(np.log(8), np.log(4096)))))
         specs.append((int(rng.choice(PRIMES)), m, k, n, d + 1, d + 1))
     jobs = make_jobs(rng, specs)
-    say("config5 batch")
+    say("1024-product batch")
     lbx.mul_batch(jobs, ctx=ctx)
     synchronize()
     got = [j["dc"].download() for j in jobs]
-    say("config5 per-product calls")
+    say("1024-product per-product calls")
     for i, j in enumerate(jobs):
         j["dc"].zero()
         C = PolynomialMatrix(j["m"], j["n"], j["sa"] + j["sb"] - 1, data=j["dc"])
@@ -134,7 +134,7 @@ def test_batch_config5(ctx):
     synchronize()
     for i, j in enumerate(jobs):
         assert np.array_equal(got[i], j["dc"].download()), i
-    say("config5 oracle sample")
+    say("1024-product oracle sample")
     for i in range(0, 1024, 97):
         if jobs[i]["m"] * jobs[i]["k"] * jobs[i]["n"] * jobs[i]["sa"] <= 64 * 64 * 64 * 600:
             assert np.array_equal(got[i], oracle_ref(jobs[i])), i

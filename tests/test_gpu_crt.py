@@ -2,7 +2,7 @@
 
 * crt_mul.json: products over a seeded random 192-bit prime P (Python big-int
   convolution fixtures).
-* Configuration 4: 32x32 . 32x32, degree 1023, coefficients uniform in [0,P),
+* 32x32 . 32x32, degree 1023, coefficients uniform in [0,P) for the 192-bit P,
   14 NTT primes, word-for-word against the oracle's CRT product.
 * Word-size moduli that are not FFT primes: 1000000007 and the largest 32-bit
   prime, against the oracle's naive product.
