@@ -115,3 +115,21 @@ def test_errors(ctx):
     with pytest.raises(LinboxError) as e:  # one storage for both
         a.at(0, 4).copy(a.at(5, 9), ctx=ctx)
     assert e.value.status == 7
+
+
+def test_many_entries(ctx):
+    """300 x 300 entries (more than 2^16), both kernels: matrowfirst -> polfirst and polfirst -> matfirst."""
+    rng = np.random.default_rng(300)
+    r = c = 300
+    s = 6
+    host = rand(rng, r * c * s).reshape(r * c, s)
+    say("many entries", r, c, s)
+    src = PolynomialMatrix.from_polfirst(host, r, c, "matrowfirst")
+    mid = PolynomialMatrix(r, c, s, "polfirst")
+    mid.copy(src, ctx=ctx)
+    assert np.array_equal(mid.to_polfirst(), host)
+    out = PolynomialMatrix(r, c, s, "matfirst")
+    out.copy(mid, 1, 4, 2, ctx=ctx)
+    want = np.zeros_like(host)
+    want[:, 2:6] = host[:, 1:5]
+    assert np.array_equal(out.to_polfirst(), want)

@@ -217,3 +217,21 @@ def test_word_moduli_above_2_29(ctx, p):
                 assert np.array_equal(got, orc.mul_naive(p, m, k, n, a, sa, b, sb))
             st, ref, _ = orc.mul_crt(np.array([p], np.uint32), m, k, n, a, sa, b, sb)
             assert st == 0 and np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("p", [268435399, 1000000007])
+def test_longest_transform(ctx, p):
+    """2^15 points (degree 32766), the longest LDS-resident transform: operands used as they are
+    (268435399) and reduced first (1000000007, below which a basis prime lies)."""
+    rng = np.random.default_rng(p % 997)
+    m, k, n, s = 2, 2, 2, 16384
+    a = rand(rng, p, m * k * s)
+    b = rand(rng, p, k * n * s)
+    say("few primes 2^15 points", p)
+    A = PolynomialMatrix.from_polfirst(a.reshape(m * k, s), m, k)
+    B = PolynomialMatrix.from_polfirst(b.reshape(k * n, s), k, n)
+    C = PolynomialMatrix(m, n, 1)
+    PolynomialMatrixFFTMulDomain(p, ctx).mul(C, A, B)  # the three-primes mirror refuses 30-bit primes
+    got = C.to_polfirst().reshape(-1)
+    st, ref, primes = orc.mul_crt(np.array([p], np.uint32), m, k, n, a, s, b, s)
+    assert st == 0 and np.array_equal(got, ref)
