@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                 const uint64_t bal = __ballot(v != 0);
                 if (bal && !found) {
                     found = true;
-                    if (lane == 0) atomicMin(&smin[0], (unsigned)(r * n + __builtin_ctzll(bal)));
+                    if (lane == 0) atomicMin(&smin[0], (unsigned)(r << 8 | __builtin_ctzll(bal)));
                 }
             }
         }
@@ -407,9 +407,11 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
         // ---- 3. row rank profile and combinations (the PLUQ of :470-488) ----
         int rank = 0;
         for (;;) {
+            // a pivot position is encoded row << 8 | column (n <= 64): the same row-major
+            // order as row * n + column, decoded without a division on the pivot's path
             const unsigned hit = smin[rank % 3];
             if (hit == kNone) break;
-            const int s = (int)(hit / n), c0 = (int)(hit - (unsigned)s * n);
+            const int s = (int)(hit >> 8), c0 = (int)(hit & 255u);
             const uint32_t a = R[s * n + c0], sc_s = scale[s];
             // Each row below s is the word vector [R_r | Z_r], updated as a.x - b.y
             // against [R_s | Z_s] with Z_s[rank] read as -scale_s (so that
@@ -456,8 +458,8 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                 const uint64_t bal = __ballot(nz), bal2 = __ballot(nz2);
                 if (!found && (bal || bal2)) {
                     found = true;
-                    const unsigned hit = bal ? (unsigned)(r * n + __builtin_ctzll(bal))
-                                             : (unsigned)(r2 * n + __builtin_ctzll(bal2));
+                    const unsigned hit = bal ? (unsigned)(r << 8 | __builtin_ctzll(bal))
+                                             : (unsigned)(r2 << 8 | __builtin_ctzll(bal2));
                     if (lane == 0) atomicMin(nxt, hit);
                 }
             }
