@@ -479,19 +479,26 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
             const uint32_t inv = scale[r];
             if (lane < rank) Z[r * n + lane] = mulm(Z[r * n + lane], inv, p, mu);  // plain product: Z~/scale~ = Z/scale
         }
-        // Qt: pivot rows first, then the others, each in increasing order
-        for (int r = tid; r < m; r += kMbThreads) {
-            int pos = pivot_of[r];
-            if (pos < 0) {
-                int before = 0;
-                for (int q = 0; q < r; ++q) before += pivot_of[q] < 0 ? 1 : 0;
-                pos = rank + before;
+        // Qt: pivot rows first, then the others, each in increasing order; a
+        // non-pivot row's place counts the non-pivot rows above it (m <= 128:
+        // two waves, one ballot per 64 rows)
+        static_assert(kMbMaxM <= 2 * 64 && kMbThreads >= 2 * 64, "two ballots cover every row");
+        if (tid < 2 * 64) {
+            const uint64_t b0 = __ballot(lane < m && pivot_of[lane] < 0);
+            const uint64_t b1 = __ballot(lane + 64 < m && pivot_of[lane + 64] < 0);
+            const int r = tid;
+            if (r < m) {
+                int pos = pivot_of[r];
+                if (pos < 0) {
+                    const uint64_t below = (1ull << lane) - 1;
+                    pos = rank + (wave == 0 ? __popcll(b0 & below) : __popcll(b0) + __popcll(b1 & below));
+                }
+                oldrow[pos] = r;
+                tmpmap[pos] = rowmap[r];
+                tmp64[pos] = shift[r];
+                tmp64b[pos] = degree[r];
+                was_piv[rowmap[r]] = pivot_of[r] >= 0;
             }
-            oldrow[pos] = r;
-            tmpmap[pos] = rowmap[r];
-            tmp64[pos] = shift[r];
-            tmp64b[pos] = degree[r];
-            was_piv[rowmap[r]] = pivot_of[r] >= 0;
         }
         __syncthreads();
         for (int i = tid; i < m; i += kMbThreads) {
