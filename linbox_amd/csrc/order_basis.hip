@@ -259,6 +259,9 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                         best = c1 < best ? c1 : best;
                     }
                     best = __ockl_wfred_min_u32(best);  // wave-wide minimum (DPP), uniform
+                    // every remaining key holds the largest shift: each later step would
+                    // pick the element already in place, so the order is final
+                    if ((best >> 7) == (uint32_t)(hi - lo)) break;
                     const int im = (int)(best & 127);
                     if (im != i) {
                         const int li = i & 63, lm = im & 63;
@@ -291,6 +294,7 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
                     const uint64_t o = __shfl_xor(best, off);
                     best = o < best ? o : best;
                 }
+                if ((best >> 7) == hi) break;  // as above: the rest is in place
                 const int im = (int)(best & 127);
                 if (im != i) {
                     const int li = i & 63, lm = im & 63;
