@@ -22,7 +22,10 @@
  *  - Errors are returned as lbx_status; the codes map one-to-one onto the
  *    reference's throw sites (see lbx_status).  lbx_strerror gives the text.
  *  - A context serialises the calls made on it; use one context per host
- *    thread for concurrency.
+ *    thread for concurrency.  Its workspace is shared by all its calls: a
+ *    call on a stream other than the previous call's stream first makes its
+ *    stream wait for the previous call's work, so calls on different streams
+ *    through one context are correct but do not overlap on the device.
  */
 #ifndef LBX_LBX_H
 #define LBX_LBX_H
@@ -48,7 +51,8 @@ typedef enum lbx_status {
     LBX_ERR_HIP = 9,                /* a HIP runtime call failed */
     LBX_ERR_OUT_OF_MEMORY = 10,     /* device allocation failed */
     LBX_ERR_UNSUPPORTED = 11,       /* shape beyond what the library implements */
-    LBX_ERR_NO_DEVICE = 12          /* no gfx950 device at the requested index */
+    LBX_ERR_NO_DEVICE = 12,         /* no gfx950 device at the requested index */
+    LBX_ERR_WORKSPACE_TOO_SMALL = 13 /* the call needs more device memory than its fixed workspace holds */
 } lbx_status;
 
 typedef enum lbx_layout {
@@ -127,9 +131,13 @@ lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t
  * primes covering (P-1)^2 k min(sa,sb), the operands used as they are when
  * every prime exceeds P (reduced first otherwise), and the mixed-radix
  * reconstruction mod P (:164-181); the product is the same.
- * c->size must be >= a.size+b.size-1 and is set to that value. */
+ * s = max_rowdeg ? max_rowdeg+1 : a.size+b.size-1 coefficients are written
+ * (mul_crtla :150-152, three-primes.inl:69-73): with s < a.size+b.size-1 the
+ * cyclic product mod x^pts - 1, pts the least power of two >= s, truncated;
+ * a.size, b.size <= pts.  c->size must be >= s and is set to s. */
 lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nlimbs, lbx_polmat *c,
-                              const lbx_polmat *a, const lbx_polmat *b, int method, void *stream);
+                              const lbx_polmat *a, const lbx_polmat *b, size_t max_rowdeg, int method,
+                              void *stream);
 
 /* One product of a batch (device pointers, polfirst storage, whole matrices). */
 typedef struct lbx_job {
@@ -184,7 +192,7 @@ lbx_status lbx_polymat_midproduct_naive_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat
  * (:274, lbx_polymat_midproduct_u32) and a product (:297, lbx_polymat_mul_u32)
  * on device buffers.  sigma->size must be >= order+1 on entry; coefficients
  * 0..*degree are written and sigma->size is set to *degree+1 (the
- * reference's sigma.resize).  Limits: m <= 128, n <= 64, m*n <= 4096, order < 2^15,
+ * reference's sigma.resize).  Limits: m <= 128, n <= 64, m*n <= 4096, order < 2^27 (transform lengths up to 2^27),
  * shift entries < 2^48 (else LBX_ERR_UNSUPPORTED).  Any p < 2^32.  The call
  * returns when the basis is complete (the recursion reads each half's degree).
  * The discrepancy's PLUQ (:470-473) is taken in its rank-profile form: pivot

@@ -17,7 +17,7 @@ STATUS_NAMES = {
     0: "LBX_OK", 1: "LBX_ERR_BAD_FFT_PRIME", 2: "LBX_ERR_K_TOO_LARGE", 3: "LBX_ERR_K_NOT_POSITIVE",
     4: "LBX_ERR_NOT_PRIMITIVE_ROOT", 5: "LBX_ERR_SIZE_MISMATCH", 6: "LBX_ERR_NOT_ENOUGH_PRIMES",
     7: "LBX_ERR_INVALID_ARG", 8: "LBX_ERR_PRIME_TOO_LARGE", 9: "LBX_ERR_HIP", 10: "LBX_ERR_OUT_OF_MEMORY",
-    11: "LBX_ERR_UNSUPPORTED", 12: "LBX_ERR_NO_DEVICE",
+    11: "LBX_ERR_UNSUPPORTED", 12: "LBX_ERR_NO_DEVICE", 13: "LBX_ERR_WORKSPACE_TOO_SMALL",
 }
 POLFIRST, MATFIRST, MATROWFIRST = 0, 1, 2
 METHOD_AUTO, METHOD_PLAIN_U64, METHOD_INT8_MFMA = 0, 1, 2
@@ -78,7 +78,7 @@ def lib():
         L.lbx_polymat_mul_fft_u32.argtypes = [vp, u32, ctypes.c_uint, sz, sz, sz, vp, vp, vp, i32, vp]
         L.lbx_ntt_forward_u32.argtypes = [vp, u32, ctypes.c_uint, u32, vp, sz, ctypes.POINTER(u32), vp]
         L.lbx_ntt_inverse_u32.argtypes = [vp, u32, ctypes.c_uint, u32, vp, sz, ctypes.POINTER(u32), vp]
-        L.lbx_polymat_mul_mp.argtypes = [vp, vp, i32, PM, PM, PM, i32, vp]
+        L.lbx_polymat_mul_mp.argtypes = [vp, vp, i32, PM, PM, PM, sz, i32, vp]
         L.lbx_polymat_mul_batch.argtypes = [vp, ctypes.POINTER(Job), sz, i32, vp]
         L.lbx_polymat_midproduct_u32.argtypes = [vp, u32, PM, PM, PM, i32, sz, sz, i32, vp]
         L.lbx_polymat_midproduct_naive_u32.argtypes = [vp, u32, PM, PM, PM, sz, sz, vp]

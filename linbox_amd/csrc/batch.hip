@@ -411,6 +411,7 @@ extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, s
     };
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     std::vector<lbx_job> bj;
     std::vector<unsigned> Ls;
     std::vector<size_t> rest;
@@ -439,7 +440,7 @@ extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, s
         lbx_polmat c{(void *)j.c, LBX_POLFIRST, j.m, j.n, j.sa + j.sb - 1, j.sa + j.sb - 1, 0};
         const size_t deg = j.sa + j.sb - 2;
         const unsigned L = log2_len(deg);
-        const bool single = j.sa + j.sb > 2 && j.p < (1u << 31) && L <= (unsigned)kMaxLdsLog &&
+        const bool single = j.sa + j.sb > 2 && j.p < (1u << 31) && L <= (unsigned)kMaxLog &&
                             (j.p - 1) % ((uint32_t)1 << L) == 0;
         if (single) {
             LBX_TRY(fft_prime_product(ctx, j.p, L, devview(&a, (void *)j.a), devview(&b, (void *)j.b),
@@ -449,7 +450,7 @@ extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, s
                                      j.m, j.k, j.n, s));
         } else {
             ctx->mu.unlock();  // the multiprecision entry point takes the lock itself
-            lbx_status st = lbx_polymat_mul_mp(ctx, &j.p, 1, &c, &a, &b, method, stream);
+            lbx_status st = lbx_polymat_mul_mp(ctx, &j.p, 1, &c, &a, &b, 0, method, stream);
             ctx->mu.lock();
             LBX_TRY(st);
         }

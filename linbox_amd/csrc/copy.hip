@@ -181,6 +181,7 @@ extern "C" lbx_status lbx_polymat_copy_u32(lbx_ctx *ctx, lbx_polmat *dst, const 
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     Staged ss, sd;
     lbx_status st = stage_in(src, 4, true, &ss);
     if (st == LBX_OK) st = stage_in(dst, 4, true, &sd);  // the rest of dst's storage is kept

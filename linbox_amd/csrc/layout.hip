@@ -27,6 +27,39 @@ __global__ void __launch_bounds__(256) scatter_kernel(const uint32_t *in, size_t
     }
 }
 
+__global__ void __launch_bounds__(256) fill_kernel(DevView out, size_t entries, size_t t0, size_t count, uint32_t value)
+{
+    const size_t total = entries * count;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const size_t e = i / count, t = t0 + (i - e * count);
+        out.base[view_index(out, e, t)] = value;
+    }
+}
+
+// pow_w table of init_powers (fft-integral.inl:1614-1650) built on the
+// device: entry idx in [N-2h, N-h) is w^((idx-(N-2h)) N/(2h)), with its Shoup
+// constant, the same words the host-built cached table holds.
+__global__ void __launch_bounds__(256) build_twiddles_kernel(uint2 *tw, uint32_t p, unsigned L, uint32_t w)
+{
+    const size_t n = (size_t)1 << L;
+    const size_t count = n > 1 ? n - 1 : 1;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < count; idx += (size_t)gridDim.x * 256) {
+        uint64_t e = 0;
+        if (n > 1) {
+            const size_t r = n - idx - 1;  // in [h, 2h)
+            const size_t h = (size_t)1 << (63 - __builtin_clzll((unsigned long long)r));
+            e = (uint64_t)(idx - (n - 2 * h)) * (n / (2 * h));
+        }
+        uint64_t acc = 1 % p, base = w % p;
+        while (e) {
+            if (e & 1) acc = acc * base % p;
+            base = base * base % p;
+            e >>= 1;
+        }
+        tw[idx] = make_uint2((uint32_t)acc, (uint32_t)((acc << 32) / p));
+    }
+}
+
 unsigned grid_for(size_t total)
 {
     size_t g = (total + 255) / 256;
@@ -45,6 +78,20 @@ hipError_t launch_scatter(const uint32_t *in, size_t stride, size_t entries, siz
                           hipStream_t s)
 {
     hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(entries * count)), dim3(256), 0, s, in, stride, entries, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(const DevView &out, size_t entries, size_t t0, size_t count, uint32_t value, hipStream_t s)
+{
+    if (!entries || !count) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(entries * count)), dim3(256), 0, s, out, entries, t0, count, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_twiddles(uint2 *tw, uint32_t p, unsigned L, uint32_t w, hipStream_t s)
+{
+    const size_t n = (size_t)1 << L;
+    hipLaunchKernelGGL(build_twiddles_kernel, dim3(grid_for(n > 1 ? n - 1 : 1)), dim3(256), 0, s, tw, p, L, w);
     return hipGetLastError();
 }
 

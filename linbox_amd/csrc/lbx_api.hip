@@ -154,13 +154,13 @@ DevView devview(const lbx_polmat *v, void *dev_base)
 // stream between stages and resolved later, so timing adds no host sync.
 class Stages {
 public:
-    Stages(lbx_ctx *ctx, hipStream_t s) : ctx_(ctx), s_(s)
+    Stages(lbx_ctx *ctx, hipStream_t s, bool enabled) : ctx_(ctx), s_(s), on_(enabled && ctx->profiling)
     {
-        if (ctx_->profiling) last_ = record();
+        if (on_) last_ = record();
     }
     void mark(const char *label_of_finished_stage)
     {
-        if (!ctx_->profiling) return;
+        if (!on_) return;
         hipEvent_t ev = record();
         ctx_->prof_pending.emplace_back(label_of_finished_stage, last_, ev);
         last_ = ev;
@@ -176,6 +176,7 @@ private:
     }
     lbx_ctx *ctx_;
     hipStream_t s_;
+    bool on_;
     hipEvent_t last_ = nullptr;
 };
 
@@ -183,9 +184,10 @@ private:
 // is queued on, so concurrent launches on other streams do not blur it.
 class KernelTimer {
 public:
-    KernelTimer(lbx_ctx *ctx, hipStream_t s, const char *label) : ctx_(ctx), s_(s), label_(label)
+    KernelTimer(lbx_ctx *ctx, hipStream_t s, const char *label, bool enabled) : ctx_(ctx), s_(s), label_(label)
     {
-        if (ctx_->profiling >= 2 && hipEventCreate(&start_) == hipSuccess) (void)hipEventRecord(start_, s_);
+        if (enabled && ctx_->profiling >= 2 && hipEventCreate(&start_) == hipSuccess)
+            (void)hipEventRecord(start_, s_);
     }
     ~KernelTimer()
     {
@@ -309,62 +311,90 @@ struct Pipe {
 
 // Forward transforms, pointwise products and inverse transform of one
 // single-prime product on device views.  deg+1 coefficients of c are written.
+// Every device buffer is taken from the workspace before the first launch.
 lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a_in, const DevView &b_in,
                              const DevView &c, size_t m, size_t k, size_t n, size_t count, int method,
-                             hipStream_t s, const MidSpec *mid)
+                             hipStream_t s, const MidSpec *mid, Workspace *ws_in)
 {
+    CtxWorkspace ctx_ws(ctx);
+    Workspace &ws = ws_in ? *ws_in : ctx_ws;
     const size_t N = (size_t)1 << L;
-    if (L > (unsigned)kMaxLdsLog) return LBX_ERR_UNSUPPORTED;
+    if (L > (unsigned)kMaxLog) return LBX_ERR_UNSUPPORTED;
     uint32_t w;
     LBX_TRY(primitive_root(p, L, &w));
     const uint32_t wi = invroot(p, L, w);
     const uint2 *twf, *twi;
-    LBX_TRY(twiddles(ctx, p, L, w, &twf));
-    LBX_TRY(twiddles(ctx, p, L, wi, &twi));
+    LBX_TRY(ws.twiddle_table(p, L, w, &twf, s));
+    LBX_TRY(ws.twiddle_table(p, L, wi, &twi, s));
     // product: both operands forward with w, inverse with w^-1; midproduct
     // (wordsize-fast.inl:277-313): small operand with w, other with w^-1,
     // inverse with w, small operand reversed over its first hdeg coefficients
     DevView a = a_in, b = b_in;
     const uint2 *twa = twf, *twb = twf, *twc = twi;
+    void *rev = nullptr;
     if (mid) {
         twc = twf;
         (mid->small_left ? twb : twa) = twi;
+        const size_t se = mid->small_left ? m * k : k * n;
+        LBX_TRY(ws.get("mid_rev", se * mid->hdeg * 4, &rev));
+    }
+    const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
+    method = choose_method(method, m, k, n, L);
+    const bool split = method == LBX_METHOD_INT8_MFMA && L >= 8 && L <= (unsigned)kMaxLdsLog;
+
+    // ---- buffers ----
+    void *al = nullptr, *bl = nullptr, *cmat = nullptr, *scr = nullptr, *ahat = nullptr, *bhat = nullptr,
+         *chat = nullptr;
+    // split-transform chunking: entries go in chunks of chunk_bytes() of head -> tail
+    // intermediate; with several chunks, heads run on the caller's stream and tails
+    // on a second one, two chunks in flight
+    const size_t mpad = (m + 31) & ~(size_t)31, npad = (n + 31) & ~(size_t)31;
+    size_t rpc = 0, cchunk = 0, slot_words = 0, nslots = 1;
+    bool fwd_pipe = false, inv_pipe = false;
+    if (split) {
+        const size_t target = std::max<size_t>(64, chunk_bytes() / (N * 4));  // entries per chunk
+        rpc = std::max<size_t>(1, target / k);  // limb rows per chunk
+        const size_t slot_entries = rpc * k;
+        cchunk = std::max(slot_entries, target);  // product entries per inverse chunk
+        if (!ws.may_pipeline()) {  // one chunk each way, so one slot and no second stream
+            rpc = std::max(mpad, npad);
+            cchunk = m * n;
+        }
+        fwd_pipe = (mpad + rpc - 1) / rpc > 1 || (npad + rpc - 1) / rpc > 1;
+        inv_pipe = m * n > cchunk;
+        // a slot holds the largest chunk either direction uses; one slot suffices
+        // when nothing is pipelined
+        slot_words = std::max(std::min(rpc, std::max(m, n)) * k, std::min(cchunk, m * n)) * N;
+        nslots = (fwd_pipe || inv_pipe) ? 2 : 1;
+        LBX_TRY(ws.get("alimb", limb_block_bytes(m, k) * N, &al));
+        LBX_TRY(ws.get("blimb", limb_block_bytes(n, k) * N, &bl));
+        LBX_TRY(ws.get("cmat", m * n * N * 4, &cmat));
+        LBX_TRY(ws.get("scratch", nslots * slot_words * 4, &scr));
+    } else {
+        LBX_TRY(ws.get("ahat", m * k * N * 4, &ahat));
+        LBX_TRY(ws.get("bhat", k * n * N * 4, &bhat));
+        LBX_TRY(ws.get("chat", m * n * N * 4, &chat));
+        if (method == LBX_METHOD_INT8_MFMA) {
+            LBX_TRY(ws.get("alimb", limb_block_bytes(m, k) * N, &al));
+            LBX_TRY(ws.get("blimb", limb_block_bytes(n, k) * N, &bl));
+            LBX_TRY(ws.get("cmat", m * n * N * 4, &cmat));
+        }
+    }
+
+    // ---- launches ----
+    if (mid) {
         const DevView &sm = mid->small_left ? a_in : b_in;
         const size_t se = mid->small_left ? m * k : k * n;
-        void *rev;
-        LBX_TRY(ensure(ctx, "mid_rev", se * mid->hdeg * 4, &rev));
         LBX_TRY_HIP(launch_reverse_gather(sm, se, mid->hdeg, (uint32_t *)rev, s));
         const DevView rv{(uint32_t *)rev, LBX_POLFIRST, sm.rows, sm.cols, mid->hdeg, 0, mid->hdeg};
         (mid->small_left ? a : b) = rv;
     }
-    const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
-    method = choose_method(method, m, k, n, L);
-
-    Stages st(ctx, s);
-    if (method == LBX_METHOD_INT8_MFMA && L >= 8) {
-        // Split transforms: the last six forward stages write per-point limb
-        // blocks and the first six inverse stages read the point-major
-        // product.  Entries go in chunks of chunk_bytes() of head -> tail
-        // intermediate; with several chunks, heads run on the caller's stream
-        // and tails on a second one, two chunks in flight.
+    const bool timing = ws.timing();
+    Stages st(ctx, s, timing);
+    if (split) {
+        // split transforms: the last six forward stages write per-point limb
+        // blocks and the first six inverse stages read the point-major product
         const unsigned BL = kSplitBL;
-        void *al, *bl, *cmat;
-        LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
-        LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
-        LBX_TRY(ensure(ctx, "cmat", m * n * N * 4, &cmat));
-        const size_t target = std::max<size_t>(64, chunk_bytes() / (N * 4));  // entries per chunk
-        const size_t mpad = (m + 31) & ~(size_t)31, npad = (n + 31) & ~(size_t)31;
-        const size_t rpc = std::max<size_t>(1, target / k);  // limb rows per chunk
-        const size_t slot_entries = rpc * k;
-        const size_t cchunk = std::max(slot_entries, target);  // product entries per inverse chunk
-        const bool fwd_pipe = (mpad + rpc - 1) / rpc > 1 || (npad + rpc - 1) / rpc > 1;
-        const bool inv_pipe = m * n > cchunk;
-        // a slot holds the largest chunk either direction uses; one slot suffices
-        // when nothing is pipelined
-        const size_t slot_words = std::max(std::min(rpc, std::max(m, n)) * k, std::min(cchunk, m * n)) * N;
-        const size_t nslots = (fwd_pipe || inv_pipe) ? 2 : 1;
-        void *scr;
-        LBX_TRY(ensure(ctx, "scratch", nslots * slot_words * 4, &scr));
         uint32_t *slot[2] = {(uint32_t *)scr, (uint32_t *)scr + (nslots - 1) * slot_words};
         Pipe fp{ctx, s, fwd_pipe};
         LBX_TRY(fp.begin());
@@ -379,12 +409,13 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
                 LBX_TRY(fp.reuse(chunk));
                 if (valid) {
                     const EntryMap map = op == 0 ? EntryMap{r0 * k, valid * k, 0} : EntryMap{r0, valid, n};
-                    KernelTimer kt(ctx, s, op == 0 ? "kernel dif_head a" : "kernel dif_head b");
+                    KernelTimer kt(ctx, s, op == 0 ? "kernel dif_head a" : "kernel dif_head b", timing);
                     LBX_TRY_HIP(launch_ntt_forward_head(op == 0 ? a : b, valid * k, map, L, BL, op == 0 ? twa : twb, p, buf, s));
                 }
                 LBX_TRY(fp.handoff(chunk));
                 {
-                    KernelTimer kt(ctx, fp.consumer(), op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b");
+                    KernelTimer kt(ctx, fp.consumer(), op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b",
+                                   timing);
                     LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
                                                       op == 0, L, BL, op == 0 ? twa : twb, p, fp.consumer()));
                 }
@@ -394,7 +425,7 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         LBX_TRY(fp.end(chunk));
         st.mark("direct FFT_DIF");
         {
-            KernelTimer kt(ctx, s, "kernel pw_mfma");
+            KernelTimer kt(ctx, s, "kernel pw_mfma", timing);
             LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N,
                                               p, inv_n, s));
         }
@@ -407,12 +438,12 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
             uint32_t *buf = slot[chunk & 1];
             LBX_TRY(ip.reuse(chunk));
             {
-                KernelTimer kt(ctx, s, "kernel dit_head");
+                KernelTimer kt(ctx, s, "kernel dit_head", timing);
                 LBX_TRY_HIP(launch_dit_head((const uint32_t *)cmat, m * n, e0, ne, L, BL, twc, p, buf, s));
             }
             LBX_TRY(ip.handoff(chunk));
             {
-                KernelTimer kt(ctx, ip.consumer(), "kernel dit_tail");
+                KernelTimer kt(ctx, ip.consumer(), "kernel dit_tail", timing);
                 LBX_TRY_HIP(launch_ntt_inverse_tail(buf, ne, L, BL, twc, p, c, e0, count, 1, ip.consumer()));
             }
             LBX_TRY(ip.done(chunk));
@@ -421,29 +452,19 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         st.mark("inverse FFT_DIT");
         return LBX_OK;
     }
-    void *ahat, *bhat, *chat;
-    LBX_TRY(ensure(ctx, "ahat", m * k * N * 4, &ahat));
-    LBX_TRY(ensure(ctx, "bhat", k * n * N * 4, &bhat));
-    LBX_TRY(ensure(ctx, "chat", m * n * N * 4, &chat));
     {
-        KernelTimer kt(ctx, s, "kernel dif a");
+        KernelTimer kt(ctx, s, "kernel dif a", timing);
         LBX_TRY_HIP(launch_ntt_forward(a, m * k, L, twa, p, (uint32_t *)ahat, s));
     }
     {
-        KernelTimer kt(ctx, s, "kernel dif b");
+        KernelTimer kt(ctx, s, "kernel dif b", timing);
         LBX_TRY_HIP(launch_ntt_forward(b, k * n, L, twb, p, (uint32_t *)bhat, s));
     }
     st.mark("direct FFT_DIF");
     if (method == LBX_METHOD_INT8_MFMA) {
-        void *al, *bl;
-        LBX_TRY(ensure(ctx, "alimb", limb_block_bytes(m, k) * N, &al));
-        LBX_TRY(ensure(ctx, "blimb", limb_block_bytes(n, k) * N, &bl));
         LBX_TRY_HIP(launch_to_limbs((const uint32_t *)ahat, (uint8_t *)al, m, k, true, N, p, s));
         LBX_TRY_HIP(launch_to_limbs((const uint32_t *)bhat, (uint8_t *)bl, n, k, false, N, p, s));
         st.mark("Polfirst to Matfirst");
-        // the point-major result reuses the forward buffer of a
-        void *cmat;
-        LBX_TRY(ensure(ctx, "cmat", m * n * N * 4, &cmat));
         LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N, p,
                                           inv_n, s));
         st.mark("Pointwise mult");
@@ -451,15 +472,15 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
         st.mark("Matfirst to Polfirst");
     } else {
         {
-            KernelTimer kt(ctx, s, "kernel pw_plain");
+            KernelTimer kt(ctx, s, "kernel pw_plain", timing);
             LBX_TRY_HIP(launch_pointwise_plain((const uint32_t *)ahat, (const uint32_t *)bhat, (uint32_t *)chat, m, k,
                                                n, N, p, inv_n, s));
         }
         st.mark("Pointwise mult");
     }
     {
-        KernelTimer kt(ctx, s, "kernel dit");
-        LBX_TRY_HIP(launch_ntt_inverse((const uint32_t *)chat, m * n, L, twc, p, c, count, 1, s));
+        KernelTimer kt(ctx, s, "kernel dit", timing);
+        LBX_TRY_HIP(launch_ntt_inverse((uint32_t *)chat, m * n, L, twc, p, c, count, 1, s));
     }
     st.mark("inverse FFT_DIT");
     return LBX_OK;
@@ -467,17 +488,38 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
 
 // Constant matrices (a.size()+b.size() <= 2): one pointwise product.
 lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c,
-                                   size_t m, size_t k, size_t n, hipStream_t s)
+                            size_t m, size_t k, size_t n, hipStream_t s, Workspace *ws_in)
 {
+    CtxWorkspace ctx_ws(ctx);
+    Workspace &ws = ws_in ? *ws_in : ctx_ws;
     void *ahat, *bhat, *chat;
-    LBX_TRY(ensure(ctx, "ahat", m * k * 4, &ahat));
-    LBX_TRY(ensure(ctx, "bhat", k * n * 4, &bhat));
-    LBX_TRY(ensure(ctx, "chat", m * n * 4, &chat));
+    LBX_TRY(ws.get("ahat", m * k * 4, &ahat));
+    LBX_TRY(ws.get("bhat", k * n * 4, &bhat));
+    LBX_TRY(ws.get("chat", m * n * 4, &chat));
     LBX_TRY_HIP(launch_gather(a, m * k, 1, (uint32_t *)ahat, 1, s));
     LBX_TRY_HIP(launch_gather(b, k * n, 1, (uint32_t *)bhat, 1, s));
     LBX_TRY_HIP(launch_pointwise_plain((const uint32_t *)ahat, (const uint32_t *)bhat, (uint32_t *)chat, m, k, n, 1,
                                        p, 1, s));
     LBX_TRY_HIP(launch_scatter((const uint32_t *)chat, 1, m * n, 1, c, s));
+    return LBX_OK;
+}
+
+lbx_status ArenaWorkspace::get(const char *, size_t bytes, void **out)
+{
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > cap_ - used_) return LBX_ERR_WORKSPACE_TOO_SMALL;
+    *out = base_ + used_;
+    used_ += need ? need : 256;
+    return LBX_OK;
+}
+
+lbx_status ArenaWorkspace::twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t s)
+{
+    const size_t n = (size_t)1 << L;
+    void *d;
+    LBX_TRY(get("twiddles", sizeof(uint2) * (n > 1 ? n - 1 : 1), &d));
+    LBX_TRY_HIP(launch_build_twiddles((uint2 *)d, p, L, w, s));
+    *out = (const uint2 *)d;
     return LBX_OK;
 }
 
@@ -544,6 +586,7 @@ lbx_status lbx_destroy(lbx_ctx *ctx)
             if (kv.second.ptr) (void)hipFree(kv.second.ptr);
         for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
         for (auto ev : ctx->events) (void)hipEventDestroy(ev);
+        if (ctx->last_done) (void)hipEventDestroy(ctx->last_done);
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
         mp_release(ctx);
         batch_release(ctx);
@@ -568,6 +611,7 @@ const char *lbx_strerror(lbx_status st)
     case LBX_ERR_OUT_OF_MEMORY: return "device out of memory";
     case LBX_ERR_UNSUPPORTED: return "unsupported size";
     case LBX_ERR_NO_DEVICE: return "no gfx950 device";
+    case LBX_ERR_WORKSPACE_TOO_SMALL: return "the product needs more device memory than the fixed workspace holds";
     }
     return "unknown status";
 }
@@ -651,14 +695,14 @@ lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lb
     const bool constant = sa + sb <= 2 && small_p;
     const bool single_prime = !constant && small_p && sa + sb > 2 && (p - 1) % pts == 0;
     if (!constant && !single_prime) {
-        if (max_rowdeg && max_rowdeg != sa + sb - 2) return LBX_ERR_UNSUPPORTED;
-        lbx_status st = lbx_polymat_mul_mp(ctx, &p, 1, c, a, b, method, stream);
+        lbx_status st = lbx_polymat_mul_mp(ctx, &p, 1, c, a, b, max_rowdeg, method, stream);
         return st;
     }
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
     lbx_status st = stage_in(a, 4, true, &sa_);
     if (st == LBX_OK) st = stage_in(b, 4, true, &sb_);
@@ -666,9 +710,12 @@ lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lb
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK) {
         const DevView da = devview(a, sa_.dev), db = devview(b, sb_.dev), dc = devview(c, sc_.dev);
-        if (constant)
+        if (constant) {
+            // the FFT domain's c.resize(deg+1) with deg = max_rowdeg: the
+            // product of constants is coefficient 0, coefficients 1..deg are zero
             st = constant_product(ctx, p, da, db, dc, m, k, n, s);
-        else
+            if (st == LBX_OK && deg > 0) st = from_hip(launch_fill(dc, m * n, 1, deg, 0u, s));
+        } else
             st = fft_prime_product(ctx, p, L, da, db, dc, m, k, n, deg + 1, method, s);
     }
     if (st == LBX_OK && sc_.owned) {
@@ -701,6 +748,7 @@ lbx_status lbx_polymat_mul_fft_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, size
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
     lbx_status st = stage_in(&va, 4, true, &sa_);
     if (st == LBX_OK) st = stage_in(&vb, 4, true, &sb_);
@@ -732,20 +780,21 @@ static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t ro
     } else if (!is_primitive_root(p, w, lpts)) {
         return LBX_ERR_NOT_PRIMITIVE_ROOT;
     }
-    if (lpts > (unsigned)kMaxLdsLog) return LBX_ERR_UNSUPPORTED;
+    if (lpts > (unsigned)kMaxLog) return LBX_ERR_UNSUPPORTED;
     if (root_used) *root_used = w;
     if (!count) return LBX_OK;
     const size_t N = (size_t)1 << lpts;
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     const uint2 *tw;
     LBX_TRY(twiddles(ctx, p, lpts, w, &tw));
     lbx_polmat v{(void *)data, LBX_POLFIRST, count, 1, N, N, 0};
     Staged sd;
     LBX_TRY(stage_in(&v, 4, true, &sd));
     const DevView dv = devview(&v, sd.dev);
-    hipError_t e = inverse ? launch_ntt_inverse((const uint32_t *)sd.dev, count, lpts, tw, p, dv, N, 1, s)
+    hipError_t e = inverse ? launch_ntt_inverse((uint32_t *)sd.dev, count, lpts, tw, p, dv, N, 1, s)
                            : launch_ntt_forward(dv, count, lpts, tw, p, (uint32_t *)sd.dev, s);
     lbx_status st = from_hip(e);
     if (st == LBX_OK && sd.owned) {

@@ -34,6 +34,10 @@ struct lbx_ctx {
     int tile_tuned[3] = {-1, -1, -1};  // batch pointwise tile per size class, -1 = not tuned yet
     hipStream_t aux = nullptr;         // second stream for chunk pipelining (default priority)
     std::vector<hipEvent_t> events;    // pool of dependency events
+    // stream order of the shared workspace: the stream of the last call and
+    // an event recorded at its end (see StreamOrder)
+    hipStream_t last_stream = nullptr;
+    hipEvent_t last_done = nullptr;
     int profiling = 0;  // 0 off, 1 stages, 2 stages and kernels
     // recorded (label, start, end) event triples, resolved by lbx_profile_get
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;
@@ -70,6 +74,49 @@ private:
     int prev_ = 0;
 };
 
+// Every call on a context uses the context's one set of workspace buffers,
+// batch plans and events.  A call on a stream other than the previous call's
+// first makes its stream wait for the previous call's work (an event recorded
+// on that stream at the end of the call), so two streams never use the
+// workspace at the same time.  Construct after taking ctx->mu and selecting the
+// device; the destructor records the event.  Not used by calls on a stream
+// under graph capture (a capture cannot wait for work outside its graph).
+class StreamOrder {
+public:
+    StreamOrder(lbx_ctx *ctx, hipStream_t s) : ctx_(ctx), s_(s)
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+            (void)hipGetLastError();
+            cs = hipStreamCaptureStatusNone;
+        }
+        active_ = cs == hipStreamCaptureStatusNone;
+        if (active_ && ctx->last_done && ctx->last_stream != s) (void)hipStreamWaitEvent(s, ctx->last_done, 0);
+    }
+    ~StreamOrder()
+    {
+        if (!active_) return;
+        if (!ctx_->last_done && hipEventCreateWithFlags(&ctx_->last_done, hipEventDisableTiming) != hipSuccess) {
+            ctx_->last_done = nullptr;
+            return;
+        }
+        if (hipEventRecord(ctx_->last_done, s_) == hipSuccess) ctx_->last_stream = s_;
+    }
+    StreamOrder(const StreamOrder &) = delete;
+    StreamOrder &operator=(const StreamOrder &) = delete;
+
+private:
+    lbx_ctx *ctx_;
+    hipStream_t s_;
+    bool active_ = false;
+};
+
+// Root of unity of order 2^k (fft-utils.h:83-104), its inverse (fft.h:119-130)
+// and the cached device twiddle table of a root (fft-integral.inl:1614-1650).
+lbx_status primitive_root(uint32_t p, unsigned k, uint32_t *w_out);
+uint32_t invroot(uint32_t p, unsigned L, uint32_t w);
+lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint2 **out);
+
 // Named device workspace, grown on demand and kept between calls.
 lbx_status ensure(lbx_ctx *ctx, const char *name, size_t bytes, void **out);
 bool is_device_ptr(const void *p);
@@ -87,6 +134,52 @@ struct Staged {
 lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out);
 void release(Staged &s);
 
+// Where one call's device buffers come from.  The context's workspace keeps
+// named buffers between calls (ensure) and caches twiddle tables; an arena
+// hands out consecutive pieces of memory the caller owns and builds the
+// twiddle tables in it on the device, so a call through it allocates nothing.
+// Every buffer a product needs is requested before its first launch, so an
+// arena that is too small fails before any work is queued.
+class Workspace {
+public:
+    virtual ~Workspace() = default;
+    virtual lbx_status get(const char *name, size_t bytes, void **out) = 0;
+    virtual lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t s) = 0;
+    // chunked head/tail pipelining on a second stream (and its events)
+    virtual bool may_pipeline() const = 0;
+    // stage and kernel timing events (lbx_profile_enable)
+    virtual bool timing() const = 0;
+};
+
+class CtxWorkspace final : public Workspace {
+public:
+    explicit CtxWorkspace(lbx_ctx *ctx) : ctx_(ctx) {}
+    lbx_status get(const char *name, size_t bytes, void **out) override { return ensure(ctx_, name, bytes, out); }
+    lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t) override
+    {
+        return twiddles(ctx_, p, L, w, out);
+    }
+    bool may_pipeline() const override { return true; }
+    bool timing() const override { return ctx_->profiling != 0; }
+
+private:
+    lbx_ctx *ctx_;
+};
+
+class ArenaWorkspace final : public Workspace {
+public:
+    ArenaWorkspace(void *base, size_t cap) : base_((uint8_t *)base), cap_(cap) {}
+    lbx_status get(const char *, size_t bytes, void **out) override;
+    lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t s) override;
+    bool may_pipeline() const override { return false; }
+    bool timing() const override { return false; }
+    size_t used() const { return used_; }
+
+private:
+    uint8_t *base_;
+    size_t cap_, used_ = 0;
+};
+
 // Midproduct variant of a transform product (matpoly-mult-fft-wordsize-fast.inl:209-316):
 // the first hdeg coefficients of the small operand (a if small_left) are
 // reversed, that operand is transformed with the root w and the other with
@@ -98,18 +191,22 @@ struct MidSpec {
 
 // Single-prime product on device views (count = deg+1 coefficients of c), or
 // with mid != nullptr the midproduct (count = c.size coefficients).
+// Buffers come from `ws` (the context's workspace when null).
 lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView &a, const DevView &b,
                              const DevView &c, size_t m, size_t k, size_t n, size_t count, int method,
-                             hipStream_t s, const MidSpec *mid = nullptr);
+                             hipStream_t s, const MidSpec *mid = nullptr, Workspace *ws = nullptr);
 // Product (mid == nullptr; c holds sa+sb-1 coefficients) or midproduct with
 // 2^L points over Z/PZ (P of nl little-endian limbs) through FFT primes and
 // CRT (matpoly-mult-fft-multiprecision.inl:141-277, 427-520).
+// A product writes `count` coefficients of c (0: sa+sb-1; fewer: the cyclic
+// product mod x^pts - 1 with pts the least power of two >= count, truncated,
+// as for max_rowdeg).
 lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a, const DevView &b, const DevView &c,
                       size_t m, size_t k, size_t n, int method, hipStream_t s, const MidSpec *mid = nullptr,
-                      unsigned mid_L = 0);
+                      unsigned mid_L = 0, size_t count = 0);
 // Product of constant matrices (one coefficient each).
 lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c, size_t m,
-                            size_t k, size_t n, hipStream_t s);
+                            size_t k, size_t n, hipStream_t s, Workspace *ws = nullptr);
 // Frees the multiprecision plans cached in a context.
 void mp_release(lbx_ctx *ctx);
 // Frees the batch plans (graphs, descriptors) cached in a context.
@@ -121,10 +218,5 @@ lbx_status batch_products(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, hipSt
 // The pointwise form LBX_METHOD_AUTO resolves to for m x k x n at 2^L points.
 int choose_method(int method, size_t m, size_t k, size_t n, unsigned L);
 
-// Root of unity of order 2^k (fft-utils.h:83-104), its inverse (fft.h:119-130)
-// and the cached device twiddle table of a root (fft-integral.inl:1614-1650).
-lbx_status primitive_root(uint32_t p, unsigned k, uint32_t *w_out);
-uint32_t invroot(uint32_t p, unsigned L, uint32_t w);
-lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint2 **out);
 
 }  // namespace lbx

@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "modarith.h"
+#include "pw_math.h"
 
 namespace lbx {
 
@@ -32,6 +33,7 @@ __host__ __device__ __forceinline__ size_t view_index(const DevView &v, size_t e
 }
 
 constexpr int kMaxLdsLog = 15;  // LDS-resident transforms up to 2^15 points
+constexpr int kMaxLog = 27;     // longest transform: 2^27 points (the 2-adic valuation of 2013265921 - 1)
 
 // Twiddle table of one root w of order 2^L: the reference's pow_w layout
 // (fft-integral.inl:64-77): subarray for butterfly span h at offset N-2h,
@@ -41,13 +43,17 @@ constexpr int kMaxLdsLog = 15;  // LDS-resident transforms up to 2^15 points
 // ---- NTT launchers (ntt.hip) ----
 // Forward DIF (natural in, bit-reversed out) of `entries` polynomials read
 // from `in` (first in.size coefficients, zero-padded to 2^L), written
-// contiguously to out[e*2^L ...].
+// contiguously to out[e*2^L ...].  Lengths 2^1..2^kMaxLdsLog stay in LDS
+// (one HBM read and write); longer ones, up to 2^kMaxLog, add HBM passes of
+// up to four stages each.  `in` and `out` may be the same polfirst storage.
 hipError_t launch_ntt_forward(const DevView &in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
                               uint32_t *out, hipStream_t s);
 // Inverse DIT (bit-reversed in, natural out) of `entries` contiguous
 // polynomials in[e*2^L ...]; coefficients 0..count-1 of entry e are written
 // to the view `out` after multiplication by `scale` (scale==1: none).
-hipError_t launch_ntt_inverse(const uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
+// Transforms longer than 2^kMaxLdsLog run in several passes and overwrite
+// `in`; shorter ones leave it intact.
+hipError_t launch_ntt_inverse(uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
                               const DevView &out, size_t count, uint32_t scale, hipStream_t s);
 
 // Output row l of a forward transform reads input entry
@@ -109,17 +115,6 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
 hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries, size_t npts, hipStream_t s);
 
 // ---- batch scheduler (batch.hip, with kernels in ntt.hip / pointwise.hip) ----
-// Constants of the plain pointwise epilogue: V = hi 2^64 + ah 2^32 + al,
-// V*scale mod p = hi*c2 + ah*c1 + al*c0 mod p.
-// Plain-form epilogue: acc (< 2^63) * scale mod p = hi(acc)*c1 + lo(acc)*c0 with
-// c0 = scale, c1 = 2^32 scale; accumulators are folded every `fold` products
-// with f62 = 2^62 mod p so that they never reach 2^64.
-struct PlainConst {
-    uint32_t c0, c0p, c1, c1p;
-    uint32_t f62;   // 2^62 mod p
-    uint32_t fold;  // products added between folds
-};
-PlainConst plain_const(uint32_t p, uint32_t scale);
 
 // One product of a batch as the device sees it: operands polfirst, per-job
 // slices of the batch workspace for the transforms.
@@ -162,6 +157,11 @@ hipError_t launch_gather(const DevView &in, size_t entries, size_t count, uint32
 // out[e*hdeg + i] = in(e, hdeg-1-i) (0 past the view's size), i < hdeg: the
 // reversal of the small midproduct operand (wordsize-fast.inl:228-235).
 hipError_t launch_reverse_gather(const DevView &in, size_t entries, size_t hdeg, uint32_t *out, hipStream_t s);
+// The pow_w twiddle table of root w of order 2^L (N-1 entries of {w^e, Shoup
+// constant}) computed on the device, identical to the host-built one.
+hipError_t launch_build_twiddles(uint2 *tw, uint32_t p, unsigned L, uint32_t w, hipStream_t s);
+// view(e, t) = value for t0 <= t < t0+count (a graph-safe fill, no memset node).
+hipError_t launch_fill(const DevView &out, size_t entries, size_t t0, size_t count, uint32_t value, hipStream_t s);
 // view(e, t) = in[e*stride + t] for t < count.
 hipError_t launch_scatter(const uint32_t *in, size_t stride, size_t entries, size_t count, const DevView &out,
                           hipStream_t s);

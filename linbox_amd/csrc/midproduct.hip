@@ -107,6 +107,7 @@ lbx_status with_views(lbx_ctx *ctx, lbx_polmat *c, const lbx_polmat *a, const lb
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
     lbx_status st = stage_in(a, 4, true, &sa_);
     if (st == LBX_OK) st = stage_in(b, 4, true, &sb_);
@@ -155,7 +156,7 @@ extern "C" lbx_status lbx_polymat_midproduct_u32(lbx_ctx *ctx, uint32_t p, lbx_p
     if (sh.deg < sh.hdeg || c->size < sh.deg - sh.hdeg || big >= sh.hdeg + sh.deg) return LBX_ERR_SIZE_MISMATCH;
     if (small > sh.hdeg || big > sh.pts || c->size > sh.pts || sh.hdeg > sh.pts) return LBX_ERR_SIZE_MISMATCH;
     if (sh.L == 0) return LBX_ERR_K_NOT_POSITIVE;  // FFT(F, 0): "k must be positive" (fft.h:136-139)
-    if (sh.L > (unsigned)kMaxLdsLog) return LBX_ERR_UNSUPPORTED;
+    if (sh.L > (unsigned)kMaxLog) return LBX_ERR_UNSUPPORTED;
     const MidSpec mid{small_left != 0, sh.hdeg};
     const bool single_prime = p < (1u << 31) && (p - 1) % sh.pts == 0;
     // c is copied in too: a view's storage outside the window must come back unchanged

@@ -616,9 +616,13 @@ void mp_release(lbx_ctx *ctx)
 }
 
 lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a, const DevView &b, const DevView &c,
-                      size_t m, size_t k, size_t n, int method, hipStream_t s, const MidSpec *mid, unsigned mid_L)
+                      size_t m, size_t k, size_t n, int method, hipStream_t s, const MidSpec *mid, unsigned mid_L,
+                      size_t count)
 {
-    const size_t sa = a.size, sb = b.size, sc = mid ? c.size : sa + sb - 1;
+    // product: s = max_rowdeg+1 coefficients when given, else sa+sb-1 (mul_crtla :150-152;
+    // three-primes.inl:69-73); with fewer than sa+sb-1 the cyclic product mod x^pts - 1
+    const size_t sa = a.size, sb = b.size, sc = mid ? c.size : (count ? count : sa + sb - 1);
+    const bool full = !mid && sc == sa + sb - 1;
     unsigned lpts = 0;
     size_t pts = 1;
     while (pts < sc) {  // mul_crtla :151-154
@@ -660,7 +664,7 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
                 DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
                 if (mid)
                     LBX_TRY(fft_prime_product(ctx, fp->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s, mid));
-                else if (sc == 1)
+                else if (sa + sb <= 2 && sc == 1)
                     LBX_TRY(constant_product(ctx, fp->primes[i], va, vb, vc, m, k, n, s));
                 else
                     LBX_TRY(fft_prime_product(ctx, fp->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
@@ -693,7 +697,7 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
     // many primes (multiword moduli): the products as one batch; a new shape
     // builds and caches a batch plan (graph capture), which pays off here
     lbx_status bst = LBX_ERR_UNSUPPORTED;
-    if (!mid && r >= kBatchMinPrimes)
+    if (full && r >= kBatchMinPrimes)
         bst = primes_as_batch(ctx, plan->primes, (const uint32_t *)ra, m * k * sa, (const uint32_t *)rb, k * n * sb,
                               (uint32_t *)rc, m, k, n, sa, sb, lpts, method, s);
     if (bst != LBX_ERR_UNSUPPORTED) LBX_TRY(bst);
@@ -703,7 +707,7 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
         DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
         if (mid)
             LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s, mid));
-        else if (sc == 1)
+        else if (sa + sb <= 2 && sc == 1)
             LBX_TRY(constant_product(ctx, plan->primes[i], va, vb, vc, m, k, n, s));
         else
             LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
@@ -727,7 +731,8 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
 using namespace lbx;
 
 extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nlimbs, lbx_polmat *c,
-                                         const lbx_polmat *a, const lbx_polmat *b, int method, void *stream)
+                                         const lbx_polmat *a, const lbx_polmat *b, size_t max_rowdeg, int method,
+                                         void *stream)
 {
     if (!ctx || !P || !a || !b || !c) return LBX_ERR_INVALID_ARG;
     if (nlimbs < 1 || nlimbs > kMaxLimbs) return LBX_ERR_PRIME_TOO_LARGE;
@@ -743,13 +748,18 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
     if (!nonzero || (!big && P[0] < 2)) return LBX_ERR_INVALID_ARG;
     if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
     if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
-    const size_t m = a->rows, k = a->cols, n = b->cols, sc = a->size + b->size - 1;
+    const size_t m = a->rows, k = a->cols, n = b->cols;
+    const size_t sc = max_rowdeg ? max_rowdeg + 1 : a->size + b->size - 1;
     if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
-    if (k > 0xffffffffu || a->size > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
+    size_t pts = 1;
+    while (pts < sc) pts <<= 1;
+    if (a->size > pts || b->size > pts) return LBX_ERR_SIZE_MISMATCH;  // the padded copies hold pts coefficients
+    if (k > 0xffffffffu || a->size > 0xffffffffu || sc > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     const size_t wb = 4 * (size_t)nlimbs;
     Staged sa_, sb_, sc_;
     lbx_status st = stage_in(a, wb, true, &sa_);
@@ -758,7 +768,7 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK)
         st = mp_product(ctx, P, nlimbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k, n,
-                        method, s);
+                        method, s, nullptr, 0, sc);
     if (st == LBX_OK && sc_.owned) {
         hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * wb, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);

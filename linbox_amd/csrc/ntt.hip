@@ -2,6 +2,7 @@
 // for the plain path and the NTT entry points, and the split head / tail
 // transforms around the point-major pointwise stage (see ntt_device.h).
 #include <algorithm>
+#include <vector>
 
 #include "ntt_device.h"
 
@@ -437,12 +438,257 @@ __global__ void __launch_bounds__(kBatchNttThreads, (batch_waves<L, INVERSE>()))
     }
 }
 
+// ---- transforms longer than the LDS-resident size (2^16 .. 2^kMaxLog) ------
+// After DIF stages [0, s) a transform of N = 2^L words is 2^s independent
+// blocks of N/2^s consecutive words, each needing a DIF of that length with
+// root w^(2^s), whose pow_w table is the tail of the big table from offset
+// N - N/2^s (the subarray of span h sits at N - 2h in both).  DIT stages
+// [0, d) likewise act inside blocks of 2^d words.  So a long forward transform
+// is a few HBM passes of RC <= 4 "column" stages (each thread holds 2^RC words
+// of one block, 2^RC apart in units of the block's 1/2^RC, and applies RC
+// radix-2 stages in registers) followed by the LDS-resident DIF of the blocks
+// of 2^kLongLds words; the inverse is the LDS-resident DIT of those blocks
+// followed by column passes for the remaining spans.  Consecutive threads take
+// consecutive words of a block, so every load and store instruction of a wave
+// covers 256 contiguous bytes, and so does every twiddle load.
+constexpr unsigned kLongLds = 14;
+
+// DIF stages [s0, s0+RC) of `entries` polynomials.  Input: the view `in`
+// (first in.size coefficients, zero beyond) when FROM_VIEW, else the
+// contiguous out[e*N ...] (in place).  Output: out[e*N ...].
+template <int RC, bool FROM_VIEW>
+__global__ void __launch_bounds__(256) dif_cols_kernel(DevView in, size_t entries, unsigned L, unsigned s0,
+                                                       const uint2 *__restrict__ tw, uint32_t p, uint32_t *out)
+{
+    constexpr int R = 1 << RC;
+    const size_t N = (size_t)1 << L;
+    const size_t B0 = N >> s0;    // block length at stage s0
+    const size_t J = B0 >> RC;    // distance between a thread's words
+    const size_t per_entry = N >> RC;
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= entries * per_entry) return;
+    const size_t j = g % J, rest = g / J;
+    const size_t e = rest % entries, blk = rest / entries;
+    const size_t base = blk * B0 + j;
+    uint32_t x[R];
+    if constexpr (FROM_VIEW) {
+        if (in.layout == 0) {
+            const uint32_t *src = in.base + e * in.stor + in.shift;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const size_t t = base + (size_t)i * J;
+                x[i] = t < in.size ? src[t] : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const size_t t = base + (size_t)i * J;
+                x[i] = t < in.size ? in.base[view_index(in, e, t)] : 0u;
+            }
+        }
+    } else {
+        const uint32_t *src = out + e * N;
+#pragma unroll
+        for (int i = 0; i < R; ++i) x[i] = src[base + (size_t)i * J];
+    }
+#pragma unroll
+    for (int t = 0; t < RC; ++t) {
+        const int half = 1 << (RC - 1 - t);
+        const unsigned st = s0 + t;
+        const uint2 *tws = tw + (N - (N >> st));
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            if (i & half) continue;
+            const uint32_t a = x[i], b = x[i + half];
+            if (st == L - 1) {  // span 1: the twiddle is w^0 = 1
+                x[i] = add_mod(a, b, p);
+                x[i + half] = sub_mod(a, b, p);
+            } else {
+                const uint2 w = tws[j + (size_t)(i & (half - 1)) * J];
+                x[i] = add_mod(a, b, p);
+                x[i + half] = shoup_mul(a + (p - b), w.x, w.y, p);
+            }
+        }
+    }
+    uint32_t *dst = out + e * N;
+#pragma unroll
+    for (int i = 0; i < R; ++i) dst[base + (size_t)i * J] = x[i];
+}
+
+// DIT stages [d0, d0+RC) (spans 2^d0 .. 2^(d0+RC-1)) of `entries` contiguous
+// polynomials buf[e*N ...], in place; with LAST the words are written to
+// the view `out` instead (coefficients t < count, times scale when SCALE).
+template <int RC, bool LAST, bool SCALE>
+__global__ void __launch_bounds__(256) dit_cols_kernel(uint32_t *buf, size_t entries, unsigned L, unsigned d0,
+                                                       const uint2 *__restrict__ tw, uint32_t p, DevView out,
+                                                       size_t count, uint32_t scale, uint32_t scale_p)
+{
+    constexpr int R = 1 << RC;
+    const size_t N = (size_t)1 << L;
+    const size_t J = (size_t)1 << d0;  // distance between a thread's words
+    const size_t per_entry = N >> RC;
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= entries * per_entry) return;
+    const size_t j = g % J, rest = g / J;
+    const size_t e = rest % entries, blk = rest / entries;
+    const size_t base = (blk << (d0 + RC)) + j;
+    uint32_t x[R];
+    uint32_t *src = buf + e * N;
+#pragma unroll
+    for (int i = 0; i < R; ++i) x[i] = src[base + (size_t)i * J];
+#pragma unroll
+    for (int t = 0; t < RC; ++t) {
+        const int half = 1 << t;
+        const unsigned d = d0 + t;
+        const uint2 *tws = tw + (N - ((size_t)2 << d));
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            if (i & half) continue;
+            const uint32_t a = x[i];
+            uint32_t b = x[i + half];
+            if (d != 0) {
+                const uint2 w = tws[j + (size_t)(i & (half - 1)) * J];
+                b = shoup_mul(b, w.x, w.y, p);
+            }
+            x[i] = add_mod(a, b, p);
+            x[i + half] = sub_mod(a, b, p);
+        }
+    }
+    if constexpr (LAST) {
+        if (out.layout == 0) {
+            uint32_t *o = out.base + e * out.stor + out.shift;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const size_t t = base + (size_t)i * J;
+                if (t < count) o[t] = SCALE ? shoup_mul(x[i], scale, scale_p, p) : x[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const size_t t = base + (size_t)i * J;
+                if (t < count) out.base[view_index(out, e, t)] = SCALE ? shoup_mul(x[i], scale, scale_p, p) : x[i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) src[base + (size_t)i * J] = x[i];
+    }
+}
+
+// Stage counts of the column passes covering `stages` stages: as even as
+// possible, at most four per pass.
+std::vector<unsigned> col_split(unsigned stages)
+{
+    std::vector<unsigned> r;
+    const unsigned npass = (stages + 3) / 4;
+    for (unsigned i = 0; i < npass; ++i) r.push_back(stages / npass + (i < stages % npass ? 1u : 0u));
+    return r;
+}
+
+template <int RC, bool FROM_VIEW>
+hipError_t dif_cols_launch(const DevView &in, size_t entries, unsigned L, unsigned s0, const uint2 *tw, uint32_t p,
+                           uint32_t *out, hipStream_t s)
+{
+    const size_t threads = entries * (((size_t)1 << L) >> RC);
+    const size_t grid = (threads + 255) / 256;
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((dif_cols_kernel<RC, FROM_VIEW>), dim3((unsigned)grid), dim3(256), 0, s, in, entries, L, s0,
+                       tw, p, out);
+    return hipGetLastError();
+}
+
+template <bool FROM_VIEW>
+hipError_t dif_cols(unsigned rc, const DevView &in, size_t entries, unsigned L, unsigned s0, const uint2 *tw,
+                    uint32_t p, uint32_t *out, hipStream_t s)
+{
+    switch (rc) {
+    case 1: return dif_cols_launch<1, FROM_VIEW>(in, entries, L, s0, tw, p, out, s);
+    case 2: return dif_cols_launch<2, FROM_VIEW>(in, entries, L, s0, tw, p, out, s);
+    case 3: return dif_cols_launch<3, FROM_VIEW>(in, entries, L, s0, tw, p, out, s);
+    case 4: return dif_cols_launch<4, FROM_VIEW>(in, entries, L, s0, tw, p, out, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int RC, bool LAST, bool SCALE>
+hipError_t dit_cols_launch(uint32_t *buf, size_t entries, unsigned L, unsigned d0, const uint2 *tw, uint32_t p,
+                           const DevView &out, size_t count, uint32_t scale, hipStream_t s)
+{
+    const size_t threads = entries * (((size_t)1 << L) >> RC);
+    const size_t grid = (threads + 255) / 256;
+    if (grid > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((dit_cols_kernel<RC, LAST, SCALE>), dim3((unsigned)grid), dim3(256), 0, s, buf, entries, L,
+                       d0, tw, p, out, count, scale, shoup_pre(scale, p));
+    return hipGetLastError();
+}
+
+template <bool LAST, bool SCALE>
+hipError_t dit_cols(unsigned rc, uint32_t *buf, size_t entries, unsigned L, unsigned d0, const uint2 *tw, uint32_t p,
+                    const DevView &out, size_t count, uint32_t scale, hipStream_t s)
+{
+    switch (rc) {
+    case 1: return dit_cols_launch<1, LAST, SCALE>(buf, entries, L, d0, tw, p, out, count, scale, s);
+    case 2: return dit_cols_launch<2, LAST, SCALE>(buf, entries, L, d0, tw, p, out, count, scale, s);
+    case 3: return dit_cols_launch<3, LAST, SCALE>(buf, entries, L, d0, tw, p, out, count, scale, s);
+    case 4: return dit_cols_launch<4, LAST, SCALE>(buf, entries, L, d0, tw, p, out, count, scale, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Forward DIF of length 2^L > 2^15: column passes over stages [0, L-14)
+// from the view into out, then the LDS-resident DIF of the 2^(L-14) blocks of
+// every entry, in place.
+hipError_t ntt_forward_long(const DevView &in, size_t entries, unsigned L, const uint2 *tw, uint32_t p, uint32_t *out,
+                            hipStream_t s)
+{
+    const size_t N = (size_t)1 << L, M = (size_t)1 << kLongLds;
+    unsigned s0 = 0;
+    for (unsigned rc : col_split(L - kLongLds)) {
+        hipError_t e = s0 == 0 ? dif_cols<true>(rc, in, entries, L, s0, tw, p, out, s)
+                               : dif_cols<false>(rc, in, entries, L, s0, tw, p, out, s);
+        if (e != hipSuccess) return e;
+        s0 += rc;
+    }
+    const size_t blocks = entries << (L - kLongLds);
+    const DevView bv{out, 0, blocks, 1, M, 0, M};
+    const EntryMap id{0, blocks, 0};
+    return fwd_launch<kLongLds, kLongLds>(bv, blocks, id, tw + (N - M), p, out, s);
+}
+
+// Inverse DIT of length 2^L > 2^15: the LDS-resident DIT of the 2^(L-14)
+// blocks of every entry in place on `in`, then column passes for spans
+// 2^14 .. 2^(L-1), the last writing the view.
+hipError_t ntt_inverse_long(uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p, const DevView &out,
+                            size_t count, uint32_t scale, hipStream_t s)
+{
+    const size_t N = (size_t)1 << L, M = (size_t)1 << kLongLds;
+    const size_t blocks = entries << (L - kLongLds);
+    const DevView bv{in, 0, blocks, 1, M, 0, M};
+    hipError_t e = inv_launch<kLongLds, 0>(in, blocks, tw + (N - M), p, bv, 0, M, 1, s);
+    if (e != hipSuccess) return e;
+    const std::vector<unsigned> split = col_split(L - kLongLds);
+    unsigned d0 = kLongLds;
+    for (size_t i = 0; i < split.size(); ++i) {
+        const bool last = i + 1 == split.size();
+        if (!last)
+            e = dit_cols<false, false>(split[i], in, entries, L, d0, tw, p, out, count, 1, s);
+        else if (scale != 1)
+            e = dit_cols<true, true>(split[i], in, entries, L, d0, tw, p, out, count, scale, s);
+        else
+            e = dit_cols<true, false>(split[i], in, entries, L, d0, tw, p, out, count, 1, s);
+        if (e != hipSuccess) return e;
+        d0 += split[i];
+    }
+    return hipSuccess;
+}
+
 }  // namespace
 
 hipError_t launch_ntt_forward(const DevView &in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
                               uint32_t *out, hipStream_t s)
 {
     if (entries == 0) return hipSuccess;
+    if (L > (unsigned)kMaxLdsLog && L <= (unsigned)kMaxLog) return ntt_forward_long(in, entries, L, tw, p, out, s);
     const EntryMap id{0, entries, 0};
     switch (L) {
 #define LBX_CASE(l) \
@@ -454,10 +700,12 @@ hipError_t launch_ntt_forward(const DevView &in, size_t entries, unsigned L, con
     }
 }
 
-hipError_t launch_ntt_inverse(const uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
+hipError_t launch_ntt_inverse(uint32_t *in, size_t entries, unsigned L, const uint2 *tw, uint32_t p,
                               const DevView &out, size_t count, uint32_t scale, hipStream_t s)
 {
     if (entries == 0) return hipSuccess;
+    if (L > (unsigned)kMaxLdsLog && L <= (unsigned)kMaxLog)
+        return ntt_inverse_long(in, entries, L, tw, p, out, count, scale, s);
     switch (L) {
 #define LBX_CASE(l) \
     case l: return inv_launch<l, 0>(in, entries, tw, p, out, 0, count, scale, s);

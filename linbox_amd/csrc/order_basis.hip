@@ -726,13 +726,18 @@ lbx_status order_basis(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_po
     if (serie->size < order || sigma->size < order + 1) return LBX_ERR_SIZE_MISMATCH;
     if (m > (size_t)kMbMaxM || n > (size_t)kMbMaxN || m * n > (size_t)kMbMaxOut * kMbThreads)
         return LBX_ERR_UNSUPPORTED;
-    if (order > ((size_t)1 << kMaxLdsLog) - 1 || order > 0x7fffffff) return LBX_ERR_UNSUPPORTED;
+    // PM_Basis: the middle products and products of the recursion need
+    // transforms of up to 2^ceil(log2(order+1)) points; M_Basis alone keeps
+    // its sigma history (order+1 coefficient matrices) on the device
+    const size_t max_order = divide ? ((size_t)1 << kMaxLog) - 1 : ((size_t)1 << kMaxLdsLog) - 1;
+    if (order > max_order) return LBX_ERR_UNSUPPORTED;
     for (size_t i = 0; i < m; ++i)
         if (shift[i] >= kShiftLimit) return LBX_ERR_UNSUPPORTED;
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     DeviceGuard guard(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
     const size_t base_order = divide ? std::min(order, kMbThreshold) : order;
     const size_t b_S = align256(m * n * base_order * 4), b_W = align256((base_order + 1) * m * m * 4);
     const size_t b_sh = align256(m * 8), b_deg = 256;

@@ -23,6 +23,7 @@
 //    transposes the NTT output into per-point blocks laid out for direct
 //    16-byte operand loads.
 #include "lbx_internal.h"
+#include "pw_math.h"
 
 namespace lbx {
 namespace {
@@ -35,9 +36,7 @@ __device__ __forceinline__ void pw_plain_body(const uint32_t *__restrict__ A, co
                                               uint32_t p, const PlainConst &pc, size_t t, int i0, int j0)
 {
     // acc += a*b is one v_mad_u64_u32; every pc.fold products the bits above
-    // 2^62 are folded back (acc = acc mod 2^62 + (acc >> 62) * (2^62 mod p)),
-    // which keeps acc < 2^62 + 2^33 between folds and below 2^64 inside them
-    constexpr uint64_t kLow62 = ((uint64_t)1 << 62) - 1;
+    // 2^62 are folded back (plain_fold), which keeps acc below 2^64
     uint64_t acc[TI][TJ];
 #pragma unroll
     for (int ii = 0; ii < TI; ++ii)
@@ -62,16 +61,14 @@ __device__ __forceinline__ void pw_plain_body(const uint32_t *__restrict__ A, co
         for (int ii = 0; ii < TI; ++ii)
 #pragma unroll
             for (int jj = 0; jj < TJ; ++jj)
-                acc[ii][jj] = (acc[ii][jj] & kLow62) + (uint64_t)(uint32_t)(acc[ii][jj] >> 62) * pc.f62;
+                acc[ii][jj] = plain_fold(acc[ii][jj], pc);
     }
 #pragma unroll
     for (int ii = 0; ii < TI; ++ii)
 #pragma unroll
         for (int jj = 0; jj < TJ; ++jj) {
             if (i0 + ii >= m || j0 + jj >= n) continue;
-            const uint32_t al = (uint32_t)acc[ii][jj], ah = (uint32_t)(acc[ii][jj] >> 32);
-            const uint32_t r = add_mod(shoup_mul(ah, pc.c1, pc.c1p, p), shoup_mul(al, pc.c0, pc.c0p, p), p);
-            C[((size_t)(i0 + ii) * n + j0 + jj) * npts + t] = r;
+            C[((size_t)(i0 + ii) * n + j0 + jj) * npts + t] = plain_finish(acc[ii][jj], p, pc);
         }
 }
 
@@ -164,23 +161,12 @@ __global__ void __launch_bounds__(256) to_limbs_kernel(const uint32_t *__restric
     }
 }
 
-// Constants of the MFMA epilogue.
-struct MfmaConst {
-    int32_t cd[7];    // 2^(8d) * scale mod p (< p < 2^31, so a signed 32-bit value)
-    uint64_t off;     // multiple of p >= 2^62
-    uint32_t r32, r32p;  // 2^32 mod p and its Shoup constant
-    uint32_t onep;       // floor(2^32/p): Shoup constant of 1
-};
-
 __device__ __forceinline__ uint32_t mfma_combine(const v4i (&acc)[7], int r, uint32_t p, const MfmaConst &mc)
 {
-    int64_t S = 0;
+    int32_t D[7];
 #pragma unroll
-    // 32 x 32 -> 64-bit signed products (one v_mad_i64_i32 each)
-    for (int d = 0; d < 7; ++d) S += (int64_t)acc[d][r] * (int64_t)mc.cd[d];
-    const uint64_t u = (uint64_t)S + mc.off;
-    const uint32_t hi = (uint32_t)(u >> 32), lo = (uint32_t)u;
-    return add_mod(shoup_mul(hi, mc.r32, mc.r32p, p), shoup_mul(lo, 1u, mc.onep, p), p);
+    for (int d = 0; d < 7; ++d) D[d] = acc[d][r];
+    return combine_diagonals(D, p, mc);
 }
 
 constexpr int kMaxKbPerChunk = 64;  // 64 steps of 64 = 4096 inner products per exact int32 chunk
@@ -451,7 +437,7 @@ __global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__r
 {
     __shared__ uint32_t tile[64][65];
     const int tid = threadIdx.x;
-    const size_t e0 = (size_t)blockIdx.x * 64, t0 = (size_t)blockIdx.y * 64;
+    const size_t t0 = (size_t)blockIdx.x * 64, e0 = (size_t)blockIdx.y * 64;
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
         const int tl = it * 4 + tid / 64, el = tid % 64;
@@ -469,22 +455,6 @@ __global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__r
 
 }  // namespace
 
-PlainConst plain_const(uint32_t p, uint32_t scale)
-{
-    PlainConst pc;
-    const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
-    pc.c0 = scale % p;
-    pc.c1 = (uint32_t)((uint64_t)r32 * pc.c0 % p);
-    pc.c0p = shoup_pre(pc.c0, p);
-    pc.c1p = shoup_pre(pc.c1, p);
-    pc.f62 = (uint32_t)(((uint64_t)1 << 62) % p);
-    // largest n with (2^62 + 2^33) + n (p-1)^2 < 2^64 (p < 2^31 here: n >= 2)
-    const unsigned __int128 room = ((unsigned __int128)1 << 64) - ((uint64_t)1 << 62) - ((uint64_t)1 << 33) - 1;
-    const uint64_t sq = (uint64_t)(p - 1) * (p - 1);
-    const unsigned __int128 nmax = sq ? room / sq : 64;
-    pc.fold = (uint32_t)(nmax > 64 ? 64 : nmax);
-    return pc;
-}
 
 hipError_t launch_batch_pointwise(const BatchJob *jobs, const PwItem *items, uint32_t nitems, uint32_t *counter,
                                   unsigned grid, hipStream_t s)
@@ -527,17 +497,7 @@ hipError_t launch_to_limbs(const uint32_t *X, uint8_t *Y, size_t rows, size_t kd
 hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t *Cmat, size_t m, size_t k, size_t n,
                                  size_t npts, uint32_t p, uint32_t scale, hipStream_t s)
 {
-    MfmaConst mc;
-    uint64_t pw = scale % p;
-    for (int d = 0; d < 7; ++d) {
-        mc.cd[d] = (int32_t)pw;
-        pw = pw * 256 % p;
-    }
-    const uint64_t two62 = (uint64_t)1 << 62;
-    mc.off = ((two62 + p - 1) / p) * p;
-    mc.r32 = (uint32_t)(((uint64_t)1 << 32) % p);
-    mc.r32p = shoup_pre(mc.r32, p);
-    mc.onep = shoup_pre(1, p);
+    const MfmaConst mc = mfma_const(p, scale);
     const size_t mpad = rpad_of(m), npad = rpad_of(n), kchunks = kpad_of(k) / 64;
     if (npts > 0x7fffffff) return hipErrorInvalidValue;
     const size_t a_bytes = limb_block_bytes(m, k);
@@ -576,8 +536,8 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
 hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries, size_t npts, hipStream_t s)
 {
     if (!entries || !npts) return hipSuccess;
-    dim3 grid((unsigned)((entries + 63) / 64), (unsigned)((npts + 63) / 64));
-    if (grid.y > 65535) return hipErrorInvalidValue;
+    if ((npts + 63) / 64 > 0x7fffffff || (entries + 63) / 64 > 65535) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((npts + 63) / 64), (unsigned)((entries + 63) / 64));
     hipLaunchKernelGGL(point_to_entry_kernel, grid, dim3(256), 0, s, X, Y, entries, npts);
     return hipGetLastError();
 }

@@ -341,14 +341,15 @@ class FFT:
         return coeffs
 
 
-def mul_mp(P, c, a, b, ctx=None, method=METHOD_AUTO, stream=None):
-    """Product over Z/PZ for a multiprecision P (int); a, b, c hold nlimbs words per coefficient."""
+def mul_mp(P, c, a, b, ctx=None, method=METHOD_AUTO, stream=None, max_rowdeg=0):
+    """Product over Z/PZ for a multiprecision P (int); a, b, c hold nlimbs words per coefficient.
+    With max_rowdeg, max_rowdeg+1 coefficients of the cyclic product (mul_crtla)."""
     ctx = ctx or default_context()
     nl = a.nlimbs
     limbs = np.array([(P >> (32 * i)) & 0xFFFFFFFF for i in range(nl)], dtype=np.uint32)
     pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
     check(lib().lbx_polymat_mul_mp(ctx.handle, limbs.ctypes.data, nl, ctypes.byref(pc), ctypes.byref(pa),
-                                   ctypes.byref(pb), method, stream))
+                                   ctypes.byref(pb), max_rowdeg, method, stream))
     c._size = pc.size
     return c
 

@@ -1,0 +1,47 @@
+// Host build of the word-size modular arithmetic the kernels use
+// (linbox_amd/csrc/modarith.h, linbox_amd/csrc/pw_math.h) for
+// tests/test_modarith_edges.py, which checks every result against Python
+// integers.  Each entry point computes one operation on plain integers.
+#include <cstdint>
+
+#include "../../linbox_amd/csrc/pw_math.h"
+
+using namespace lbx;
+
+extern "C" {
+
+uint32_t lbx_test_csub(uint32_t x, uint32_t p) { return csub(x, p); }
+uint32_t lbx_test_shoup_pre(uint32_t w, uint32_t p) { return shoup_pre(w, p); }
+uint32_t lbx_test_shoup_lazy(uint32_t x, uint32_t w, uint32_t p) { return shoup_lazy(x, w, shoup_pre(w, p), p); }
+uint32_t lbx_test_shoup_mul(uint32_t x, uint32_t w, uint32_t p) { return shoup_mul(x, w, shoup_pre(w, p), p); }
+uint32_t lbx_test_add_mod(uint32_t a, uint32_t b, uint32_t p) { return add_mod(a, b, p); }
+uint32_t lbx_test_sub_mod(uint32_t a, uint32_t b, uint32_t p) { return sub_mod(a, b, p); }
+uint32_t lbx_test_to_limbs(uint32_t x, uint32_t p) { return to_limbs(x, p); }
+uint32_t lbx_test_powmod(uint32_t a, uint64_t e, uint32_t p) { return powmod_h(a, e, p); }
+uint32_t lbx_test_invmod(uint32_t a, uint32_t p) { return invmod_h(a, p); }
+
+// combine_diagonals over seven int32 anti-diagonal sums.
+uint32_t lbx_test_combine(const int32_t *D, uint32_t p, uint32_t scale)
+{
+    const MfmaConst mc = mfma_const(p, scale);
+    int32_t d[7];
+    for (int i = 0; i < 7; ++i) d[i] = D[i];
+    return combine_diagonals(d, p, mc);
+}
+
+// The plain form's accumulation of n products a[i]*b[i] with the fold every
+// pc.fold products and the final scaled reduction, as in the kernel.
+uint32_t lbx_test_plain_dot(const uint32_t *a, const uint32_t *b, int n, uint32_t p, uint32_t scale)
+{
+    const PlainConst pc = plain_const(p, scale);
+    uint64_t acc = 0;
+    for (int q0 = 0; q0 < n; q0 += (int)pc.fold) {
+        const int q1 = n < q0 + (int)pc.fold ? n : q0 + (int)pc.fold;
+        for (int q = q0; q < q1; ++q) acc += (uint64_t)a[q] * b[q];
+        acc = plain_fold(acc, pc);
+    }
+    return plain_finish(acc, p, pc);
+}
+
+uint32_t lbx_test_plain_fold_count(uint32_t p) { return plain_const(p, 1).fold; }
+}

@@ -14,6 +14,8 @@ output is computed from the mathematical definition with Python integers:
                       matpoly-mult-naive.h:56-82), including max_rowdeg truncation
                       (cyclic wrap, matpoly-mult-fft-wordsize-fast.inl:64-80).
 * crt_mul.json     -- the same over Z/PZ for a seeded random 192-bit prime P.
+* polymat_mul_wrap.json -- max_rowdeg (cyclic wrap and truncation) over word
+                      moduli that are not FFT primes and over multiword moduli.
 * midproduct.json  -- the FFT-prime midproduct written as the cyclic
                       correlation it computes (matpoly-mult-fft-wordsize-fast.inl:209-316):
                       with pts = 2^ceil(log2 deg) and the first hdeg
@@ -244,6 +246,31 @@ def prime_cases(crt):
     return res
 
 
+def wrap_cases(rng):
+    """max_rowdeg on the routes that do not use one FFT prime: word moduli that
+    are not FFT primes for the length (the few-primes route of
+    three-primes.inl:67-84) and multiword moduli (mul_crtla,
+    multiprecision.inl:141-156): deg+1 coefficients of the product mod
+    x^pts - 1, pts = 2^ceil(log2(deg+1)); also max_rowdeg above the full degree
+    (zero coefficients past it) and constant operands."""
+    shapes = [(2, 2, 2, 6, 5, 5), (2, 3, 1, 8, 8, 6), (3, 2, 2, 9, 7, 9), (2, 2, 2, 3, 3, 7), (2, 2, 3, 1, 1, 3),
+              (1, 2, 1, 16, 16, 16)]
+    moduli = [(97, 1), (1000000007, 1), (536870909, 1), (3221225473, 1), (4294967291, 1),
+              (random_prime(61, random.Random(61)), 2), (random_prime(192, random.Random(192)), 6)]
+    cases = []
+    for (P, nl) in moduli:
+        for (m, k, n, sa, sb, mr) in shapes:
+            a = [rng.randrange(P) for _ in range(m * k * sa)]
+            b = [rng.randrange(P) for _ in range(k * n * sb)]
+            c = polymat_mul(P, m, k, n, a, sa, b, sb, mr)
+            cases.append({"P_hex": hex(P), "P_limbs": to_limbs(P, nl), "nlimbs": nl, "m": m, "k": k, "n": n,
+                          "sa": sa, "sb": sb, "max_rowdeg": mr,
+                          "a": [l for x in a for l in to_limbs(x, nl)],
+                          "b": [l for x in b for l in to_limbs(x, nl)],
+                          "c": [l for x in c for l in to_limbs(x, nl)]})
+    return cases
+
+
 def main():
     rng = random.Random(20191)
     with open(os.path.join(HERE, "ntt_kat.json"), "w") as f:
@@ -257,6 +284,8 @@ def main():
         json.dump(prime_cases(crt), f, separators=(",", ":"))
     with open(os.path.join(HERE, "midproduct.json"), "w") as f:
         json.dump(midproduct_cases(random.Random(20192)), f, separators=(",", ":"))
+    with open(os.path.join(HERE, "polymat_mul_wrap.json"), "w") as f:
+        json.dump(wrap_cases(random.Random(20193)), f, separators=(",", ":"))
 
 
 if __name__ == "__main__":

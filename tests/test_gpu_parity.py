@@ -382,3 +382,142 @@ def test_config2_full_size(ctx):
 def test_config3_full_size(ctx):
     """128x128.128x128, degree 8191, p=2013265921 (the headline shape)."""
     full_size_check(ctx, 2013265921, 128, 8192, 0x4C494E424F58 + 2)
+
+
+# ------------------------------------------- edge cases of the product ---
+def test_constant_matrices_with_max_rowdeg(ctx):
+    """sa = sb = 1 with max_rowdeg = 3: the FFT domain's result, coefficient 0
+    the product of the constants and coefficients 1..3 zero, whatever c held
+    (host and device storage, c pre-filled with nonzero words)."""
+    from linbox_amd.hip import DeviceBuffer, synchronize
+    p, m, k, n, deg = 2013265921, 5, 6, 4, 3
+    rng = np.random.default_rng(31)
+    a, b = rand(rng, p, m * k), rand(rng, p, k * n)
+    st, ref = orc.mul_fft_prime(p, m, k, n, a, 1, b, 1, max_rowdeg=deg)
+    assert st == 0
+    ref = ref.reshape(m * n, deg + 1)
+    assert (ref[:, 1:] == 0).all()
+    assert np.array_equal(ref[:, 0], orc.mul_naive(p, m, k, n, a, 1, b, 1))
+    for method in (lbx.METHOD_AUTO, lbx.METHOD_PLAIN_U64, lbx.METHOD_INT8_MFMA):
+        say("constant max_rowdeg host", method)
+        C = PolynomialMatrix.from_polfirst(np.full((m * n, deg + 1), 0x5A5A5A5, dtype=np.uint32), m, n)
+        PolynomialMatrixFFTMulDomain(p, ctx, method).mul(C, PolynomialMatrix.from_polfirst(a.reshape(-1, 1), m, k),
+                                                         PolynomialMatrix.from_polfirst(b.reshape(-1, 1), k, n), deg)
+        assert C.size() == deg + 1
+        assert np.array_equal(C.to_polfirst(), ref)
+        say("constant max_rowdeg device", method)
+        da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+        dc = DeviceBuffer.from_numpy(np.full(m * n * (deg + 1), 0x77777777, dtype=np.uint32))
+        C = PolynomialMatrix(m, n, deg + 1, data=dc)
+        PolynomialMatrixFFTMulDomain(p, ctx, method).mul(C, PolynomialMatrix(m, k, 1, data=da),
+                                                         PolynomialMatrix(k, n, 1, data=db), deg)
+        synchronize()
+        assert np.array_equal(dc.download().reshape(m * n, deg + 1), ref)
+
+
+def test_two_streams_one_context(ctx):
+    """Products queued back to back on two streams through one context (device
+    pointers, no host synchronisation between them) share the context's
+    workspace; the second stream waits for the first call's work, so both
+    results are exact."""
+    from linbox_amd.hip import DeviceBuffer, Stream
+    p = 998244353
+    cases = [(32, 32, 32, 1024, 1000), (32, 48, 16, 700, 1024), (40, 20, 24, 300, 500)]
+    for method in METHODS:
+        streams = [Stream(), Stream(), Stream()]
+        bufs, refs = [], []
+        for i, (m, k, n, sa, sb) in enumerate(cases):
+            rng = np.random.default_rng(100 + i)
+            a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+            _, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+            da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+            dc = DeviceBuffer(m * n * (sa + sb - 1) * 4)
+            dc.zero()
+            bufs.append((da, db, dc))
+            refs.append(ref)
+        say("two streams", method)
+        for i, (m, k, n, sa, sb) in enumerate(cases):
+            da, db, dc = bufs[i]
+            PolynomialMatrixMulDomain(p, ctx, method, stream=streams[i].value).mul(
+                PolynomialMatrix(m, n, sa + sb - 1, data=dc), PolynomialMatrix(m, k, sa, data=da),
+                PolynomialMatrix(k, n, sb, data=db))
+        for s in streams:
+            s.synchronize()
+        for i in range(len(cases)):
+            assert np.array_equal(bufs[i][2].download(), refs[i]), (method, i)
+
+
+def test_mfma_int32_chunk_boundary(ctx):
+    """Inner dimensions above 4096 run the INT8 pointwise stage in several
+    exact int32 chunks of 4096 inner products.  Random residues, and constant
+    operands whose centred digits are at the extremes (-128 in the three low
+    digits: the anti-diagonal D_2 grows by 3 * 2^14 = 49152 per inner product,
+    so from k = 43691 on one int32 accumulation of the whole inner product
+    would wrap), against the oracle."""
+    p = 2013265921
+    for (m, k, n, sa, sb) in [(2, 5000, 3, 16, 15), (16, 4200, 16, 8, 9), (2, 4500, 2, 200, 200),
+                              (3, 8193, 2, 3, 3)]:
+        rng = np.random.default_rng(k + m)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        assert st == 0
+        say("int32 chunks random", m, k, n, sa, sb)
+        got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, lbx.METHOD_INT8_MFMA)
+        assert np.array_equal(got.reshape(-1), ref)
+    # centred value with digits (-128, -128, -128, -59): c = -998277248, |c| < p/2
+    c = -128 - 128 * 256 - 128 * 65536 - 59 * (1 << 24)
+    assert abs(c) < p // 2
+    x = c % p
+    digits = (x + 0x80808080) ^ 0x80808080  # the kernels' digit split (of the centred value)
+    for (m, k, n) in [(2, 48000, 2), (1, 45000, 1)]:
+        # a(x) = x_const, b(x) = x_const + 0*x: both transforms are constant at every point
+        a = np.full(m * k, x, dtype=np.uint32)
+        b = np.zeros(k * n * 2, dtype=np.uint32)
+        b[0::2] = x
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, 1, b, 2)
+        assert st == 0
+        assert ref.reshape(m * n, 2)[0, 0] == k * x * x % p
+        say("int32 chunks extreme digits", m, k, n, hex(digits))
+        got = gpu_mul(ctx, p, m, k, n, a, 1, b, 2, lbx.METHOD_INT8_MFMA)
+        assert np.array_equal(got.reshape(-1), ref)
+
+
+def test_mul_fft_seam_values(ctx):
+    """lbx_polymat_mul_fft_u32 (mul_fft, wordsize-fast.inl:86-190) on padded
+    polfirst buffers of pts coefficients: with operands filling all pts
+    coefficients the result is the cyclic product mod x^pts - 1 (the oracle's
+    product truncated by max_rowdeg = pts-1); with operands padded by zeros it
+    is the plain product followed by zeros.  Host and device buffers, both forms."""
+    from linbox_amd.hip import DeviceBuffer, synchronize
+    L_ = lbx.lib()
+    for (p, m, k, n, lpts) in [(998244353, 3, 4, 5, 6), (2013265921, 17, 16, 18, 10), (469762049, 2, 3, 2, 13)]:
+        pts = 1 << lpts
+        rng = np.random.default_rng(lpts)
+        a, b = rand(rng, p, m * k * pts), rand(rng, p, k * n * pts)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, pts, b, pts, max_rowdeg=pts - 1)
+        assert st == 0
+        ha = a.reshape(m * k, pts).copy()
+        hb = b.reshape(k * n, pts).copy()
+        ha[:, pts // 2:] = 0
+        hb[:, pts // 2:] = 0
+        _, ref_pad = orc.mul_fft_prime(p, m, k, n, ha[:, :pts // 2].reshape(-1), pts // 2,
+                                       hb[:, :pts // 2].reshape(-1), pts // 2)
+        ref_pad = np.concatenate([ref_pad.reshape(m * n, pts - 1), np.zeros((m * n, 1), np.uint32)], axis=1)
+        for method in METHODS:
+            for (xa, xb, want) in [(a, b, ref), (ha.reshape(-1), hb.reshape(-1), ref_pad.reshape(-1))]:
+                say("mul_fft seam host", p, m, k, n, lpts, method)
+                c = np.full(m * n * pts, 3, dtype=np.uint32)
+                st = L_.lbx_polymat_mul_fft_u32(ctx.handle, p, lpts, m, k, n, xa.ctypes.data, xb.ctypes.data,
+                                                c.ctypes.data, method, None)
+                assert st == 0
+                assert np.array_equal(c, want)
+                say("mul_fft seam device", p, m, k, n, lpts, method)
+                da, db = DeviceBuffer.from_numpy(xa), DeviceBuffer.from_numpy(xb)
+                dc = DeviceBuffer(m * n * pts * 4)
+                st = L_.lbx_polymat_mul_fft_u32(ctx.handle, p, lpts, m, k, n, da.data_ptr(), db.data_ptr(),
+                                                dc.data_ptr(), method, None)
+                assert st == 0
+                synchronize()
+                assert np.array_equal(dc.download(), want)
+                # the operands are left intact (the reference transforms its copies in place)
+                assert np.array_equal(da.download(), xa) and np.array_equal(db.download(), xb)

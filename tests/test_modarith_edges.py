@@ -1,0 +1,140 @@
+"""Edge values of the word-size modular arithmetic in the kernels
+(linbox_amd/csrc/modarith.h and pw_math.h), built for the host and checked
+against Python integers: csub, Shoup products (the reference's
+mul_precomp_b_without_reduction, fft-simd.h:123-132), the balanced signed
+digits of the INT8 form, the MFMA epilogue that recombines the seven digit
+anti-diagonals, and the plain form's 64-bit accumulation with its folds.
+
+Values: 0, 1, p-1, p, 2p-1, 2^31-1, 2^32-1 and the extremes each function's
+contract allows, for the four FFT primes of the workloads; the host helpers
+(powmod, invmod, shoup_pre) also for 4294967291 (p > 2^31, where the
+word-size kernels are not used and the few-primes route keeps 64-bit words)."""
+import ctypes
+import os
+import random
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "cpp", "modarith_harness.cpp")
+OUT = os.path.join(HERE, "cpp", "_build", "libmodarith.so")
+ROOT = os.path.dirname(HERE)
+
+PRIMES = [998244353, 2013265921, 469762049, 167772161]
+WIDE = 4294967291
+
+
+@pytest.fixture(scope="module")
+def lib():
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-shared", "-fPIC", "-o", OUT, SRC], check=True)
+    L = ctypes.CDLL(OUT)
+    u32, u64 = ctypes.c_uint32, ctypes.c_uint64
+    for name, args in [("csub", [u32, u32]), ("shoup_pre", [u32, u32]), ("shoup_lazy", [u32, u32, u32]),
+                       ("shoup_mul", [u32, u32, u32]), ("add_mod", [u32, u32, u32]), ("sub_mod", [u32, u32, u32]),
+                       ("to_limbs", [u32, u32]), ("powmod", [u32, u64, u32]), ("invmod", [u32, u32]),
+                       ("combine", [ctypes.POINTER(ctypes.c_int32), u32, u32]),
+                       ("plain_dot", [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.c_int, u32, u32]),
+                       ("plain_fold_count", [u32])]:
+        f = getattr(L, "lbx_test_" + name)
+        f.argtypes = args
+        f.restype = u32
+    return L
+
+
+def edges(p):
+    return sorted({0, 1, 2, p // 2, (p - 1) // 2, (p + 1) // 2, p - 2, p - 1})
+
+
+def test_csub_and_add_sub(lib):
+    for p in PRIMES:
+        for x in [0, 1, p - 1, p, p + 1, 2 * p - 2, 2 * p - 1]:
+            assert lib.lbx_test_csub(x, p) == x % p, (p, x)
+        for a in edges(p):
+            for b in edges(p):
+                assert lib.lbx_test_add_mod(a, b, p) == (a + b) % p
+                assert lib.lbx_test_sub_mod(a, b, p) == (a - b) % p
+
+
+def test_shoup_products(lib):
+    rng = random.Random(1)
+    for p in PRIMES:
+        ws = edges(p) + [rng.randrange(p) for _ in range(6)]
+        xs = [0, 1, p - 1, p, 2 * p - 1, (1 << 31) - 1, (1 << 32) - 1] + [rng.getrandbits(32) for _ in range(6)]
+        for w in ws:
+            assert lib.lbx_test_shoup_pre(w, p) == (w << 32) // p
+            for x in xs:
+                lazy = lib.lbx_test_shoup_lazy(x, w, p)
+                assert 0 <= lazy < 2 * p and lazy % p == x * w % p, (p, w, x)
+                assert lib.lbx_test_shoup_mul(x, w, p) == x * w % p, (p, w, x)
+
+
+def test_balanced_digits(lib):
+    for p in PRIMES:
+        for x in edges(p) + [p // 3, 2 * p // 3]:
+            v = lib.lbx_test_to_limbs(x, p)
+            d = [((v >> (8 * s)) & 0xFF) for s in range(4)]
+            d = [t - 256 if t >= 128 else t for t in d]
+            assert all(-128 <= t <= 127 for t in d)
+            c = x if x <= p // 2 else x - p
+            assert sum(t << (8 * s) for s, t in enumerate(d)) == c, (p, x, d)
+            assert abs(c) < (1 << 30)
+
+
+def test_mfma_epilogue_extremes(lib):
+    """sum_d D_d 2^(8d) * scale mod p for anti-diagonals at the int32 chunk bound
+    |D_d| <= 4 * 4096 * 2^14 = 2^28 (both signs), zeros and random values."""
+    rng = random.Random(2)
+    B = 1 << 28
+    for p in PRIMES:
+        for scale in [1, p - 1, pow(1 << 14, p - 2, p), rng.randrange(1, p)]:
+            cases = [[0] * 7, [B] * 7, [-B] * 7, [B, -B] * 3 + [B], [-B, B] * 3 + [-B], [1, 0, 0, 0, 0, 0, -1]]
+            cases += [[rng.randint(-B, B) for _ in range(7)] for _ in range(20)]
+            for D in cases:
+                arr = (ctypes.c_int32 * 7)(*D)
+                want = sum(D[d] << (8 * d) for d in range(7)) * scale % p
+                assert lib.lbx_test_combine(arr, p, scale) == want, (p, scale, D)
+
+
+def test_plain_accumulation_never_wraps(lib):
+    """Inner products of (p-1)*(p-1) terms, the largest the plain form adds,
+    over lengths around multiples of the fold period: a 64-bit wrap would
+    change the result."""
+    for p in PRIMES:
+        fold = lib.lbx_test_plain_fold_count(p)
+        assert 2 <= fold <= 64
+        # the documented bound: 2^62 + 2^33 + fold (p-1)^2 < 2^64, and fold is the largest such n (or 64)
+        assert (1 << 62) + (1 << 33) + fold * (p - 1) ** 2 < 1 << 64
+        if fold < 64:
+            assert (1 << 62) + (1 << 33) + (fold + 1) * (p - 1) ** 2 >= (1 << 64) - 1
+        for n in [1, fold - 1, fold, fold + 1, 3 * fold, 4096, 4097]:
+            if n < 1:
+                continue
+            a = (ctypes.c_uint32 * n)(*([p - 1] * n))
+            for scale in [1, pow(1 << 13, p - 2, p)]:
+                want = n * (p - 1) ** 2 * scale % p
+                assert lib.lbx_test_plain_dot(a, a, n, p, scale) == want, (p, n, scale)
+
+
+def test_host_helpers_wide_modulus(lib):
+    """powmod / invmod / shoup_pre for moduli up to 2^32-5 (the few-primes route's range)."""
+    rng = random.Random(3)
+    for p in PRIMES + [WIDE, 3221225473, 2147483647]:
+        for a in [1, 2, p - 1, p - 2] + [rng.randrange(1, p) for _ in range(5)]:
+            for e in [0, 1, 2, p - 2, p - 1, (1 << 63) + 5]:
+                assert lib.lbx_test_powmod(a, e, p) == pow(a, e, p)
+            assert lib.lbx_test_invmod(a, p) * a % p == 1
+            assert lib.lbx_test_shoup_pre(a, p) == (a << 32) // p
+
+
+def test_planted_mistake_is_caught(lib):
+    """A digit split without the centring step (digits of x itself) misses
+    residues above p/2: the reconstruction check would see it."""
+    p = 2013265921
+    x = p - 1
+    raw = (x + 0x80808080) ^ 0x80808080  # no centring
+    d = [((raw >> (8 * s)) & 0xFF) for s in range(4)]
+    d = [t - 256 if t >= 128 else t for t in d]
+    c = x - p
+    assert sum(t << (8 * s) for s, t in enumerate(d)) != c
