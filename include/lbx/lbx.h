@@ -102,6 +102,23 @@ lbx_status lbx_trim(lbx_ctx *ctx);
 lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
                                const lbx_polmat *b, size_t max_rowdeg, int method, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lbx_polymat_mul_u32 restricted to device pointers and to
+ * its single-prime FFT and constant-matrix routes, with every device buffer
+ * (transform scratch, limb blocks, twiddle tables) cut from one fixed 64 MiB
+ * device array per GPU, held until the process ends; the call makes no
+ * device allocation.  A product needing more than 64 MiB of workspace returns
+ * LBX_ERR_WORKSPACE_TOO_SMALL before any work is queued (e.g. the plain form
+ * needs 4 (mk+kn+mn) 2^L bytes plus twiddles), host pointers
+ * LBX_ERR_INVALID_ARG, and moduli that need the few-primes or multiprecision
+ * route LBX_ERR_UNSUPPORTED.  All calls on one GPU share the array: calls
+ * that overlap on the device (other streams, other contexts, other host
+ * threads) race; calls queued in order on one stream are correct.  Results
+ * equal lbx_polymat_mul_u32's. */
+lbx_status lbx_polymat_mul_u32_noalloc(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
+                                       const lbx_polmat *b, size_t max_rowdeg, int method, void *stream);
+
 /* PolynomialMatrixFFTPrimeMulDomain<F>::mul_fft(lpts, c, a, b)
  * (matpoly-mult-fft-wordsize-fast.inl:86-190): the narrower seam on padded
  * polfirst buffers of 2^lpts coefficients per entry; a and b are left intact

@@ -28,6 +28,7 @@ EXPORTED = [
     "lbx_ntt_forward_u32", "lbx_ntt_inverse_u32", "lbx_polymat_mul_mp", "lbx_polymat_mul_batch",
     "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_order_basis_pm_u32",
     "lbx_order_basis_m_u32", "lbx_polymat_copy_u32", "lbx_profile_enable", "lbx_profile_get",
+    "lbx_polymat_mul_u32_noalloc",
 ]
 
 
@@ -75,6 +76,7 @@ def lib():
         L.lbx_strerror.argtypes = [i32]
         L.lbx_strerror.restype = ctypes.c_char_p
         L.lbx_polymat_mul_u32.argtypes = [vp, u32, PM, PM, PM, sz, i32, vp]
+        L.lbx_polymat_mul_u32_noalloc.argtypes = [vp, u32, PM, PM, PM, sz, i32, vp]
         L.lbx_polymat_mul_fft_u32.argtypes = [vp, u32, ctypes.c_uint, sz, sz, sz, vp, vp, vp, i32, vp]
         L.lbx_ntt_forward_u32.argtypes = [vp, u32, ctypes.c_uint, u32, vp, sz, ctypes.POINTER(u32), vp]
         L.lbx_ntt_inverse_u32.argtypes = [vp, u32, ctypes.c_uint, u32, vp, sz, ctypes.POINTER(u32), vp]

@@ -557,6 +557,11 @@ lbx_status lbx_init(lbx_ctx **out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return LBX_ERR_NO_DEVICE;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return LBX_ERR_NO_DEVICE;
+    {
+        DeviceGuard g(device);
+        const lbx_status st = noalloc_attach(device);
+        if (st != LBX_OK) return st;
+    }
     lbx_ctx *ctx = new lbx_ctx();
     ctx->device = device;
     *out = ctx;

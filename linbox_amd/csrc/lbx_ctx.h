@@ -207,6 +207,9 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
 // Product of constant matrices (one coefficient each).
 lbx_status constant_product(lbx_ctx *ctx, uint32_t p, const DevView &a, const DevView &b, const DevView &c, size_t m,
                             size_t k, size_t n, hipStream_t s, Workspace *ws = nullptr);
+// Records the device address of the fixed workspace of
+// lbx_polymat_mul_u32_noalloc on `device` (current); once per device.
+lbx_status noalloc_attach(int device);
 // Frees the multiprecision plans cached in a context.
 void mp_release(lbx_ctx *ctx);
 // Frees the batch plans (graphs, descriptors) cached in a context.
