@@ -21,12 +21,25 @@ The JSON line also carries:
                 over a non-FFT prime below 2^29 (few-primes route, and the
                 multiprecision route for comparison), and one 128x128 x 8192
                 layout conversion;
+  ntt_passes    the forward and inverse NTT phases against the workload's
+                algorithmic bytes, 4(mk+kn)(sa+pts) and 4mn(pts+D+1), as a
+                fraction of 8 TB/s;
+  int8_mfma_utilisation  32 pts mkn INT8 operations of the pointwise kernel over
+                its time, as a fraction of the 5.0 POPS dense INT8 peak;
+  self_check    the last timed product (each form) checked at two random points
+                and on 64 random coefficients computed exactly; a mismatch stops
+                the run with an error;
+  host_operands the same product with numpy operands through the C ABI, PCIe
+                copies included (never `value`);
   cpu_baseline  the oracle's restatement of the reference's single-prime
                 mul_fft timed on this host on a bounded sample of the same
-                workload and extrapolated to one product.
+                workload and extrapolated to one product, per stage, on one
+                thread and on up to 16; the host's nproc and CPU model; and the
+                16x16 degree-255 configuration timed whole on one thread.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-batch]
-                       [--skip-order-basis] [--skip-configs] [--only order_basis|configs|batch]
+                       [--skip-order-basis] [--skip-configs] [--skip-host]
+                       [--only order_basis|configs|batch]
 """
 import argparse
 import json
@@ -43,6 +56,7 @@ P = 2013265921
 M = K = N = 128
 SA = SB = 8192
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0  # dense INT8 MFMA, MI355X
 
 
 def kernel_bytes(m, k, n, sa, sb, pts):
@@ -108,7 +122,6 @@ def run_gpu(args):
     sc = SA + SB - 1
     da, db = hip.DeviceBuffer.from_numpy(a), hip.DeviceBuffer.from_numpy(b)
     dc = hip.DeviceBuffer(M * N * sc * 4)
-    del a, b
     stream = hip.Stream()
     A, B = PolynomialMatrix(M, K, SA, data=da), PolynomialMatrix(K, N, SB, data=db)
     C = PolynomialMatrix(M, N, sc, data=dc)
@@ -136,11 +149,88 @@ def run_gpu(args):
 
     ms_int8, wall_int8, stages_int8 = timed(lbx.METHOD_INT8_MFMA, args.steps, args.warmup)
     print("int8 form: %.3f ms/product" % ms_int8, file=sys.stderr, flush=True)
+    check = self_check(a, b, dc.download(), rng)
+    print("self-check of the last timed product: %s" % check["result"], file=sys.stderr, flush=True)
+    if check["result"] != "exact":
+        raise SystemExit("bench: the timed product is wrong (%s)" % check)
     plain_steps = max(2, args.steps // 4)
     ms_plain, _, stages_plain = timed(lbx.METHOD_PLAIN_U64, plain_steps, 1)
     print("plain form: %.3f ms/product" % ms_plain, file=sys.stderr, flush=True)
+    plain_ok = self_check(a, b, dc.download(), rng)
+    if plain_ok["result"] != "exact":
+        raise SystemExit("bench: the plain-form product is wrong (%s)" % plain_ok)
+    da.free(), db.free(), dc.free()
+    host = None if args.skip_host else host_leg(ctx, lbx, a, b)
     ctx.close()
-    return ms_int8, wall_int8, stages_int8, ms_plain, stages_plain
+    return ms_int8, wall_int8, stages_int8, ms_plain, stages_plain, check, host
+
+
+def self_check(a, b, c, rng, points=2, coeffs=64):
+    """Check the 128x128 . 128x128 product c = a.b left by the timed loop:
+    C(x) = A(x) B(x) mod p at `points` random x (Schwartz-Zippel: a wrong
+    coefficient passes with probability <= deg/p per point), and `coeffs`
+    random coefficients c[i][j][t] = sum_q sum_u a[i][q][u] b[q][j][t-u] mod p
+    computed exactly (16-bit split of a so no 64-bit sum overflows)."""
+    sc = SA + SB - 1
+    A = a.reshape(M * K, SA)
+    B = b.reshape(K * N, SB)
+    C = c.reshape(M * N, sc)
+    pp = np.uint64(P)
+
+    def horner(X, x):
+        acc = np.zeros(X.shape[0], dtype=np.uint64)
+        xx = np.uint64(x)
+        for t in range(X.shape[1] - 1, -1, -1):
+            acc = (acc * xx + X[:, t]) % pp
+        return acc
+
+    def matmul_mod(X, Y):
+        lo, hi = Y & np.uint64(0xFFFF), Y >> np.uint64(16)
+        return ((X @ hi) % pp * np.uint64(1 << 16) % pp + (X @ lo) % pp) % pp
+
+    for x in rng.integers(2, P, points):
+        Ax = horner(A, x).reshape(M, K)
+        Bx = horner(B, x).reshape(K, N)
+        Cx = horner(C, x).reshape(M, N)
+        if not np.array_equal(matmul_mod(Ax, Bx), Cx):
+            return {"result": "mismatch at x=%d" % int(x)}
+    alo = (A & 0xFFFF).astype(np.uint64)
+    ahi = (A >> 16).astype(np.uint64)
+    Bu = B.astype(np.uint64)
+    for _ in range(coeffs):
+        i, j, t = int(rng.integers(M)), int(rng.integers(N)), int(rng.integers(sc))
+        u0, u1 = max(0, t - (SB - 1)), min(t, SA - 1)
+        total = 0
+        for q in range(K):
+            av_lo, av_hi = alo[i * K + q, u0:u1 + 1], ahi[i * K + q, u0:u1 + 1]
+            bv = Bu[q * N + j, t - u1:t - u0 + 1][::-1]
+            total += int(av_lo @ bv) + (int(av_hi @ bv) << 16)
+        if total % P != int(C[i * N + j, t]):
+            return {"result": "mismatch at c[%d][%d][%d]" % (i, j, t)}
+    return {"result": "exact", "points": points, "coefficients": coeffs}
+
+
+def host_leg(ctx, lbx, a, b, reps=2):
+    """The same product with host (numpy) operands through the C ABI, as a
+    LinBox caller's matrices are: the call stages a and b into device buffers
+    kept in the context, computes, and copies c back (1 GiB in, 1 GiB out over
+    PCIe).  Wall time per call, PCIe included; never the headline value."""
+    from linbox_amd import PolynomialMatrix, PolynomialMatrixMulDomain
+    A = PolynomialMatrix.from_polfirst(a.reshape(M * K, SA), M, K)
+    B = PolynomialMatrix.from_polfirst(b.reshape(K * N, SB), K, N)
+    C = PolynomialMatrix(M, N, SA + SB - 1)
+    dom = PolynomialMatrixMulDomain(P, ctx, lbx.METHOD_INT8_MFMA)
+    dom.mul(C, A, B)  # staging buffers
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dom.mul(C, A, B)
+        times.append(time.perf_counter() - t0)
+    ms = min(times) * 1e3
+    print("host operands: %.1f ms per product (PCIe included)" % ms, file=sys.stderr, flush=True)
+    return {"ms_per_product": round(ms, 2), "products_per_s": round(1e3 / ms, 2), "reps": reps,
+            "bytes_over_pcie": int(a.nbytes + b.nbytes + C.data.nbytes),
+            "note": "numpy operands through lbx_polymat_mul_u32 (INT8 form): copies in and out included"}
 
 
 def run_batch(steps=5):
@@ -272,7 +362,12 @@ def run_configs(reps=10):
     B = PolynomialMatrix(dim, dim, sa, data=db, nlimbs=nl)
     C = PolynomialMatrix(dim, dim, 2 * sa - 1, data=dc, nlimbs=nl)
     ms = timed(lambda: lbx.mul_mp(Pm, C, A, B, ctx=ctx, stream=stream.value))
-    out["32x32_deg1023_P192"] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1), "fft_primes": 14}
+    out["32x32_deg1023_P192"] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1), "fft_primes": 14,
+                                 "pointwise": "auto: plain uint64 form, the 14 per-prime products as one call "
+                                              "of the batch kernels"}
+    ms = timed(lambda: lbx.mul_mp(Pm, C, A, B, ctx=ctx, method=lbx.METHOD_INT8_MFMA, stream=stream.value))
+    out["32x32_deg1023_P192_int8"] = {"ms": round(ms, 4), "products_per_s": round(1e3 / ms, 1), "fft_primes": 14,
+                                      "pointwise": "int8_mfma per prime (14 single-prime products)"}
     # middle product of a PM_Basis level: coefficients 2048..4095 of sigma1.serie
     p, m, n, o1, order = 998244353, 64, 32, 2048, 4096
     ds = dev(rng.integers(0, p, m * m * (o1 + 1), dtype=np.uint64).astype(np.uint32))
@@ -446,7 +541,8 @@ def cpu_baseline(budget_s):
     sample = ("oracle mul_fft stages on one thread: %d forward and %d inverse 2^14-point NTTs, "
               "%d pointwise 128x128x128 products mod p; extrapolated to 32768 forward NTTs, "
               "16384 points and 16384 inverse NTTs (one product)" % (n_fwd, n_inv, n_pt))
-    # the same stages over the host's cores (OpenMP over polynomials and points), half the budget
+    # the same stages over the host's cores (OpenMP over polynomials and points), half the budget;
+    # at most 16 threads, the host CPU share of one GPU on the measurement boxes
     threads = max(1, min(16, os.cpu_count() or 1))
     mt_budget = budget_s / 2
 
@@ -471,8 +567,48 @@ def cpu_baseline(budget_s):
     t_mt = (M * K + K * N) * mt_f + pts * mt_p + M * N * mt_i
     mt = {"value": 1.0 / t_mt, "unit": "products/s", "cores": threads, "kind": "port", "s_per_product": round(t_mt, 2),
           "sample": "the same stages on %d OpenMP threads: %d forward and %d inverse NTTs, %d points, extrapolated "
-                    "to one product" % (threads, nf, ni, npx)}
-    return 1.0 / t_prod, sample, t_prod, mt
+                    "to one product" % (threads, nf, ni, npx),
+          "thread_cap": "16 (the host CPU share of one GPU on the measurement boxes)",
+          "stages_s": {"direct FFT_DIF": round((M * K + K * N) * mt_f, 3), "Pointwise mult": round(pts * mt_p, 3),
+                       "inverse FFT_DIT": round(M * N * mt_i, 3)}}
+    stages = {"direct FFT_DIF": round((M * K + K * N) * t_fwd, 3), "Pointwise mult": round(pts * t_pt, 3),
+              "inverse FFT_DIT": round(M * N * t_inv, 3)}
+    return 1.0 / t_prod, sample, t_prod, mt, stages
+
+
+def host_info():
+    """nproc, the CPU model and the thread count the CPU legs may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model,
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_config1():
+    """The 16x16 . 16x16, degree 255 product over 998244353 (the reference's own
+    CPU-runnable configuration) through the oracle's mul_fft restatement on one
+    thread, repeated until at least one second has passed, mean time per product
+    (the timing loop of benchmark-polynomial-matrix-mul-fft.C:94-97)."""
+    from oracle import oracle as orc
+    p, m, sa = 998244353, 16, 256
+    a = orc.fill_uniform(0x4C494E424F58 + 0, p, m * m * sa)
+    b = orc.fill_uniform(0x4C494E424F58 + 100, p, m * m * sa)
+    reps, t = 0, 0.0
+    while t < 1.0:
+        t0 = time.perf_counter()
+        st, _ = orc.mul_fft_prime(p, m, m, m, a, sa, b, sa)
+        t += time.perf_counter() - t0
+        reps += 1
+        assert st == 0
+    return {"workload": "16x16.16x16 deg 255 p=998244353", "ms_per_product": round(t / reps * 1e3, 3),
+            "products_per_s": round(reps / t, 2), "reps": reps, "cores": 1, "kind": "port"}
 
 
 def main():
@@ -485,6 +621,7 @@ def main():
     ap.add_argument("--skip-batch", action="store_true", help="omit the 1024-product batch measurement")
     ap.add_argument("--skip-order-basis", action="store_true", help="omit the PM_Basis measurement")
     ap.add_argument("--skip-configs", action="store_true", help="omit the other workload configurations")
+    ap.add_argument("--skip-host", action="store_true", help="omit the host-operand (PCIe-inclusive) leg")
     ap.add_argument("--only", choices=["order_basis", "configs", "batch"],
                     help="run only this leg and print its JSON object (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -506,7 +643,7 @@ def main():
                "batch": run_batch}[args.only]
         print(json.dumps({args.only: leg()}))
         return
-    ms_int8, wall, stages, ms_plain, stages_plain = run_gpu(args)
+    ms_int8, wall, stages, ms_plain, stages_plain, check, host = run_gpu(args)
     batch = None if args.skip_batch else run_batch()
     order_basis = None if args.skip_order_basis else run_order_basis(args.skip_cpu)
     configs = None if args.skip_configs else run_configs()
@@ -515,10 +652,30 @@ def main():
     if args.skip_cpu:
         cpu = None
     else:
-        v, sample, t_prod, mt = cpu_baseline(args.cpu_budget)
+        v, sample, t_prod, mt, cstages = cpu_baseline(args.cpu_budget)
         cpu = {"value": v, "unit": "products/s", "cores": 1, "kind": "port", "sample": sample,
-               "s_per_product": round(t_prod, 2), "all_cores": mt}
+               "s_per_product": round(t_prod, 2), "stages_s": cstages, "host": host_info(), "all_cores": mt,
+               "config_16x16_deg255": cpu_config1()}
     value = 1000.0 / ms_int8
+    stage = {k: v for k, v in stages}
+    # the NTT passes against the workload table's bytes: forward 4 (mk+kn)(sa+pts) over the
+    # "direct FFT_DIF" stage, inverse 4 mn (pts+D+1) over "inverse FFT_DIT"
+    deg = SA + SB - 2
+    ntt = {}
+    for name, label, nbytes in [("forward", "direct FFT_DIF", 4 * (M * K + K * N) * (SA + pts)),
+                                ("inverse", "inverse FFT_DIT", 4 * M * N * (pts + deg + 1))]:
+        if label in stage:
+            gbs = nbytes / (stage[label] * 1e-3) / 1e9
+            ntt[name] = {"stage": label, "bytes": nbytes, "ms": round(stage[label], 4), "gbs": round(gbs, 1),
+                         "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4)}
+    # INT8 MFMA utilisation of the pointwise stage: 32 pts mkn int8 ops (16 digit-pair products
+    # of k-deep dot products, 2 ops each) over its time, against the dense INT8 peak
+    mfma = None
+    if "kernel pw_mfma" in stage:
+        ops = 32 * pts * M * K * N
+        tops = ops / (stage["kernel pw_mfma"] * 1e-3) / 1e12
+        mfma = {"kernel": "pw_mfma", "int8_ops": ops, "ms": round(stage["kernel pw_mfma"], 4),
+                "tops": round(tops, 1), "peak_tops": INT8_PEAK_TOPS, "frac": round(tops / INT8_PEAK_TOPS, 4)}
     out = {
         "metric": "bit-exact polymatrix mul/s, 128^2 deg 8191: INT8-MFMA vs uint64-ALU form",
         "value": round(value, 3),
@@ -541,6 +698,10 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages if not k.startswith("kernel ")},
         "kernels": kernels,
         "gmulmod_per_s": round(40265318400 / (ms_int8 * 1e-3) / 1e9, 2),
+        "ntt_passes": ntt,
+        "int8_mfma_utilisation": mfma,
+        "self_check": check,
+        "host_operands": host,
         "wall_s_timed": round(wall, 4),
         "roofline": roofline,
         "batch_1024_products": batch,

@@ -523,7 +523,8 @@ lbx_status ArenaWorkspace::twiddle_table(uint32_t p, unsigned L, uint32_t w, con
     return LBX_OK;
 }
 
-lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out)
+lbx_status stage_in(lbx_ctx *ctx, const char *slot, const lbx_polmat *src, size_t word_bytes, bool copy_in,
+                    Staged *out)
 {
     out->v = *src;
     if (is_device_ptr(src->data)) {
@@ -531,7 +532,7 @@ lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Stag
         return LBX_OK;
     }
     const size_t bytes = storage_words(src) * word_bytes;
-    LBX_TRY_HIP(hipMalloc(&out->dev, bytes ? bytes : 16));
+    LBX_TRY(ensure(ctx, slot, bytes, &out->dev));
     out->owned = true;
     if (copy_in) LBX_TRY_HIP(hipMemcpy(out->dev, src->data, bytes, hipMemcpyHostToDevice));
     return LBX_OK;
@@ -539,8 +540,7 @@ lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Stag
 
 void release(Staged &s)
 {
-    if (s.owned && s.dev) (void)hipFree(s.dev);
-    s.dev = nullptr;
+    s.dev = nullptr;  // staging buffers belong to the context
 }
 
 }  // namespace lbx
@@ -709,9 +709,11 @@ lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lb
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(a, 4, true, &sa_);
-    if (st == LBX_OK) st = stage_in(b, 4, true, &sb_);
-    if (st == LBX_OK) st = stage_in(c, 4, true, &sc_);
+    // host storage of c is copied in only when the call leaves part of it as it is
+    const bool c_in = !(c->shift == 0 && deg + 1 >= c->storage_size);
+    lbx_status st = stage_in(ctx, "stage_a", a, 4, true, &sa_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, 4, true, &sb_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, 4, c_in, &sc_);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK) {
         const DevView da = devview(a, sa_.dev), db = devview(b, sb_.dev), dc = devview(c, sc_.dev);
@@ -755,9 +757,9 @@ lbx_status lbx_polymat_mul_fft_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, size
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(&va, 4, true, &sa_);
-    if (st == LBX_OK) st = stage_in(&vb, 4, true, &sb_);
-    if (st == LBX_OK) st = stage_in(&vc, 4, false, &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", &va, 4, true, &sa_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", &vb, 4, true, &sb_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", &vc, 4, false, &sc_);
     if (st == LBX_OK)
         st = fft_prime_product(ctx, p, lpts, devview(&va, sa_.dev), devview(&vb, sb_.dev), devview(&vc, sc_.dev), m,
                                k, n, pts, method, s);
@@ -797,7 +799,7 @@ static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t ro
     LBX_TRY(twiddles(ctx, p, lpts, w, &tw));
     lbx_polmat v{(void *)data, LBX_POLFIRST, count, 1, N, N, 0};
     Staged sd;
-    LBX_TRY(stage_in(&v, 4, true, &sd));
+    LBX_TRY(stage_in(ctx, "stage_a", &v, 4, true, &sd));
     const DevView dv = devview(&v, sd.dev);
     hipError_t e = inverse ? launch_ntt_inverse((uint32_t *)sd.dev, count, lpts, tw, p, dv, N, 1, s)
                            : launch_ntt_forward(dv, count, lpts, tw, p, (uint32_t *)sd.dev, s);

@@ -125,13 +125,15 @@ lbx_status check_view(const lbx_polmat *v);
 DevView devview(const lbx_polmat *v, void *dev_base);
 
 // Host operands are staged through device copies of their whole storage
-// (word_bytes per coefficient slot).
+// (word_bytes per coefficient slot), in the context's workspace buffer `slot`
+// (kept between calls like the rest of the workspace).
 struct Staged {
     lbx_polmat v;
     void *dev = nullptr;
-    bool owned = false;
+    bool owned = false;  // dev is a staging copy of host storage
 };
-lbx_status stage_in(const lbx_polmat *src, size_t word_bytes, bool copy_in, Staged *out);
+lbx_status stage_in(lbx_ctx *ctx, const char *slot, const lbx_polmat *src, size_t word_bytes, bool copy_in,
+                    Staged *out);
 void release(Staged &s);
 
 // Where one call's device buffers come from.  The context's workspace keeps

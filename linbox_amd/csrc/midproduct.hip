@@ -109,9 +109,9 @@ lbx_status with_views(lbx_ctx *ctx, lbx_polmat *c, const lbx_polmat *a, const lb
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(a, 4, true, &sa_);
-    if (st == LBX_OK) st = stage_in(b, 4, true, &sb_);
-    if (st == LBX_OK) st = stage_in(c, 4, copy_c, &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", a, 4, true, &sa_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, 4, true, &sb_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, 4, copy_c, &sc_);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK) st = body(devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), s);
     if (st == LBX_OK && sc_.owned) {

@@ -762,9 +762,9 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
     StreamOrder stream_order(ctx, s);
     const size_t wb = 4 * (size_t)nlimbs;
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(a, wb, true, &sa_);
-    if (st == LBX_OK) st = stage_in(b, wb, true, &sb_);
-    if (st == LBX_OK) st = stage_in(c, wb, true, &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", a, wb, true, &sa_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wb, true, &sb_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wb, !(c->shift == 0 && sc >= c->storage_size), &sc_);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK)
         st = mp_product(ctx, P, nlimbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k, n,

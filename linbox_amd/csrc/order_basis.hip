@@ -752,8 +752,8 @@ lbx_status order_basis(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_po
     R.mb.deg = (uint64_t *)(base + b_S + b_W + b_sh);
 
     Staged ss, sg;
-    lbx_status st = stage_in(serie, 4, true, &ss);
-    if (st == LBX_OK) st = stage_in(sigma, 4, true, &sg);
+    lbx_status st = stage_in(ctx, "stage_a", serie, 4, true, &ss);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", sigma, 4, true, &sg);
     std::vector<uint64_t> sh(shift, shift + m);
     size_t d = 0;
     if (st == LBX_OK) st = from_hip(hipMemcpyAsync(R.mb.shift, sh.data(), m * 8, hipMemcpyHostToDevice, s));

@@ -224,34 +224,44 @@ bool check_mul_fft(const Field &F, RandIter &Gen, size_t lpts)
     return lbx_test::report(ok && threw, "mul_fft", shape);
 }
 
+// Horner evaluation at w^i: every point up to 2^10 points, 64 sampled points above.
 template <typename Field, typename RandIter>
 bool check_fft(const Field &F, RandIter &Gen, size_t k)
 {
     typedef typename Field::Element Elt;
     const size_t n = (size_t)1 << k;
-    std::vector<Elt> in(n), in_br(n), out(n), out_br(n);
+    std::vector<Elt> in(n), in_br(n);
     FFT<Field, NoSimd<Elt>> fft(F, k);
     for (auto &v : in) Gen.random(v);
-    Elt x = F.one;
     const Elt w = fft.root();
-    for (size_t i = 0; i < n; F.mulin(x, w), ++i) {
+    auto br = [&](size_t i) {
         size_t ib = 0;
         for (size_t j = 0, t = i; j < k; ++j, t >>= 1) ib = (ib << 1) | (t & 1);
+        return ib;
+    };
+    for (size_t i = 0; i < n; ++i) in_br[br(i)] = in[i];
+    std::vector<size_t> pts;
+    for (size_t i = 0; i < n && (k <= 10 || pts.size() < 64); ++i) {
+        Elt r;
+        pts.push_back(k <= 10 ? i : (size_t)(Gen.random(r) % n));
+    }
+    std::vector<Elt> y(in), z(in_br);
+    fft.FFT_direct(y.data());
+    fft.FFT_inverse(z.data());
+    bool ok = true;
+    for (size_t i : pts) {
+        Elt x = F.one, b = w;  // x = w^i
+        for (size_t e = i; e; e >>= 1) {
+            if (e & 1) F.mulin(x, b);
+            F.mulin(b, b);
+        }
         Elt acc = F.zero;
         for (size_t t = n; t-- > 0;) {
             F.mulin(acc, x);
             F.addin(acc, in[t]);
         }
-        out[i] = acc;
-        out_br[ib] = acc;
-        in_br[ib] = in[i];
+        ok = ok && y[br(i)] == acc && z[i] == acc;
     }
-    std::vector<Elt> y(in);
-    fft.FFT_direct(y.data());
-    bool ok = y == out_br;
-    y = in_br;
-    fft.FFT_inverse(y.data());
-    ok = ok && y == out;
     Elt wi = fft.invroot(), one = F.one;
     F.mulin(wi, w);
     ok = ok && wi == one;
