@@ -220,6 +220,23 @@ int choose_method(int method, size_t m, size_t k, size_t n, unsigned L)
     return big ? LBX_METHOD_INT8_MFMA : LBX_METHOD_PLAIN_U64;
 }
 
+// The plain form's operand layout: point-major blocks from the split
+// transforms (each operand word read once per output tile) for large matrices,
+// polfirst transform output (one point per lane, operands re-read once per
+// 8-wide register tile) otherwise.  Measured on gfx950 (2013265921, whole
+// products): 128^3 at 2^14 points 5.58 ms point-major against 7.23 ms; 128^3 at
+// 2^11 0.865 against 0.863 ms; 64^3 at 2^13 0.749 against 0.679 ms; 32^3 at 2^13
+// 0.289 against 0.192 ms -- the split transforms cost more than the re-reads
+// below ~100 rows.  LBX_PLAIN_LAYOUT = "point" or "entry" overrides the choice
+// (measurement and tests).
+bool plain_point_major(size_t m, size_t k, size_t n)
+{
+    const char *e = std::getenv("LBX_PLAIN_LAYOUT");
+    if (e && !std::strcmp(e, "point")) return true;
+    if (e && !std::strcmp(e, "entry")) return false;
+    return m >= 96 && n >= 96 && k >= 64;
+}
+
 // Bytes of head -> tail intermediate per chunk.  The default, 2 GiB, gives one
 // chunk per operand up to 128x128 entries of 2^14 points: smaller chunks
 // measured slower there (the hand-off through the last-level cache gains less
@@ -340,7 +357,8 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     }
     const uint32_t inv_n = invmod_h((uint32_t)(N % p), p);
     method = choose_method(method, m, k, n, L);
-    const bool split = method == LBX_METHOD_INT8_MFMA && L >= 8 && L <= (unsigned)kMaxLdsLog;
+    const bool int8 = method == LBX_METHOD_INT8_MFMA;
+    const bool split = L >= 8 && L <= (unsigned)kMaxLdsLog && (int8 || plain_point_major(m, k, n));
 
     // ---- buffers ----
     void *al = nullptr, *bl = nullptr, *cmat = nullptr, *scr = nullptr, *ahat = nullptr, *bhat = nullptr,
@@ -392,8 +410,9 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     const bool timing = ws.timing();
     Stages st(ctx, s, timing);
     if (split) {
-        // split transforms: the last six forward stages write per-point limb
-        // blocks and the first six inverse stages read the point-major product
+        // split transforms: the last six forward stages write per-point blocks
+        // (digits for the INT8 form, words for the plain one) and the first
+        // six inverse stages read the point-major product
         const unsigned BL = kSplitBL;
         uint32_t *slot[2] = {(uint32_t *)scr, (uint32_t *)scr + (nslots - 1) * slot_words};
         Pipe fp{ctx, s, fwd_pipe};
@@ -414,20 +433,26 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
                 }
                 LBX_TRY(fp.handoff(chunk));
                 {
-                    KernelTimer kt(ctx, fp.consumer(), op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b",
+                    KernelTimer kt(ctx, fp.consumer(),
+                                   int8 ? (op == 0 ? "kernel dif_tail_limbs a" : "kernel dif_tail_limbs b")
+                                        : (op == 0 ? "kernel dif_tail_words a" : "kernel dif_tail_words b"),
                                    timing);
                     LBX_TRY_HIP(launch_dif_tail_limbs(buf, (uint8_t *)(op == 0 ? al : bl), rows, r0, nr, valid, k,
-                                                      op == 0, L, BL, op == 0 ? twa : twb, p, fp.consumer()));
+                                                      op == 0, L, BL, op == 0 ? twa : twb, p, fp.consumer(), int8));
                 }
                 LBX_TRY(fp.done(chunk));
             }
         }
         LBX_TRY(fp.end(chunk));
         st.mark("direct FFT_DIF");
-        {
+        if (int8) {
             KernelTimer kt(ctx, s, "kernel pw_mfma", timing);
             LBX_TRY_HIP(launch_pointwise_mfma((const uint8_t *)al, (const uint8_t *)bl, (uint32_t *)cmat, m, k, n, N,
                                               p, inv_n, s));
+        } else {
+            KernelTimer kt(ctx, s, "kernel pw_plain_pm", timing);
+            LBX_TRY_HIP(launch_pointwise_plain_pm((const uint32_t *)al, (const uint32_t *)bl, (uint32_t *)cmat, m, k,
+                                                  n, N, p, inv_n, s));
         }
         st.mark("Pointwise mult");
         Pipe ip{ctx, s, inv_pipe};

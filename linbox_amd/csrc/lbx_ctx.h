@@ -220,6 +220,8 @@ void batch_release(lbx_ctx *ctx);
 // kernels, the context lock held; LBX_ERR_UNSUPPORTED (nothing launched) when a
 // job is outside their range.
 lbx_status batch_products(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, hipStream_t s);
+// Whether the plain form takes point-major operands (split transforms) for m x k x n.
+bool plain_point_major(size_t m, size_t k, size_t n);
 // The pointwise form LBX_METHOD_AUTO resolves to for m x k x n at 2^L points.
 int choose_method(int method, size_t m, size_t k, size_t n, unsigned L);
 

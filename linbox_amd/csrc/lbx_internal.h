@@ -86,9 +86,11 @@ struct EntryMap {
 constexpr unsigned kSplitBL = 6;
 hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const EntryMap &map, unsigned L, unsigned BL,
                                    const uint2 *tw, uint32_t p, uint32_t *out, hipStream_t s);
+// With limbs = false the tail writes the centred values (int32) instead of
+// their digits, in the same block geometry ([rpad][kpad] words per point).
 hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows_total, size_t row0, size_t rows_chunk,
                                  size_t rows_valid, size_t kdim, bool row_major, unsigned L, unsigned BL,
-                                 const uint2 *tw, uint32_t p, hipStream_t s);
+                                 const uint2 *tw, uint32_t p, hipStream_t s, bool limbs = true);
 hipError_t launch_dit_head(const uint32_t *Cm, size_t entries, size_t e_first, size_t e_count, unsigned L,
                            unsigned BL, const uint2 *tw, uint32_t p, uint32_t *Y, hipStream_t s);
 hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned L, unsigned BL, const uint2 *tw,
@@ -111,6 +113,11 @@ hipError_t launch_to_limbs(const uint32_t *X, uint8_t *Y, size_t rows, size_t kd
 // INT8-MFMA pointwise: per point, C[t][i*n+j] (point-major) from limb blocks.
 hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t *Cmat, size_t m, size_t k,
                                  size_t n, size_t npts, uint32_t p, uint32_t scale, hipStream_t s);
+// Plain-form pointwise on point-major blocks of centred int32 values (the
+// geometry of limb_block_bytes: A as [rpad(m)][kpad(k)], B^T as [rpad(n)][kpad(k)]
+// words per point, padding zero): C[t][i*n+j] = scale * sum_q A[i][q] B^T[j][q] mod p.
+hipError_t launch_pointwise_plain_pm(const uint32_t *Aw, const uint32_t *Bw, uint32_t *Cmat, size_t m, size_t k,
+                                     size_t n, size_t npts, uint32_t p, uint32_t scale, hipStream_t s);
 // Point-major [t][e] -> polfirst [e][t] transpose of `entries` x npts words.
 hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries, size_t npts, hipStream_t s);
 

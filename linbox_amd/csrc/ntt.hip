@@ -222,8 +222,9 @@ __device__ __forceinline__ void dif_block_stages(uint32_t *row, int sub, const u
 
 // Last BL DIF stages of block blockIdx.x for the 64 entries of limb row
 // blockIdx.z, inner chunk blockIdx.y; writes that 256-byte run of every
-// point's limb block.
-template <int L, int BL>
+// point's block: four planes of 64 signed 8-bit digits (LIMBS), or the 64
+// centred values as int32 words (the point-major operands of the plain form).
+template <int L, int BL, bool LIMBS>
 __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32_t *__restrict__ X,
                                                                     uint8_t *__restrict__ Y, int rows_valid, int row0,
                                                                     int kdim, int row_major,
@@ -261,22 +262,32 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
 #pragma unroll
     for (int u = 0; u < 16 / B::TPE; ++u)
 #pragma unroll
-        for (int w = 0; w < B::TPE; ++w) row[B::TPE * (sub + B::TPE * u) + w] = to_limbs(x[u * B::TPE + w], p);
+        for (int w = 0; w < B::TPE; ++w)
+            row[B::TPE * (sub + B::TPE * u) + w] =
+                LIMBS ? to_limbs(x[u * B::TPE + w], p) : (uint32_t)centre(x[u * B::TPE + w], p);
     __syncthreads();
-    // point t = b*P + lam: 16-byte chunk ch of the run = plane ch>>2, entries
-    // 16(ch&3)..+15 (byte j of word i holds entry 16(ch&3) + 4i + j)
+    // point t = b*P + lam: with LIMBS, 16-byte chunk ch of the run = plane
+    // ch>>2, entries 16(ch&3)..+15 (byte j of word i holds entry 16(ch&3) + 4i + j);
+    // without, chunk ch = the words of entries 4ch..4ch+3
 #pragma unroll 2
     for (int it = 0; it < 4; ++it) {
         const int id = tid + T * it;
         const int lam = id >> 4, ch = id & 15;
-        const int sft = 8 * (ch >> 2);
-        const uint32_t *col = sm + (16 * (ch & 3)) * B::STRIDE + lam;
         uint32_t w4[4];
+        if constexpr (LIMBS) {
+            const int sft = 8 * (ch >> 2);
+            const uint32_t *col = sm + (16 * (ch & 3)) * B::STRIDE + lam;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            w4[i] = ((col[(4 * i) * B::STRIDE] >> sft) & 0xffu) | (((col[(4 * i + 1) * B::STRIDE] >> sft) & 0xffu) << 8) |
-                    (((col[(4 * i + 2) * B::STRIDE] >> sft) & 0xffu) << 16) |
-                    (((col[(4 * i + 3) * B::STRIDE] >> sft) & 0xffu) << 24);
+            for (int i = 0; i < 4; ++i)
+                w4[i] = ((col[(4 * i) * B::STRIDE] >> sft) & 0xffu) |
+                        (((col[(4 * i + 1) * B::STRIDE] >> sft) & 0xffu) << 8) |
+                        (((col[(4 * i + 2) * B::STRIDE] >> sft) & 0xffu) << 16) |
+                        (((col[(4 * i + 3) * B::STRIDE] >> sft) & 0xffu) << 24);
+        } else {
+            const uint32_t *col = sm + (4 * ch) * B::STRIDE + lam;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) w4[i] = col[i * B::STRIDE];
+        }
         const size_t t = (size_t)b * P + lam;
         *reinterpret_cast<uint4 *>(Y + t * block_bytes + ((size_t)(row0 + r) * kchunks + c) * 256 + ch * 16) =
             make_uint4(w4[0], w4[1], w4[2], w4[3]);
@@ -754,10 +765,10 @@ hipError_t launch_ntt_inverse_tail(const uint32_t *in, size_t entries, unsigned 
 
 template <int L, int BL>
 static hipError_t tail_launch(dim3 grid, const uint32_t *X, uint8_t *Y, int rows_valid, int row0, int kdim,
-                              int row_major, const uint2 *tw, uint32_t p, size_t block_bytes, int kchunks,
+                              int row_major, const uint2 *tw, uint32_t p, size_t block_bytes, int kchunks, bool limbs,
                               hipStream_t s)
 {
-    auto k = dif_tail_limbs_kernel<L, BL>;
+    auto k = limbs ? dif_tail_limbs_kernel<L, BL, true> : dif_tail_limbs_kernel<L, BL, false>;
     hipError_t e = allow_lds(k, Blk<BL>::LDS_BYTES);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, grid, dim3(Blk<BL>::T), Blk<BL>::LDS_BYTES, s, X, Y, rows_valid, row0, kdim, row_major,
@@ -778,7 +789,7 @@ static hipError_t dit_head_launch(dim3 grid, const uint32_t *Cm, size_t entries,
 
 hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows_total, size_t row0, size_t rows_chunk,
                                  size_t rows_valid, size_t kdim, bool row_major, unsigned L, unsigned BL,
-                                 const uint2 *tw, uint32_t p, hipStream_t s)
+                                 const uint2 *tw, uint32_t p, hipStream_t s, bool limbs)
 {
     const size_t rpad = (rows_total + 31) & ~(size_t)31, kchunks = (kdim + 63) / 64;
     if (rows_chunk == 0) return hipSuccess;
@@ -791,7 +802,7 @@ hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows_tota
 #define LBX_CASE(l, bl)                                                                                            \
     case l * 16 + bl:                                                                                              \
         return tail_launch<l, bl>(grid, X, Y, (int)rows_valid, (int)row0, (int)kdim, row_major ? 1 : 0, tw, p,     \
-                                  block_bytes, (int)kchunks, s);
+                                  block_bytes, (int)kchunks, limbs, s);
         LBX_CASE(8, 6) LBX_CASE(9, 6) LBX_CASE(10, 6) LBX_CASE(11, 6) LBX_CASE(12, 6) LBX_CASE(13, 6) LBX_CASE(14, 6)
         LBX_CASE(15, 6)
 #undef LBX_CASE

@@ -432,6 +432,96 @@ __global__ void __launch_bounds__(256, 2) pw_mfma_rega_kernel(const uint8_t *__r
     }
 }
 
+// Plain form on point-major operands (the split transforms' centred int32
+// blocks, [rpad][kpad] words per point for A and for B^T): per point and per
+// output tile of (16 RI) x (16 RJ), the 256 threads stage 64-deep slices of
+// the A and B^T rows in LDS (row stride 68 words, 16-byte aligned) and each
+// thread accumulates RI x RJ outputs i = i0 + ti + 16 ii, j = j0 + tj + 16 jj
+// in signed 64-bit registers (one v_mad_i64_i32 per product), reading four
+// k-steps of its rows as one 16-byte LDS load and folding after those four
+// products (plain_s_fold allows six).  Every operand word is read from HBM once
+// per output tile.
+constexpr int kPmKc = 64;       // k-depth of one LDS stage
+constexpr int kPmStride = 68;   // LDS words per staged row
+
+template <int RI, int RJ>
+__global__ void __launch_bounds__(256, 2) pw_plain_pm_kernel(const uint32_t *__restrict__ Aw,
+                                                             const uint32_t *__restrict__ Bw,
+                                                             uint32_t *__restrict__ Cmat, int m, int n, int mpad,
+                                                             int npad, int kpad, size_t npts, uint32_t p, PlainSConst pc)
+{
+    constexpr int TM = 16 * RI, TN = 16 * RJ;
+    __shared__ uint4 lds4[(TM + TN) * kPmStride / 4];
+    uint32_t *lA = reinterpret_cast<uint32_t *>(lds4);
+    uint32_t *lB = lA + TM * kPmStride;
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int tiles_m = (mpad + TM - 1) / TM, tiles_n = (npad + TN - 1) / TN, tiles = tiles_m * tiles_n;
+    const size_t a_words = (size_t)mpad * kpad, b_words = (size_t)npad * kpad;
+    for (size_t work = blockIdx.x; work < npts * (size_t)tiles; work += gridDim.x) {
+        const size_t t = work / tiles;
+        const int tile = (int)(work - t * tiles);
+        const int i0 = (tile / tiles_n) * TM, j0 = (tile % tiles_n) * TN;
+        const uint32_t *A = Aw + t * a_words, *B = Bw + t * b_words;
+        int64_t acc[RI][RJ];
+#pragma unroll
+        for (int ii = 0; ii < RI; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < RJ; ++jj) acc[ii][jj] = 0;
+        for (int k0 = 0; k0 < kpad; k0 += kPmKc) {
+            __syncthreads();  // the previous stage (or tile) is done with the LDS rows
+            // 16 chunks of 16 bytes per staged row; rows past the padded operand read as zero
+            for (int c = tid; c < (TM + TN) * 16; c += 256) {
+                const int r = c >> 4, ch = c & 15;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (r < TM) {
+                    if (i0 + r < mpad)
+                        v = *reinterpret_cast<const uint4 *>(A + (size_t)(i0 + r) * kpad + k0 + 4 * ch);
+                    *reinterpret_cast<uint4 *>(lA + r * kPmStride + 4 * ch) = v;
+                } else {
+                    const int rb = r - TM;
+                    if (j0 + rb < npad)
+                        v = *reinterpret_cast<const uint4 *>(B + (size_t)(j0 + rb) * kpad + k0 + 4 * ch);
+                    *reinterpret_cast<uint4 *>(lB + rb * kPmStride + 4 * ch) = v;
+                }
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int q4 = 0; q4 < kPmKc / 4; ++q4) {
+                uint4 a[RI], b[RJ];
+#pragma unroll
+                for (int ii = 0; ii < RI; ++ii)
+                    a[ii] = *reinterpret_cast<const uint4 *>(lA + (ti + 16 * ii) * kPmStride + 4 * q4);
+#pragma unroll
+                for (int jj = 0; jj < RJ; ++jj)
+                    b[jj] = *reinterpret_cast<const uint4 *>(lB + (tj + 16 * jj) * kPmStride + 4 * q4);
+                static_assert(kPlainSFold >= 4, "four products between folds");
+#pragma unroll
+                for (int ii = 0; ii < RI; ++ii) {
+                    const int32_t a0 = (int32_t)a[ii].x, a1 = (int32_t)a[ii].y, a2 = (int32_t)a[ii].z,
+                                  a3 = (int32_t)a[ii].w;
+#pragma unroll
+                    for (int jj = 0; jj < RJ; ++jj) {
+                        int64_t v = acc[ii][jj];
+                        v += (int64_t)a0 * (int32_t)b[jj].x;
+                        v += (int64_t)a1 * (int32_t)b[jj].y;
+                        v += (int64_t)a2 * (int32_t)b[jj].z;
+                        v += (int64_t)a3 * (int32_t)b[jj].w;
+                        acc[ii][jj] = plain_s_fold(v, pc);
+                    }
+                }
+            }
+        }
+        uint32_t *Ct = Cmat + t * (size_t)m * n;
+#pragma unroll
+        for (int ii = 0; ii < RI; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < RJ; ++jj) {
+                const int i = i0 + ti + 16 * ii, j = j0 + tj + 16 * jj;
+                if (i < m && j < n) Ct[(size_t)i * n + j] = plain_s_finish(acc[ii][jj], p, pc);
+            }
+    }
+}
+
 __global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__restrict__ X, uint32_t *__restrict__ Y,
                                                              size_t entries, size_t npts)
 {
@@ -531,6 +621,46 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
     hipLaunchKernelGGL(pw_mfma_kernel, dim3((unsigned)npts), dim3(256), 0, s, Al, Bl, Cmat, (int)m, (int)n,
                        (int)kchunks, (int)mpad, (int)npad, limb_block_bytes(m, k), limb_block_bytes(n, k), p, mc);
     return hipGetLastError();
+}
+
+template <int RI, int RJ>
+static hipError_t pm_launch(const uint32_t *Aw, const uint32_t *Bw, uint32_t *Cmat, size_t m, size_t n, size_t mpad,
+                            size_t npad, size_t kpad, size_t npts, uint32_t p, const PlainSConst &pc, hipStream_t s)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const size_t tiles = ((mpad + 16 * RI - 1) / (16 * RI)) * ((npad + 16 * RJ - 1) / (16 * RJ));
+    const size_t work = npts * tiles;
+    const size_t grid = work < (size_t)cus * 2 ? work : (size_t)cus * 2;
+    hipLaunchKernelGGL((pw_plain_pm_kernel<RI, RJ>), dim3((unsigned)grid), dim3(256), 0, s, Aw, Bw, Cmat, (int)m,
+                       (int)n, (int)mpad, (int)npad, (int)kpad, npts, p, pc);
+    return hipGetLastError();
+}
+
+hipError_t launch_pointwise_plain_pm(const uint32_t *Aw, const uint32_t *Bw, uint32_t *Cmat, size_t m, size_t k,
+                                     size_t n, size_t npts, uint32_t p, uint32_t scale, hipStream_t s)
+{
+    const size_t mpad = rpad_of(m), npad = rpad_of(n), kpad = kpad_of(k);
+    if (mpad > 0x7fffffff || npad > 0x7fffffff || kpad > 0x7fffffff) return hipErrorInvalidValue;
+    const PlainSConst pc = plain_s_const(p, scale);
+    // the widest tile each dimension fills: 16 x {2, 4, 8} rows and columns
+    const int ri = mpad >= 128 ? 8 : (mpad >= 64 ? 4 : 2), rj = npad >= 128 ? 8 : (npad >= 64 ? 4 : 2);
+    switch (ri * 16 + rj) {
+    case 8 * 16 + 8: return pm_launch<8, 8>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 8 * 16 + 4: return pm_launch<8, 4>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 4 * 16 + 8: return pm_launch<4, 8>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 4 * 16 + 4: return pm_launch<4, 4>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 8 * 16 + 2: return pm_launch<8, 2>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 2 * 16 + 8: return pm_launch<2, 8>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 4 * 16 + 2: return pm_launch<4, 2>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    case 2 * 16 + 4: return pm_launch<2, 4>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    default: return pm_launch<2, 2>(Aw, Bw, Cmat, m, n, mpad, npad, kpad, npts, p, pc, s);
+    }
 }
 
 hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries, size_t npts, hipStream_t s)

@@ -96,4 +96,49 @@ LBX_HD uint32_t plain_finish(uint64_t acc, uint32_t p, const PlainConst &pc)
     return add_mod(shoup_mul(ah, pc.c1, pc.c1p, p), shoup_mul(al, pc.c0, pc.c0p, p), p);
 }
 
+// Plain form on centred operands: a, b in (-p/2, p/2] as int32, so |a b| < 2^60
+// and acc += a*b is one v_mad_i64_i32.  After at most kPlainSFold = 6 products
+// the bits above 2^60 are folded back (plain_s_fold: acc mod 2^60 +
+// floor(acc / 2^60) (2^60 mod p), one v_mad_i64_i32 more, which leaves
+// |acc| < 2^60 + 2^34, and 2^60 + 2^34 + 6 * 2^60 < 2^63); at the end
+// acc + off (off a multiple of p >= 2^63, so the sum is in [0, 2^64)) is
+// reduced as in plain_finish.
+struct PlainSConst {
+    uint32_t c0, c0p, c1, c1p;  // scale and 2^32 scale mod p, with Shoup constants
+    uint32_t f60;               // 2^60 mod p
+    uint64_t off;               // multiple of p, >= 2^63
+};
+constexpr uint32_t kPlainSFold = 6;
+
+inline PlainSConst plain_s_const(uint32_t p, uint32_t scale)
+{
+    PlainSConst pc;
+    const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
+    pc.c0 = scale % p;
+    pc.c1 = (uint32_t)((uint64_t)r32 * pc.c0 % p);
+    pc.c0p = shoup_pre(pc.c0, p);
+    pc.c1p = shoup_pre(pc.c1, p);
+    pc.f60 = (uint32_t)(((uint64_t)1 << 60) % p);
+    const uint64_t two63 = (uint64_t)1 << 63;
+    pc.off = ((two63 + p - 1) / p) * p;
+    return pc;
+}
+
+// x in [0,p) -> its centred representative in (-p/2, p/2].
+LBX_HD int32_t centre(uint32_t x, uint32_t p) { return (int32_t)(x > (p >> 1) ? x - p : x); }
+
+LBX_HD int64_t plain_s_fold(int64_t acc, const PlainSConst &pc)
+{
+    constexpr int64_t kLow60 = ((int64_t)1 << 60) - 1;
+    // floor(acc / 2^60) is in [-8, 7] and f60 < 2^31: a 32 x 32 -> 64-bit signed product
+    return (acc & kLow60) + (int64_t)(int32_t)(acc >> 60) * (int64_t)(int32_t)pc.f60;
+}
+
+LBX_HD uint32_t plain_s_finish(int64_t acc, uint32_t p, const PlainSConst &pc)
+{
+    const uint64_t u = (uint64_t)acc + pc.off;
+    const uint32_t al = (uint32_t)u, ah = (uint32_t)(u >> 32);
+    return add_mod(shoup_mul(ah, pc.c1, pc.c1p, p), shoup_mul(al, pc.c0, pc.c0p, p), p);
+}
+
 }  // namespace lbx

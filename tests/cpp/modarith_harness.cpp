@@ -44,4 +44,23 @@ uint32_t lbx_test_plain_dot(const uint32_t *a, const uint32_t *b, int n, uint32_
 }
 
 uint32_t lbx_test_plain_fold_count(uint32_t p) { return plain_const(p, 1).fold; }
+
+// The point-major plain form: centred operands, signed 64-bit accumulation
+// folded every kPlainSFold products, as the kernel does.
+uint32_t lbx_test_plain_s_dot(const uint32_t *a, const uint32_t *b, int n, uint32_t p, uint32_t scale)
+{
+    const PlainSConst pc = plain_s_const(p, scale);
+    int64_t acc = 0;
+    uint32_t cnt = 0;
+    for (int q = 0; q < n; ++q) {
+        acc += (int64_t)centre(a[q], p) * centre(b[q], p);
+        if (++cnt == kPlainSFold) {
+            cnt = 0;
+            acc = plain_s_fold(acc, pc);
+        }
+    }
+    return plain_s_finish(acc, p, pc);
+}
+
+int32_t lbx_test_centre(uint32_t x, uint32_t p) { return centre(x, p); }
 }

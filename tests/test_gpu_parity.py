@@ -521,3 +521,33 @@ def test_mul_fft_seam_values(ctx):
                 assert np.array_equal(dc.download(), want)
                 # the operands are left intact (the reference transforms its copies in place)
                 assert np.array_equal(da.download(), xa) and np.array_equal(db.download(), xb)
+
+
+def test_mul_plain_point_major(ctx, monkeypatch):
+    """The plain form on point-major operands (split transforms writing
+    canonical words, LDS-tiled 64-bit accumulation): output tiles of
+    32..128 rows and columns, several tiles per point, inner dimensions that
+    are not multiples of the 64-deep stage, head/tail chunking, and both
+    operand layouts forced on the same shapes."""
+    cases = [(2013265921, 33, 100, 70, 200, 57), (998244353, 128, 128, 128, 40, 41),
+             (469762049, 200, 70, 40, 300, 260), (2013265921, 32, 16, 32, 1025, 1000),
+             (167772161, 40, 130, 260, 129, 128), (2013265921, 64, 64, 64, 4096, 4000)]
+    for (p, m, k, n, sa, sb) in cases:
+        rng = np.random.default_rng(m * k * n + sa)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        assert st == 0
+        for layout in ("point", "entry"):
+            monkeypatch.setenv("LBX_PLAIN_LAYOUT", layout)
+            say("plain point-major", layout, p, m, k, n, sa, sb)
+            got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, lbx.METHOD_PLAIN_U64)
+            assert np.array_equal(got.reshape(-1), ref), layout
+    monkeypatch.setenv("LBX_PLAIN_LAYOUT", "point")
+    monkeypatch.setenv("LBX_CHUNK_BYTES", str(1 << 20))
+    for (p, m, k, n, sa, sb) in [(2013265921, 37, 20, 45, 300, 200), (998244353, 5, 3, 7, 500, 400)]:
+        rng = np.random.default_rng(m + n)
+        a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
+        st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
+        say("plain point-major chunked", p, m, k, n, sa, sb)
+        got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, lbx.METHOD_PLAIN_U64)
+        assert np.array_equal(got.reshape(-1), ref)

@@ -36,7 +36,8 @@ def lib():
                        ("to_limbs", [u32, u32]), ("powmod", [u32, u64, u32]), ("invmod", [u32, u32]),
                        ("combine", [ctypes.POINTER(ctypes.c_int32), u32, u32]),
                        ("plain_dot", [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.c_int, u32, u32]),
-                       ("plain_fold_count", [u32])]:
+                       ("plain_fold_count", [u32]),
+                       ("plain_s_dot", [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.c_int, u32, u32])]:
         f = getattr(L, "lbx_test_" + name)
         f.argtypes = args
         f.restype = u32
@@ -115,6 +116,32 @@ def test_plain_accumulation_never_wraps(lib):
             for scale in [1, pow(1 << 13, p - 2, p)]:
                 want = n * (p - 1) ** 2 * scale % p
                 assert lib.lbx_test_plain_dot(a, a, n, p, scale) == want, (p, n, scale)
+
+
+def test_plain_signed_accumulation_never_wraps(lib):
+    """The point-major plain form: centred operands in (-p/2, p/2], signed
+    64-bit sums folded every six products.  Runs of the largest products of
+    either sign (both operands at -(p-1)/2, or one at +(p-1)/2) and random
+    values, over lengths around the fold period and the k of the workloads."""
+    lib.lbx_test_centre.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+    lib.lbx_test_centre.restype = ctypes.c_int32
+    rng = random.Random(4)
+    for p in PRIMES:
+        h = (p - 1) // 2
+        assert lib.lbx_test_centre(h, p) == h and lib.lbx_test_centre(h + 1, p) == h + 1 - p
+        assert lib.lbx_test_centre(0, p) == 0 and lib.lbx_test_centre(p - 1, p) == -1
+        lo = h + 1  # centred value -(p-1)/2
+        for n in [1, 5, 6, 7, 12, 13, 128, 129, 4097]:
+            for (x, y) in [(lo, lo), (h, lo), (h, h)]:
+                a = (ctypes.c_uint32 * n)(*([x] * n))
+                b = (ctypes.c_uint32 * n)(*([y] * n))
+                for scale in [1, pow(1 << 14, p - 2, p)]:
+                    assert lib.lbx_test_plain_s_dot(a, b, n, p, scale) == n * x * y * scale % p, (p, n, x, y)
+            xs = [rng.randrange(p) for _ in range(n)]
+            ys = [rng.randrange(p) for _ in range(n)]
+            a = (ctypes.c_uint32 * n)(*xs)
+            b = (ctypes.c_uint32 * n)(*ys)
+            assert lib.lbx_test_plain_s_dot(a, b, n, p, 1) == sum(u * v for u, v in zip(xs, ys)) % p
 
 
 def test_host_helpers_wide_modulus(lib):
