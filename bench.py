@@ -267,9 +267,22 @@ def run_batch(steps=5):
         return e0.elapsed_ms(e1) / reps
 
     batch = lambda: lbx.mul_batch(jobs, ctx=ctx, stream=stream.value)
-    batch()  # builds the plan, tunes tiles, captures the graph
+    batch()  # builds the plan, tunes tiles and forms, captures the graph
     stream.synchronize()
     ms_batch = timed(batch, steps)
+    tuning = json.loads(lbx.lib().lbx_batch_tuning(ctx.handle).decode())
+    # share of the batch's multiply-adds (pts m k n per job) done on INT8 MFMA
+    macs, macs_mfma = 0, 0
+    for j in jobs:
+        deg = j["sa"] + j["sb"] - 2
+        pts = 1 << max(1, deg.bit_length())
+        mac = pts * j["m"] * j["k"] * j["n"]
+        macs += mac
+        d = min(j["m"], j["k"], j["n"])
+        cls = None if d < 16 or j["k"] > 4096 else ("min(m,k,n) in [16,32)" if d < 32 else "min(m,k,n)>=32")
+        choice = tuning["int8_capable"].get(cls) if cls else None
+        if choice and choice.get("form") == "int8_mfma" and 4 <= deg.bit_length() <= 13:
+            macs_mfma += mac
 
     def singles():
         for j in jobs:
@@ -287,7 +300,8 @@ def run_batch(steps=5):
             "graph_ms": round(ms_batch, 3), "graph_products_per_s": round(1024 / (ms_batch * 1e-3), 1),
             "per_product_ms": round(ms_single, 3),
             "per_product_products_per_s": round(1024 / (ms_single * 1e-3), 1),
-            "speedup": round(ms_single / ms_batch, 2)}
+            "speedup": round(ms_single / ms_batch, 2), "tuning": tuning,
+            "mac_share_on_int8_mfma": round(macs_mfma / macs, 4)}
 
 
 def staged_bytes(m, k, n, sa, sb):

@@ -168,12 +168,23 @@ typedef struct lbx_job {
 
 /* Independent products of uneven shapes, degrees and primes, the mix that
  * OrderBasis::PM_Basis issues through PolynomialMatrixMulDomain::mul
- * (order-basis.h:219-326 -> :297).  Every job must use an FFT prime below 2^31
- * with 2^ceil(log2(sa+sb-1)) | p-1.  All forward transforms, all pointwise
- * products and all inverse transforms run as three persistent work-queue
- * launches replayed from a cached hipGraph; results equal one
- * lbx_polymat_mul_u32 per job. */
+ * (order-basis.h:219-326 -> :297).  Jobs with an FFT prime below 2^31 whose
+ * transform length 2^ceil(log2(sa+sb-1)) is in [2^4, 2^13] run in the batch
+ * kernels: persistent work-queue launches for the forward transforms (one per
+ * length), the pointwise products (plain form, and for jobs with every
+ * dimension >= 16 the INT8 MFMA form when the autotuner measured it faster)
+ * and the inverse transforms, replayed from a cached hipGraph; other jobs take
+ * the single-product routes after it.  Results equal one lbx_polymat_mul_u32
+ * per job.  method is used for the jobs outside the batch kernels. */
 lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream);
+
+/* No counterpart in the reference (which has no autotuner): the batch
+ * scheduler's tuned choices on this context as a JSON object -- the plain
+ * form's register tile per size class, and per class of INT8-capable jobs the
+ * pointwise form, the points per MFMA work item and the measured times of
+ * both forms (null where not tuned yet).  The string belongs to the context
+ * and stays valid until the next call of this function on it. */
+const char *lbx_batch_tuning(lbx_ctx *ctx);
 
 /* Middle product, PolynomialMatrixFFTMulDomain::midproduct
  * (matpoly-mult-fft-wordsize.inl:90-127 -> matpoly-mult-fft-wordsize-fast.inl:209-316;

@@ -28,7 +28,7 @@ EXPORTED = [
     "lbx_ntt_forward_u32", "lbx_ntt_inverse_u32", "lbx_polymat_mul_mp", "lbx_polymat_mul_batch",
     "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_order_basis_pm_u32",
     "lbx_order_basis_m_u32", "lbx_polymat_copy_u32", "lbx_profile_enable", "lbx_profile_get",
-    "lbx_polymat_mul_u32_noalloc",
+    "lbx_polymat_mul_u32_noalloc", "lbx_batch_tuning",
 ]
 
 
@@ -91,8 +91,10 @@ def lib():
         L.lbx_profile_enable.argtypes = [vp, i32]
         L.lbx_profile_get.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]
         L.lbx_profile_get.restype = i32
+        L.lbx_batch_tuning.argtypes = [vp]
+        L.lbx_batch_tuning.restype = ctypes.c_char_p
         for name in EXPORTED:
-            if name not in ("lbx_strerror", "lbx_profile_get"):
+            if name not in ("lbx_strerror", "lbx_profile_get", "lbx_batch_tuning"):
                 getattr(L, name).restype = i32
         _lib = L
     return _lib

@@ -8,12 +8,17 @@
 // branches), one pointwise kernel for every job.  The whole sequence is
 // captured once per batch shape into a hipGraph and replayed; a call only
 // uploads the job descriptors (the caller's pointers) and launches the graph.
-// The pointwise register tile of each job is picked by an autotuner that times
-// the candidate tiles on the first batch of each size class.
+// An autotuner picks, on the first batch of each size class, the plain form's
+// register tile (2x2, 4x4, 8x8) and, for jobs whose three dimensions are all
+// at least 16, the pointwise form: the plain form, or the INT8 MFMA form
+// (a limb-conversion kernel, the MFMA products with 1, 4 or 16 points per
+// work item, and a transpose back to the polfirst layout the inverse
+// transforms read, on a branch parallel to the plain jobs' pointwise kernel).
 // Jobs the batch kernels do not cover (transform length outside [2^4, 2^13],
 // no 2^L-th root of unity mod p, p >= 2^31, constant matrices) run through the
 // single-product routes after the graph, on the same stream.
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -24,15 +29,18 @@
 namespace lbx {
 
 struct BatchPlan {
-    std::vector<BatchJob> jobs;            // host copy: workspace slices, twiddles, tiles
+    std::vector<BatchJob> jobs;            // host copy: workspace slices, twiddles, tiles, forms
     BatchJob *jobs_d = nullptr;
     NttItem *ntt_d = nullptr;              // forward items grouped by L, then inverse items grouped by L
-    PwItem *pw_d = nullptr;
+    PwItem *pw_d = nullptr;                // plain-form jobs
+    LimbItem *limb_d = nullptr;            // INT8 jobs
+    MfmaItem *mfma_d = nullptr;
+    TrItem *tr_d = nullptr;
     uint32_t *counters = nullptr;          // one per kernel
-    void *ws = nullptr;                    // transform workspace of every job
+    void *ws = nullptr;                    // transform workspace of every job, then INT8 blocks
     std::vector<unsigned> Ls;              // lengths present
     std::vector<uint32_t> fwd_off, fwd_cnt, inv_off, inv_cnt;
-    uint32_t npw = 0;
+    uint32_t npw = 0, nlimb = 0, nmfma = 0, ntr = 0;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
 };
@@ -80,6 +88,42 @@ int cus()
             n = 256;
     }
     return n;
+}
+
+// Class of an INT8-capable job (every dimension >= 16, one exact int32 chunk
+// of inner products): 0 for a smallest dimension below 32, 1 above; -1 when
+// the job stays in the plain form.
+int int8_class(const lbx_job &j)
+{
+    const size_t d = std::min(j.m, std::min(j.k, j.n));
+    if (d < 16 || j.k > 4096) return -1;
+    return d < 32 ? 0 : 1;
+}
+
+size_t rpad32(size_t x) { return (x + 31) & ~(size_t)31; }
+size_t limb_bytes(size_t rows, size_t kdim) { return rpad32(rows) * ((kdim + 63) & ~(size_t)63) * 4; }
+
+void limb_items_for(const BatchJob &J, uint32_t job, std::vector<LimbItem> &out)
+{
+    const uint32_t npts = 1u << J.L, kchunks = (J.k + 63) / 64;
+    for (uint32_t op = 0; op < 2; ++op) {
+        const uint32_t rows = (uint32_t)rpad32(op == 0 ? J.m : J.n);
+        for (uint32_t t0 = 0; t0 < npts; t0 += 64)
+            for (uint32_t rc = 0; rc < rows * kchunks; ++rc) out.push_back(LimbItem{job, op, t0, rc});
+    }
+}
+
+void mfma_items_for(const BatchJob &J, uint32_t job, uint32_t ppi, std::vector<MfmaItem> &out)
+{
+    const uint32_t npts = 1u << J.L;
+    for (uint32_t t0 = 0; t0 < npts; t0 += ppi) out.push_back(MfmaItem{job, t0, std::min(ppi, npts - t0), 0});
+}
+
+void tr_items_for(const BatchJob &J, uint32_t job, std::vector<TrItem> &out)
+{
+    const uint32_t npts = 1u << J.L, entries = J.m * J.n;
+    for (uint32_t t0 = 0; t0 < npts; t0 += 64)
+        for (uint32_t e0 = 0; e0 < entries; e0 += 64) out.push_back(TrItem{job, e0, t0, 0});
 }
 
 void pw_items_for(const BatchJob &J, uint32_t job, std::vector<PwItem> &out)
@@ -160,8 +204,124 @@ lbx_status tune_tiles(lbx_ctx *ctx, BatchPlan *plan, const std::vector<int> &cls
     return LBX_OK;
 }
 
+// Times a sequence of work-queue launches (each with its own counter) twice
+// and returns the second run's time.
+template <class F>
+lbx_status time_launches(F launches, int nctr, hipStream_t s, float *ms)
+{
+    void *cnt = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    lbx_status st = from_hip(hipMalloc(&cnt, 4 * nctr + 16));
+    if (st == LBX_OK) st = from_hip(hipEventCreate(&e0));
+    if (st == LBX_OK) st = from_hip(hipEventCreate(&e1));
+    for (int rep = 0; rep < 2 && st == LBX_OK; ++rep) {  // first run warms up
+        st = from_hip(launch_zero_counters((uint32_t *)cnt, nctr, s));
+        if (st == LBX_OK) st = from_hip(hipEventRecord(e0, s));
+        if (st == LBX_OK) st = launches((uint32_t *)cnt);
+        if (st == LBX_OK) st = from_hip(hipEventRecord(e1, s));
+        if (st == LBX_OK) st = from_hip(hipEventSynchronize(e1));
+        if (st == LBX_OK) st = from_hip(hipEventElapsedTime(ms, e0, e1));
+    }
+    (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (cnt) (void)hipFree(cnt);
+    return st;
+}
+
+// For each class of INT8-capable jobs: times their pointwise stage in the plain
+// form (tuned register tiles) and in the INT8 form (limb conversion, MFMA
+// products with 1, 4 or 16 points per work item, transpose back) and keeps
+// the fastest form and item size.  LBX_BATCH_FORM=plain|int8 forces the form.
+lbx_status tune_forms(lbx_ctx *ctx, BatchPlan *plan, const std::vector<int> &icls, hipStream_t s)
+{
+    const char *force = std::getenv("LBX_BATCH_FORM");
+    for (int c = 0; c < 2; ++c) {
+        std::vector<uint32_t> members;
+        for (uint32_t j = 0; j < plan->jobs.size(); ++j)
+            if (icls[j] == c) members.push_back(j);
+        if (members.empty()) continue;
+        if (force && (!std::strcmp(force, "plain") || !std::strcmp(force, "int8"))) {
+            for (uint32_t j : members) {
+                plan->jobs[j].form = std::strcmp(force, "int8") == 0 ? 1u : 0u;
+            }
+            if (ctx->ppi_tuned[c] < 0) ctx->ppi_tuned[c] = 4;
+            continue;
+        }
+        if (ctx->form_tuned[c] < 0) {
+            std::vector<BatchJob> jobs = plan->jobs;
+            void *jd = nullptr;
+            lbx_status st = upload(&jd, jobs.data(), jobs.size() * sizeof(BatchJob), s);
+            // plain
+            std::vector<PwItem> pw;
+            for (uint32_t j : members) pw_items_for(jobs[j], j, pw);
+            void *pwd = nullptr;
+            if (st == LBX_OK) st = upload(&pwd, pw.data(), pw.size() * sizeof(PwItem), s);
+            float plain_ms = 1e30f;
+            if (st == LBX_OK)
+                st = time_launches([&](uint32_t *cnt) {
+                    return from_hip(launch_batch_pointwise((const BatchJob *)jd, (const PwItem *)pwd,
+                                                           (uint32_t)pw.size(), cnt,
+                                                           (unsigned)std::min<size_t>(pw.size(), (size_t)cus() * 8),
+                                                           s));
+                }, 1, s, &plain_ms);
+            // INT8 with each item size
+            std::vector<LimbItem> li;
+            std::vector<TrItem> tr;
+            for (uint32_t j : members) {
+                limb_items_for(jobs[j], j, li);
+                tr_items_for(jobs[j], j, tr);
+            }
+            void *lid = nullptr, *trd = nullptr;
+            if (st == LBX_OK) st = upload(&lid, li.data(), li.size() * sizeof(LimbItem), s);
+            if (st == LBX_OK) st = upload(&trd, tr.data(), tr.size() * sizeof(TrItem), s);
+            float best = 1e30f;
+            int best_ppi = 4;
+            for (uint32_t ppi : {1u, 4u, 16u}) {
+                std::vector<MfmaItem> mi;
+                for (uint32_t j : members) mfma_items_for(jobs[j], j, ppi, mi);
+                void *mid = nullptr;
+                if (st == LBX_OK) st = upload(&mid, mi.data(), mi.size() * sizeof(MfmaItem), s);
+                float ms = 1e30f;
+                if (st == LBX_OK)
+                    st = time_launches([&](uint32_t *cnt) {
+                        lbx_status r = from_hip(launch_batch_limbs(
+                            (const BatchJob *)jd, (const LimbItem *)lid, (uint32_t)li.size(), cnt,
+                            (unsigned)std::min<size_t>(li.size(), (size_t)cus() * 8), s));
+                        if (r == LBX_OK)
+                            r = from_hip(launch_batch_mfma((const BatchJob *)jd, (const MfmaItem *)mid,
+                                                           (uint32_t)mi.size(), cnt + 1,
+                                                           (unsigned)std::min<size_t>(mi.size(), (size_t)cus() * 4),
+                                                           s));
+                        if (r == LBX_OK)
+                            r = from_hip(launch_batch_point_to_entry(
+                                (const BatchJob *)jd, (const TrItem *)trd, (uint32_t)tr.size(), cnt + 2,
+                                (unsigned)std::min<size_t>(tr.size(), (size_t)cus() * 8), s));
+                        return r;
+                    }, 3, s, &ms);
+                if (mid) (void)hipFree(mid);
+                if (ms < best) {
+                    best = ms;
+                    best_ppi = (int)ppi;
+                }
+            }
+            (void)hipStreamSynchronize(s);
+            for (void *x : {jd, pwd, lid, trd})
+                if (x) (void)hipFree(x);
+            if (st != LBX_OK) return st;
+            ctx->form_tuned[c] = best < plain_ms ? 1 : 0;
+            ctx->ppi_tuned[c] = best_ppi;
+            ctx->form_ms[c][0] = plain_ms;
+            ctx->form_ms[c][1] = best;
+        }
+        for (uint32_t j : members) plan->jobs[j].form = (uint32_t)ctx->form_tuned[c];
+    }
+    return LBX_OK;
+}
+
 // Capture: zero counters; forward kernels of every length as parallel
-// branches; pointwise; inverse kernels as parallel branches.
+// branches; pointwise (plain jobs, and INT8 jobs on a parallel branch:
+// limbs, MFMA, transpose); inverse kernels as parallel branches.
 lbx_status capture(lbx_ctx *ctx, BatchPlan *plan)
 {
     if (!ctx->aux) LBX_TRY_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
@@ -181,14 +341,37 @@ lbx_status capture(lbx_ctx *ctx, BatchPlan *plan)
         st = from_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         capturing = st == LBX_OK;
     }
-    const uint32_t nctr = (uint32_t)(2 * nb + 1);
+    const uint32_t nctr = (uint32_t)(2 * nb + 4);
     if (st == LBX_OK) st = from_hip(launch_zero_counters(plan->counters, (int)nctr, cs));
+    hipStream_t i8 = nullptr;
+    if (st == LBX_OK && plan->nmfma) st = from_hip(hipStreamCreateWithFlags(&i8, hipStreamNonBlocking));
     for (int phase = 0; phase < 3 && st == LBX_OK; ++phase) {
         if (phase == 1) {
+            uint32_t *ctr = plan->counters + 2 * nb;
+            hipEvent_t fork = nullptr, join = nullptr;
+            if (plan->nmfma) {  // INT8 jobs on a parallel branch
+                if ((st = new_event(&fork)) != LBX_OK) break;
+                if ((st = from_hip(hipEventRecord(fork, cs))) != LBX_OK) break;
+                st = from_hip(hipStreamWaitEvent(i8, fork, 0));
+                if (st == LBX_OK)
+                    st = from_hip(launch_batch_limbs(plan->jobs_d, plan->limb_d, plan->nlimb, ctr + 1,
+                                                     (unsigned)std::min<size_t>(plan->nlimb, (size_t)cus() * 8), i8));
+                if (st == LBX_OK)
+                    st = from_hip(launch_batch_mfma(plan->jobs_d, plan->mfma_d, plan->nmfma, ctr + 2,
+                                                    (unsigned)std::min<size_t>(plan->nmfma, (size_t)cus() * 4), i8));
+                if (st == LBX_OK)
+                    st = from_hip(launch_batch_point_to_entry(plan->jobs_d, plan->tr_d, plan->ntr, ctr + 3,
+                                                              (unsigned)std::min<size_t>(plan->ntr, (size_t)cus() * 8),
+                                                              i8));
+            }
             const unsigned grid = (unsigned)std::min<size_t>(plan->npw, (size_t)cus() * 8);
-            if (plan->npw)
-                st = from_hip(launch_batch_pointwise(plan->jobs_d, plan->pw_d, plan->npw, plan->counters + 2 * nb,
-                                                     grid, cs));
+            if (st == LBX_OK && plan->npw)
+                st = from_hip(launch_batch_pointwise(plan->jobs_d, plan->pw_d, plan->npw, ctr, grid, cs));
+            if (st == LBX_OK && plan->nmfma) {
+                if ((st = new_event(&join)) != LBX_OK) break;
+                st = from_hip(hipEventRecord(join, i8));
+                if (st == LBX_OK) st = from_hip(hipStreamWaitEvent(cs, join, 0));
+            }
             continue;
         }
         const bool inv = phase == 2;
@@ -219,6 +402,7 @@ lbx_status capture(lbx_ctx *ctx, BatchPlan *plan)
     for (auto e : evs) (void)hipEventDestroy(e);
     for (auto s : br)
         if (s) (void)hipStreamDestroy(s);
+    if (i8) (void)hipStreamDestroy(i8);
     return st;
 }
 
@@ -230,19 +414,33 @@ lbx_status build_plan(lbx_ctx *ctx, const std::vector<lbx_job> &jobs, const std:
         if (plan->jobs_d) (void)hipFree(plan->jobs_d);
         if (plan->ntt_d) (void)hipFree(plan->ntt_d);
         if (plan->pw_d) (void)hipFree(plan->pw_d);
+        if (plan->limb_d) (void)hipFree(plan->limb_d);
+        if (plan->mfma_d) (void)hipFree(plan->mfma_d);
+        if (plan->tr_d) (void)hipFree(plan->tr_d);
         if (plan->counters) (void)hipFree(plan->counters);
         if (plan->ws) (void)hipFree(plan->ws);
         delete plan;
         return st;
     };
-    // workspace slices and descriptors
+    // workspace slices and descriptors: polfirst transforms of every job, then
+    // the limb blocks and point-major product of every INT8-capable job
     size_t words = 0;
-    std::vector<size_t> off(jobs.size());
+    std::vector<size_t> off(jobs.size()), ioff(jobs.size(), 0);
+    std::vector<int> icls(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) {
         off[j] = words;
         words += (jobs[j].m * jobs[j].k + jobs[j].k * jobs[j].n + jobs[j].m * jobs[j].n) << Ls[j];
     }
-    lbx_status st = from_hip(hipMalloc(&plan->ws, words * 4 + 16));
+    size_t bytes = words * 4;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        icls[j] = int8_class(jobs[j]);
+        if (icls[j] < 0) continue;
+        bytes = (bytes + 255) & ~(size_t)255;
+        ioff[j] = bytes;
+        bytes += ((limb_bytes(jobs[j].m, jobs[j].k) + limb_bytes(jobs[j].n, jobs[j].k)) << Ls[j]) +
+                 ((jobs[j].m * jobs[j].n * 4) << Ls[j]);
+    }
+    lbx_status st = from_hip(hipMalloc(&plan->ws, bytes + 256));
     if (st != LBX_OK) return fail(st);
     std::vector<int> cls(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) {
@@ -267,11 +465,21 @@ lbx_status build_plan(lbx_ctx *ctx, const std::vector<lbx_job> &jobs, const std:
         if ((st = primitive_root(x.p, L, &root)) != LBX_OK) return fail(st);
         if ((st = twiddles(ctx, x.p, L, root, &J.twf)) != LBX_OK) return fail(st);
         if ((st = twiddles(ctx, x.p, L, invroot(x.p, L, root), &J.twi)) != LBX_OK) return fail(st);
-        J.pc = plain_const(x.p, invmod_h((uint32_t)(((uint64_t)1 << L) % x.p), x.p));
+        const uint32_t inv_n = invmod_h((uint32_t)(((uint64_t)1 << L) % x.p), x.p);
+        J.pc = plain_const(x.p, inv_n);
+        J.form = 0;
+        if (icls[j] >= 0) {
+            uint8_t *base = (uint8_t *)plan->ws + ioff[j];
+            J.alimb = base;
+            J.blimb = base + (limb_bytes(x.m, x.k) << L);
+            J.cmat = (uint32_t *)(J.blimb + (limb_bytes(x.n, x.k) << L));
+            J.mc = mfma_const(x.p, inv_n);
+        }
         plan->jobs.push_back(J);
         cls[j] = size_class(x);
     }
     if ((st = tune_tiles(ctx, plan, cls, s)) != LBX_OK) return fail(st);
+    if ((st = tune_forms(ctx, plan, icls, s)) != LBX_OK) return fail(st);
     // transform items grouped by length, biggest operands first within a length
     for (unsigned L = kBatchMaxL; L >= (unsigned)kBatchMinL; --L)
         if (std::find(Ls.begin(), Ls.end(), L) != Ls.end()) plan->Ls.push_back(L);
@@ -300,8 +508,23 @@ lbx_status build_plan(lbx_ctx *ctx, const std::vector<lbx_job> &jobs, const std:
         return cx > cy;
     });
     std::vector<PwItem> pw;
-    for (uint32_t j : order) pw_items_for(plan->jobs[j], j, pw);
+    std::vector<LimbItem> li;
+    std::vector<MfmaItem> mi;
+    std::vector<TrItem> tr;
+    for (uint32_t j : order) {
+        const BatchJob &J = plan->jobs[j];
+        if (J.form == 1) {
+            limb_items_for(J, j, li);
+            mfma_items_for(J, j, (uint32_t)std::max(1, ctx->ppi_tuned[icls[j]]), mi);
+            tr_items_for(J, j, tr);
+        } else {
+            pw_items_for(J, j, pw);
+        }
+    }
     plan->npw = (uint32_t)pw.size();
+    plan->nlimb = (uint32_t)li.size();
+    plan->nmfma = (uint32_t)mi.size();
+    plan->ntr = (uint32_t)tr.size();
     void *p;
     if ((st = upload(&p, plan->jobs.data(), plan->jobs.size() * sizeof(BatchJob), s)) != LBX_OK) return fail(st);
     plan->jobs_d = (BatchJob *)p;
@@ -309,7 +532,13 @@ lbx_status build_plan(lbx_ctx *ctx, const std::vector<lbx_job> &jobs, const std:
     plan->ntt_d = (NttItem *)p;
     if ((st = upload(&p, pw.data(), pw.size() * sizeof(PwItem), s)) != LBX_OK) return fail(st);
     plan->pw_d = (PwItem *)p;
-    if ((st = from_hip(hipMalloc(&p, 4 * (2 * plan->Ls.size() + 1) + 16))) != LBX_OK) return fail(st);
+    if ((st = upload(&p, li.data(), li.size() * sizeof(LimbItem), s)) != LBX_OK) return fail(st);
+    plan->limb_d = (LimbItem *)p;
+    if ((st = upload(&p, mi.data(), mi.size() * sizeof(MfmaItem), s)) != LBX_OK) return fail(st);
+    plan->mfma_d = (MfmaItem *)p;
+    if ((st = upload(&p, tr.data(), tr.size() * sizeof(TrItem), s)) != LBX_OK) return fail(st);
+    plan->tr_d = (TrItem *)p;
+    if ((st = from_hip(hipMalloc(&p, 4 * (2 * plan->Ls.size() + 4) + 16))) != LBX_OK) return fail(st);
     plan->counters = (uint32_t *)p;
     if ((st = from_hip(hipStreamSynchronize(s))) != LBX_OK) return fail(st);
     if ((st = capture(ctx, plan)) != LBX_OK) {
@@ -343,6 +572,9 @@ void batch_release(lbx_ctx *ctx)
         if (p->jobs_d) (void)hipFree(p->jobs_d);
         if (p->ntt_d) (void)hipFree(p->ntt_d);
         if (p->pw_d) (void)hipFree(p->pw_d);
+        if (p->limb_d) (void)hipFree(p->limb_d);
+        if (p->mfma_d) (void)hipFree(p->mfma_d);
+        if (p->tr_d) (void)hipFree(p->tr_d);
         if (p->counters) (void)hipFree(p->counters);
         if (p->ws) (void)hipFree(p->ws);
         delete p;
@@ -390,6 +622,37 @@ lbx_status batch_products(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, hipSt
 }  // namespace lbx
 
 using namespace lbx;
+
+extern "C" const char *lbx_batch_tuning(lbx_ctx *ctx)
+{
+    if (!ctx) return "{}";
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    static const char *tiles[3] = {"2x2", "4x4", "8x8"};
+    std::string js = "{\"plain_tile\": {";
+    const char *cls[3] = {"max(m,n)<=4", "max(m,n)<=16", "max(m,n)>16"};
+    for (int c = 0; c < 3; ++c) {
+        js += std::string(c ? ", " : "") + "\"" + cls[c] + "\": ";
+        js += ctx->tile_tuned[c] < 0 ? std::string("null") : std::string("\"") + tiles[ctx->tile_tuned[c]] + "\"";
+    }
+    js += "}, \"int8_capable\": {";
+    const char *icl[2] = {"min(m,k,n) in [16,32)", "min(m,k,n)>=32"};
+    for (int c = 0; c < 2; ++c) {
+        js += std::string(c ? ", " : "") + "\"" + icl[c] + "\": ";
+        if (ctx->form_tuned[c] < 0) {
+            js += "null";
+            continue;
+        }
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "{\"form\": \"%s\", \"points_per_mfma_item\": %d, \"plain_ms\": %.4f, \"int8_ms\": %.4f}",
+                      ctx->form_tuned[c] ? "int8_mfma" : "plain_u64", ctx->ppi_tuned[c], ctx->form_ms[c][0],
+                      ctx->form_ms[c][1]);
+        js += buf;
+    }
+    js += "}}";
+    ctx->tuning_json = js;
+    return ctx->tuning_json.c_str();
+}
 
 extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream)
 {

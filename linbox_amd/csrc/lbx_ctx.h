@@ -32,6 +32,13 @@ struct lbx_ctx {
     std::map<std::string, lbx::MpPlan *> mp_plans;
     std::map<std::string, lbx::BatchPlan *> batch_plans;
     int tile_tuned[3] = {-1, -1, -1};  // batch pointwise tile per size class, -1 = not tuned yet
+    // batch INT8 choice per class of INT8-capable jobs (min(m,k,n) in [16,32) / >= 32):
+    // form (0 plain, 1 INT8 MFMA) and points per MFMA item, -1 = not tuned yet
+    int form_tuned[2] = {-1, -1};
+    int ppi_tuned[2] = {-1, -1};
+    // last tuning measurements (ms) of those classes: plain, best INT8
+    float form_ms[2][2] = {{0, 0}, {0, 0}};
+    std::string tuning_json;  // lbx_batch_tuning's last answer
     hipStream_t aux = nullptr;         // second stream for chunk pipelining (default priority)
     std::vector<hipEvent_t> events;    // pool of dependency events
     // stream order of the shared workspace: the stream of the last call and

@@ -127,17 +127,34 @@ hipError_t launch_point_to_entry(const uint32_t *X, uint32_t *Y, size_t entries,
 // slices of the batch workspace for the transforms.
 struct BatchJob {
     uint32_t p, L, m, k, n, sa, sb, sc;
-    uint32_t tile;            // pointwise register tile: 0 = 2x2, 1 = 4x4, 2 = 8x8
-    uint32_t pad_;
+    uint32_t tile;            // plain form: pointwise register tile, 0 = 2x2, 1 = 4x4, 2 = 8x8
+    uint32_t form;            // pointwise form: 0 plain, 1 INT8 MFMA
     const uint32_t *a, *b;
     uint32_t *c;
     uint32_t *ahat, *bhat, *chat;  // m*k, k*n, m*n rows of 2^L words
     const uint2 *twf, *twi;        // twiddles of the root and of its inverse
-    PlainConst pc;                 // epilogue constants with scale 2^-L
+    PlainConst pc;                 // plain epilogue constants with scale 2^-L
+    uint8_t *alimb, *blimb;        // INT8: per-point limb blocks of A and B^T (limb_block_bytes each)
+    uint32_t *cmat;                // INT8: point-major product [t][i*n+j]
+    MfmaConst mc;                  // INT8 epilogue constants with scale 2^-L
 };
 // Transform work item: entries [e0, e0+count) of operand op (0 a, 1 b, 2 c).
 struct NttItem {
     uint32_t job, op, e0, count;
+};
+// INT8 batch work items.  Limbs: the 64-point x 64-entry tile of operand op
+// (0 a, 1 b) of one job at points t0.., limb row r and inner chunk c
+// (rc = r * kchunks + c).  MFMA: points [t0, t0+count) of one job.
+// Transpose: entries [e0, e0+64) x points [t0, t0+64) of one job's
+// point-major product into its polfirst chat.
+struct LimbItem {
+    uint32_t job, op, t0, rc;
+};
+struct MfmaItem {
+    uint32_t job, t0, count, pad_;
+};
+struct TrItem {
+    uint32_t job, e0, t0, pad_;
 };
 // Pointwise work item: 256 points from pt0 of output tile (ti, tj).
 struct PwItem {
@@ -154,6 +171,16 @@ hipError_t launch_batch_ntt(unsigned L, const BatchJob *jobs, const NttItem *ite
                             uint32_t *counter, bool inverse, unsigned grid, hipStream_t s);
 hipError_t launch_batch_pointwise(const BatchJob *jobs, const PwItem *items, uint32_t nitems, uint32_t *counter,
                                   unsigned grid, hipStream_t s);
+// The INT8 stage of a batch's INT8 jobs, each a persistent work-queue kernel
+// over its items (counter at zero on entry): NTT output -> limb blocks, the
+// MFMA products (inner dimension at most 4096, one exact int32 chunk), and the
+// point-major -> polfirst transpose of the result for the inverse transforms.
+hipError_t launch_batch_limbs(const BatchJob *jobs, const LimbItem *items, uint32_t nitems, uint32_t *counter,
+                              unsigned grid, hipStream_t s);
+hipError_t launch_batch_mfma(const BatchJob *jobs, const MfmaItem *items, uint32_t nitems, uint32_t *counter,
+                             unsigned grid, hipStream_t s);
+hipError_t launch_batch_point_to_entry(const BatchJob *jobs, const TrItem *items, uint32_t nitems,
+                                       uint32_t *counter, unsigned grid, hipStream_t s);
 // Zero `n` counters with a kernel (graph-safe, no memset node).
 hipError_t launch_zero_counters(uint32_t *counters, int n, hipStream_t s);
 

@@ -123,16 +123,15 @@ __global__ void zero_counters_kernel(uint32_t *c, int n)
 // (16 consecutive digits) is one 16-byte load.
 __host__ __device__ inline size_t rpad_of(size_t rows) { return (rows + 31) & ~(size_t)31; }
 __host__ __device__ inline size_t kpad_of(size_t kdim) { return (kdim + 63) & ~(size_t)63; }
+__device__ __forceinline__ size_t limb_block_bytes_dev(size_t rows, size_t kdim) { return rpad_of(rows) * kpad_of(kdim) * 4; }
 
-__global__ void __launch_bounds__(256) to_limbs_kernel(const uint32_t *__restrict__ X, uint8_t *__restrict__ Y,
-                                                       int rows, int kdim, int row_major, size_t npts, uint32_t p,
-                                                       size_t block_bytes, int kchunks)
+// One 64-point x 64-entry tile: points t0.., inner chunk c, row r of the
+// padded range; reads 64 consecutive points of 64 entries (coalesced) and
+// writes, for each point, that row's 256-byte run of the limb block.
+__device__ __forceinline__ void limbs_tile(const uint32_t *__restrict__ X, uint8_t *__restrict__ Y, int rows,
+                                           int kdim, int row_major, size_t npts, uint32_t p, size_t block_bytes,
+                                           int kchunks, size_t t0, int c, int r, int tid, uint32_t (&tile)[64][65])
 {
-    __shared__ uint32_t tile[64][65];
-    const int tid = threadIdx.x;
-    const size_t t0 = (size_t)blockIdx.x * 64;
-    const int c = blockIdx.y;   // 64-wide chunk of the inner dimension
-    const int r = blockIdx.z;   // row (padded range)
     const int q0 = c * 64;
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
@@ -161,6 +160,15 @@ __global__ void __launch_bounds__(256) to_limbs_kernel(const uint32_t *__restric
     }
 }
 
+__global__ void __launch_bounds__(256) to_limbs_kernel(const uint32_t *__restrict__ X, uint8_t *__restrict__ Y,
+                                                       int rows, int kdim, int row_major, size_t npts, uint32_t p,
+                                                       size_t block_bytes, int kchunks)
+{
+    __shared__ uint32_t tile[64][65];
+    limbs_tile(X, Y, rows, kdim, row_major, npts, p, block_bytes, kchunks, (size_t)blockIdx.x * 64, blockIdx.y,
+               blockIdx.z, threadIdx.x, tile);
+}
+
 __device__ __forceinline__ uint32_t mfma_combine(const v4i (&acc)[7], int r, uint32_t p, const MfmaConst &mc)
 {
     int32_t D[7];
@@ -171,22 +179,15 @@ __device__ __forceinline__ uint32_t mfma_combine(const v4i (&acc)[7], int r, uin
 
 constexpr int kMaxKbPerChunk = 64;  // 64 steps of 64 = 4096 inner products per exact int32 chunk
 
-// One workgroup per point; each wave computes 32x32 output tiles as 2x2
-// 16x16 MFMA tiles, for each of the 16 digit pairs.
-__global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict__ Al, const uint8_t *__restrict__ Bl,
-                                                      uint32_t *__restrict__ Cmat, int m, int n, int kchunks,
-                                                      int mpad, int npad, size_t a_bytes, size_t b_bytes, uint32_t p,
-                                                      MfmaConst mc)
+// One wave: the 32x32 output tile (i0, j0) of one point from its limb
+// blocks A and B (read from HBM/L2), as 2x2 16x16 MFMA tiles for each of the
+// 16 digit pairs, k in exact int32 chunks of 4096.
+__device__ __forceinline__ void mfma_tile_global(const uint8_t *__restrict__ A, const uint8_t *__restrict__ B,
+                                                 uint32_t *__restrict__ Ct, int m, int n, int kchunks, int i0,
+                                                 int j0, uint32_t p, const MfmaConst &mc, int l16, int lg)
 {
-    const size_t t = blockIdx.x;
-    const uint8_t *A = Al + t * a_bytes;
-    const uint8_t *B = Bl + t * b_bytes;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int l16 = lane & 15, lg = lane >> 4;
-    const int tiles_n = npad / 32, ntiles = (mpad / 32) * tiles_n;
     const size_t row_bytes = (size_t)kchunks * 256;
-    for (int tile = wave; tile < ntiles; tile += 4) {
-        const int i0 = (tile / tiles_n) * 32, j0 = (tile % tiles_n) * 32;
+    {
         uint32_t res[2][2][4];
 #pragma unroll
         for (int ai = 0; ai < 2; ++ai)
@@ -233,7 +234,6 @@ __global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict_
                     for (int r = 0; r < 4; ++r)
                         res[ai][bj][r] = add_mod(res[ai][bj][r], mfma_combine(acc[ai][bj], r, p, mc), p);
         }
-        uint32_t *Ct = Cmat + t * (size_t)m * n;
 #pragma unroll
         for (int ai = 0; ai < 2; ++ai)
 #pragma unroll
@@ -244,6 +244,20 @@ __global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict_
                     if (i < m && j < n) Ct[(size_t)i * n + j] = res[ai][bj][r];
                 }
     }
+}
+
+// One workgroup per point; each wave computes 32x32 output tiles.
+__global__ void __launch_bounds__(256) pw_mfma_kernel(const uint8_t *__restrict__ Al, const uint8_t *__restrict__ Bl,
+                                                      uint32_t *__restrict__ Cmat, int m, int n, int kchunks,
+                                                      int mpad, int npad, size_t a_bytes, size_t b_bytes, uint32_t p,
+                                                      MfmaConst mc)
+{
+    const size_t t = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tiles_n = npad / 32, ntiles = (mpad / 32) * tiles_n;
+    for (int tile = wave; tile < ntiles; tile += 4)
+        mfma_tile_global(Al + t * a_bytes, Bl + t * b_bytes, Cmat + t * (size_t)m * n, m, n, kchunks,
+                         (tile / tiles_n) * 32, (tile % tiles_n) * 32, p, mc, lane & 15, lane >> 4);
 }
 
 // LDS-staged form, used for the other shapes whose A limb block fits in 64 KiB
@@ -522,12 +536,11 @@ __global__ void __launch_bounds__(256, 2) pw_plain_pm_kernel(const uint32_t *__r
     }
 }
 
-__global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__restrict__ X, uint32_t *__restrict__ Y,
-                                                             size_t entries, size_t npts)
+// Entries e0.. x points t0.. of a point-major [t][e] array into polfirst [e][t].
+__device__ __forceinline__ void point_to_entry_tile(const uint32_t *__restrict__ X, uint32_t *__restrict__ Y,
+                                                    size_t entries, size_t npts, size_t e0, size_t t0, int tid,
+                                                    uint32_t (&tile)[64][65])
 {
-    __shared__ uint32_t tile[64][65];
-    const int tid = threadIdx.x;
-    const size_t t0 = (size_t)blockIdx.x * 64, e0 = (size_t)blockIdx.y * 64;
 #pragma unroll 4
     for (int it = 0; it < 16; ++it) {
         const int tl = it * 4 + tid / 64, el = tid % 64;
@@ -543,6 +556,83 @@ __global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__r
     }
 }
 
+__global__ void __launch_bounds__(256) point_to_entry_kernel(const uint32_t *__restrict__ X, uint32_t *__restrict__ Y,
+                                                             size_t entries, size_t npts)
+{
+    __shared__ uint32_t tile[64][65];
+    point_to_entry_tile(X, Y, entries, npts, (size_t)blockIdx.y * 64, (size_t)blockIdx.x * 64, threadIdx.x, tile);
+}
+
+// ---- INT8 stage of a batch: persistent work-queue kernels -------------------
+// Each 256-thread group takes an item index from the counter, makes it
+// wave-uniform (scalar loads of the job descriptor) and processes it.
+__device__ __forceinline__ uint32_t next_item(uint32_t *counter, uint32_t &slot)
+{
+    if (threadIdx.x == 0) slot = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(slot);
+    __syncthreads();  // everyone has the index (and is done with the previous item's LDS)
+    return it;
+}
+
+__global__ void __launch_bounds__(256) batch_limbs_kernel(const BatchJob *__restrict__ jobs,
+                                                          const LimbItem *__restrict__ items, uint32_t nitems,
+                                                          uint32_t *counter)
+{
+    __shared__ uint32_t tile[64][65];
+    __shared__ uint32_t slot;
+    for (;;) {
+        const uint32_t it = next_item(counter, slot);
+        if (it >= nitems) return;
+        const LimbItem w = items[it];
+        const BatchJob &J = jobs[w.job];
+        const bool isa = w.op == 0;
+        const int rows = isa ? J.m : J.n, kdim = J.k;
+        const int kchunks = (kdim + 63) / 64;
+        const size_t block_bytes = limb_block_bytes_dev(rows, kdim);
+        limbs_tile(isa ? J.ahat : J.bhat, isa ? J.alimb : J.blimb, rows, kdim, isa ? 1 : 0, (size_t)1 << J.L, J.p,
+                   block_bytes, kchunks, w.t0, (int)(w.rc % kchunks), (int)(w.rc / kchunks), threadIdx.x, tile);
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_mfma_kernel(const BatchJob *__restrict__ jobs,
+                                                         const MfmaItem *__restrict__ items, uint32_t nitems,
+                                                         uint32_t *counter)
+{
+    __shared__ uint32_t slot;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (;;) {
+        const uint32_t it = next_item(counter, slot);
+        if (it >= nitems) return;
+        const MfmaItem w = items[it];
+        const BatchJob &J = jobs[w.job];
+        const int mpad = (J.m + 31) & ~31, npad = (J.n + 31) & ~31, kchunks = (J.k + 63) / 64;
+        const int tiles_n = npad / 32, tiles = (mpad / 32) * tiles_n;
+        const size_t a_bytes = limb_block_bytes_dev(J.m, J.k), b_bytes = limb_block_bytes_dev(J.n, J.k);
+        for (uint32_t u = wave; u < w.count * (uint32_t)tiles; u += 4) {
+            const size_t t = w.t0 + u / tiles;
+            const int tile = (int)(u % tiles);
+            mfma_tile_global(J.alimb + t * a_bytes, J.blimb + t * b_bytes, J.cmat + t * (size_t)J.m * J.n, J.m, J.n,
+                             kchunks, (tile / tiles_n) * 32, (tile % tiles_n) * 32, J.p, J.mc, lane & 15, lane >> 4);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_point_to_entry_kernel(const BatchJob *__restrict__ jobs,
+                                                                   const TrItem *__restrict__ items, uint32_t nitems,
+                                                                   uint32_t *counter)
+{
+    __shared__ uint32_t tile[64][65];
+    __shared__ uint32_t slot;
+    for (;;) {
+        const uint32_t it = next_item(counter, slot);
+        if (it >= nitems) return;
+        const TrItem w = items[it];
+        const BatchJob &J = jobs[w.job];
+        point_to_entry_tile(J.cmat, J.chat, (size_t)J.m * J.n, (size_t)1 << J.L, w.e0, w.t0, threadIdx.x, tile);
+    }
+}
+
 }  // namespace
 
 
@@ -550,6 +640,27 @@ hipError_t launch_batch_pointwise(const BatchJob *jobs, const PwItem *items, uin
                                   unsigned grid, hipStream_t s)
 {
     hipLaunchKernelGGL(batch_pointwise_kernel, dim3(grid), dim3(256), 0, s, jobs, items, nitems, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_limbs(const BatchJob *jobs, const LimbItem *items, uint32_t nitems, uint32_t *counter,
+                              unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_limbs_kernel, dim3(grid), dim3(256), 0, s, jobs, items, nitems, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_mfma(const BatchJob *jobs, const MfmaItem *items, uint32_t nitems, uint32_t *counter,
+                             unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_mfma_kernel, dim3(grid), dim3(256), 0, s, jobs, items, nitems, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_point_to_entry(const BatchJob *jobs, const TrItem *items, uint32_t nitems,
+                                       uint32_t *counter, unsigned grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_point_to_entry_kernel, dim3(grid), dim3(256), 0, s, jobs, items, nitems, counter);
     return hipGetLastError();
 }
 

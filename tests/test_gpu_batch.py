@@ -138,3 +138,38 @@ def test_batch_1024_products(ctx):
     for i in range(0, 1024, 97):
         if jobs[i]["m"] * jobs[i]["k"] * jobs[i]["n"] * jobs[i]["sa"] <= 64 * 64 * 64 * 600:
             assert np.array_equal(got[i], oracle_ref(jobs[i])), i
+
+
+@pytest.mark.parametrize("form", ["int8", "plain", None])
+def test_batch_int8_jobs(form, monkeypatch):
+    """Jobs whose three dimensions are all >= 16 (16..64, several inner
+    dimensions, lengths 2^4..2^13) take the INT8 MFMA stage of the batch
+    (limb conversion, MFMA products, transpose) when forced to or when the
+    tuner measures it faster; every job against the oracle.  A fresh context
+    each time, so the form is decided anew."""
+    if form:
+        monkeypatch.setenv("LBX_BATCH_FORM", form)
+    rng = np.random.default_rng({"int8": 1, "plain": 2, None: 3}[form])
+    specs = []
+    for (m, k, n, d) in [(16, 16, 16, 7), (32, 16, 64, 100), (64, 64, 64, 300), (16, 48, 32, 2000), (64, 32, 16, 4000),
+                         (32, 32, 32, 20), (17, 33, 20, 60)]:
+        specs.append((int(rng.choice(PRIMES)), m, k, n, d + 1, d + 1 - int(rng.integers(0, 3))))
+    specs += [(998244353, 4, 8, 2, 30, 30), (469762049, 8, 16, 32, 100, 90)]  # plain-form jobs beside them
+    jobs = make_jobs(rng, specs)
+    c = lbx.Context(0)
+    try:
+        say("batch int8 jobs", form)
+        lbx.mul_batch(jobs, ctx=c)
+        synchronize()
+        tuning = lbx.lib().lbx_batch_tuning(c.handle).decode()
+        print("tuning:", tuning)
+        for j in jobs:
+            assert np.array_equal(j["dc"].download(), oracle_ref(j)), (j["p"], j["m"], j["k"], j["n"], j["sa"])
+        # a replay with new data through the cached plan and graph
+        refill(rng, jobs)
+        lbx.mul_batch(jobs, ctx=c)
+        synchronize()
+        for j in jobs:
+            assert np.array_equal(j["dc"].download(), oracle_ref(j))
+    finally:
+        c.close()
