@@ -21,9 +21,15 @@
 // coefficients over the original pivot rows, so that a row that ends at zero
 // gives its combination of pivot rows after one inverse of its scale.
 #include <algorithm>
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "lbx_ctx.h"
+
+// wave-wide 64-bit minimum and maximum of the device library (DPP reductions)
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_min_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
 
 namespace lbx {
 namespace {
@@ -87,8 +93,10 @@ struct Acc {
     }
 };
 
-__global__ void __launch_bounds__(256) gather_coeff_major(DevView v, size_t count, uint32_t *out)
+// Also clears the grid barrier's arrival count for the M_Basis launch that follows.
+__global__ void __launch_bounds__(256) gather_coeff_major(DevView v, size_t count, uint32_t *out, unsigned *bar)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) bar[0] = 0;
     const size_t entries = v.rows * v.cols, total = entries * count;
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
         const size_t t = i / entries, e = i - t * entries;
@@ -191,8 +199,9 @@ template <typename A, bool MONT>
 __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__restrict__ S, uint32_t *__restrict__ W,
                                                              DevView out, uint64_t *__restrict__ shift_io,
                                                              uint64_t *__restrict__ deg_out, int order, int m, int n,
-                                                             int stage, uint32_t p, uint64_t mu, uint32_t r64,
-                                                             uint32_t f62, uint32_t pinv, uint32_t r2m, uint32_t one)
+                                                             int stage, int out_count, uint32_t p, uint64_t mu,
+                                                             uint32_t r64, uint32_t f62, uint32_t pinv, uint32_t r2m,
+                                                             uint32_t one)
 {
     // one = 1 in the elimination's representation (2^32 mod p with MONT)
     auto mul = [&](uint32_t x, uint32_t y) -> uint32_t {
@@ -562,14 +571,483 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_kernel(const uint32_t *__r
         }
         __syncthreads();
     }
-    // sigma.resize(max_degree+1), shift2 = shift (:592-597)
+    // sigma.resize(max_degree+1), shift2 = shift (:592-597); out_count coefficients
+    // written when nonzero, zero above the degree
     const int md = s_maxdeg;
-    for (size_t o = tid; o < (size_t)(md + 1) * mm; o += kMbThreads) {
+    const size_t cnt = out_count ? (size_t)out_count : (size_t)md + 1;
+    for (size_t o = tid; o < cnt * mm; o += kMbThreads) {
         const size_t l = o / mm, e = o - l * mm, i = e / m, j = e - i * m;
-        out.base[view_index(out, e, l)] = W[(l * m + rowmap[i]) * m + j];
+        out.base[view_index(out, e, l)] = l <= (size_t)md ? W[(l * m + rowmap[i]) * m + j] : 0u;
     }
     for (int i = tid; i < m; i += kMbThreads) shift_io[i] = shift[i];
     if (tid == 0) *deg_out = (uint64_t)md;
+}
+
+// ---------------------------------------------------------------------------
+// M_Basis over several workgroups (orders up to kMcMaxOrder: the PM_Basis
+// base case).  Row operations on sigma act on every column alike, so sigma's
+// columns are split over G workgroups: group g keeps columns [g cw, g cw + cw)
+// of every coefficient of sigma, and rows g cw .. of the serie that those
+// columns meet, resident in LDS for the whole call.  Per step each group
+// forms its share of the discrepancy (the sum over its columns), the shares
+// are exchanged through global memory with one grid barrier, and every group
+// then runs the same sort, elimination and bookkeeping on the summed
+// discrepancy (so every group takes the same decisions) and applies the row
+// operations to its own columns.  The steps follow m_basis_kernel and the
+// reference's order-basis.h:346-600; the differences are in how they are
+// computed:
+//  * the shift sort handles one value at a time: the reference's selection
+//    sort places the elements of the smallest remaining value in order of
+//    position, and each element it displaces follows a chain of those
+//    positions, which one pass of the wave resolves;
+//  * each wave keeps its rows of [R | Z] in registers during elimination and
+//    stores a row only when it is that wave's candidate for the next pivot;
+//  * a x - b y takes one Montgomery reduction (a x + b (p - y) < 2^64 for
+//    p < 2^31), and rows carry no scale: every row below pivot j is
+//    multiplied by its entry a_j, so a row's scale is the product of the a_j
+//    of the pivots above it, read from a prefix product at the end.
+constexpr int kMcMaxOrder = 32;
+constexpr int kMcMaxGroups = 64;
+constexpr int kMcLdsWords = 36864;  // dynamic LDS words (the static arrays take the rest of the 160 KiB)
+constexpr unsigned kMcSpinLimit = 1u << 24;  // polls of the grid barrier before it reports a failure
+
+// a x - b y mod p on Montgomery forms, ny = p - y (odd p < 2^31)
+__device__ __forceinline__ uint32_t amb1(uint32_t a, uint32_t x, uint32_t b, uint32_t ny, uint32_t p, uint32_t pinv)
+{
+    const uint64_t t = (uint64_t)a * x + (uint64_t)b * ny;
+    const uint32_t q = (uint32_t)t * pinv;
+    const uint32_t r = (uint32_t)((t + (uint64_t)q * p) >> 32);
+    return r >= p ? r - p : r;
+}
+
+// All groups of the grid meet here for the `count`-th time.  bar[0] counts
+// arrivals, bar[1] is set when a poll limit runs out (the host reports the
+// call as failed, and later barriers no longer wait).  The shares exchanged
+// through the barrier are written and read with agent-scope atomics.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned count)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = count * gridDim.x;
+        unsigned polls = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (++polls == kMcSpinLimit) {
+                __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    }
+    __syncthreads();
+}
+
+// sum of x[j] y[j*ystride], len terms, on two independent accumulators
+__device__ __forceinline__ void dot2(AccSmall &a0, AccSmall &a1, const uint32_t *x, const uint32_t *y, int ystride,
+                                     int len, uint32_t f62)
+{
+    int j = 0;
+    for (; j + 3 < len; j += 4) {
+        const uint32_t x0 = x[j], x1 = x[j + 1], x2 = x[j + 2], x3 = x[j + 3];
+        const uint32_t y0 = y[j * ystride], y1 = y[(j + 1) * ystride], y2 = y[(j + 2) * ystride],
+                       y3 = y[(j + 3) * ystride];
+        a0.add2(x0, y0, x1, y1, f62);
+        a1.add2(x2, y2, x3, y3, f62);
+    }
+    for (; j < len; ++j) a0.add1(x[j], y[j * ystride], f62);
+}
+
+__device__ __forceinline__ uint32_t sum2(const AccSmall &a0, const AccSmall &a1, uint32_t p, uint64_t mu)
+{
+    return csub(a0.get(p, mu, 0) + a1.get(p, mu, 0), p);
+}
+
+// G = gridDim.x groups of kMbThreads threads; ROWS >= m / kMbWaves rows of the
+// elimination per wave.  S: serie, coefficient-major S[(t*m + j)*n + c]; part:
+// 2 x G x m x n words of exchanged discrepancy shares; bar: the grid barrier's
+// two words, zero at launch.  Dynamic LDS, in words: W (order+1) x m x cw
+// (W[(l*m + phys)*cw + jj], coefficient l of sigma's column g*cw+jj in
+// physical row phys), Sg order x cw x n (serie rows of the group's columns),
+// V m x (2n+1) (row r of the discrepancy under elimination, [R_r | Z_r],
+// logical order), D m x n (discrepancy before elimination, by physical row).
+// out_count: coefficients of sigma written (zero above the degree); 0 writes
+// degree+1 of them.
+template <int ROWS>
+__global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
+    const uint32_t *__restrict__ S, DevView out, uint64_t *__restrict__ shift_io, uint64_t *__restrict__ deg_out,
+    uint32_t *__restrict__ part, unsigned *__restrict__ bar, int order, int m, int n, int cw, int out_count,
+    uint32_t p, uint64_t mu, uint32_t f62, uint32_t pinv, uint32_t r2m, uint32_t one)
+{
+    const int G = gridDim.x, g = blockIdx.x, j0 = g * cw;
+    const int cols = m - j0 < cw ? m - j0 : cw;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int mn = m * n, vs = 2 * n + 1, cst = m * cw;  // cst: words per coefficient of the group's W
+    extern __shared__ uint32_t dyn[];
+    uint32_t *W = dyn;
+    uint32_t *Sg = W + (order + 1) * cst;
+    uint32_t *V = Sg + order * cw * n;
+    uint32_t *D = V + m * vs;
+    __shared__ uint64_t shift[kMbMaxM], degree[kMbMaxM], tmp64[kMbMaxM], tmp64b[kMbMaxM];
+    __shared__ int rowmap[kMbMaxM], tmpmap[kMbMaxM], oldrow[kMbMaxM], pivot_of[kMbMaxM], comp[kMbMaxM],
+        cidx[kMbMaxM], was_piv[kMbMaxM], qtab[kMbMaxM], pivrow[kMbMaxN];
+    __shared__ uint32_t pref[kMbMaxN + 1], rinv[kMbMaxM];
+    __shared__ unsigned cand[2][kMbWaves];  // each wave's candidate for the next pivot, row << 8 | column
+    __shared__ int s_maxdeg, s_nc;
+
+    for (int i = tid; i < (order + 1) * cst; i += kMbThreads) {
+        const int q = i / cw, jj = i - q * cw, ph = q % m, l = q / m;
+        W[i] = (l == 0 && ph == j0 + jj) ? 1u : 0u;
+    }
+    for (int i = tid; i < order * cw * n; i += kMbThreads) {
+        const int q = i / n, c = i - q * n, jj = q % cw, t = q / cw;
+        Sg[i] = jj < cols ? S[((size_t)t * m + j0 + jj) * n + c] : 0u;
+    }
+    for (int i = tid; i < m; i += kMbThreads) {
+        rowmap[i] = i;
+        shift[i] = shift_io[i];
+        degree[i] = 0;
+        was_piv[i] = 0;
+    }
+    if (tid == 0) {
+        s_maxdeg = 0;
+        pref[0] = one;
+    }
+    __syncthreads();
+    unsigned nbar = 0;  // grid barriers passed (the same count in every group)
+    const uint64_t below = (1ull << lane) - 1;
+
+    for (int k = 0; k < order; ++k) {
+        const int maxdeg = s_maxdeg, lim = k < maxdeg ? k : maxdeg;
+        // ---- 1. selection sort of the shift (order-basis.h:388-399), one value at a time ----
+        if (wave == 0) {
+            const bool v0 = lane < m, v1 = lane + 64 < m;
+            uint64_t k0 = v0 ? shift[lane] : 0, k1 = v1 ? shift[lane + 64] : 0;
+            int o0 = lane, o1 = lane + 64;
+            for (int start = 0; start < m - 1;) {
+                const bool a0 = v0 && lane >= start, a1 = v1 && lane + 64 >= start;
+                uint64_t lo = ~(uint64_t)0, hi = 0;
+                if (a0) { lo = k0; hi = k0; }
+                if (a1) { lo = k1 < lo ? k1 : lo; hi = k1 > hi ? k1 : hi; }
+                lo = __ockl_wfred_min_u64(lo);
+                hi = __ockl_wfred_max_u64(hi);
+                if (lo == hi) break;  // what remains holds one value: every later swap is the identity
+                // the smallest value's positions q_0 < q_1 < ... (c of them): step start+j
+                // swaps positions start+j and q_j, since the value's elements not yet placed
+                // never move and q_j >= start+j
+                const bool m0 = a0 && k0 == lo, m1 = a1 && k1 == lo;
+                const uint64_t b0 = __ballot(m0), b1 = __ballot(m1);
+                const int n0 = __popcll(b0), end = start + n0 + __popcll(b1);
+                const int r0 = __popcll(b0 & below), r1 = n0 + __popcll(b1 & below);
+                if (m0) qtab[r0] = lane;
+                if (m1) qtab[r1] = lane + 64;
+                __builtin_amdgcn_s_waitcnt(0);
+                __builtin_amdgcn_wave_barrier();
+                // destination: start+rank for the value's elements; an element displaced
+                // from x in [start, end) goes to q_{x-start}, and on while that is below end
+                auto dest = [&](int x, bool isv, int rk) -> int {
+                    if (isv) return start + rk;
+                    if (x < start || x >= end) return x;
+                    int y = qtab[x - start];
+                    while (y < end) y = qtab[y - start];
+                    return y;
+                };
+                const int d0 = v0 ? dest(lane, m0, r0) : lane;
+                const int d1 = v1 ? dest(lane + 64, m1, r1) : lane + 64;
+                __builtin_amdgcn_s_waitcnt(0);
+                __builtin_amdgcn_wave_barrier();
+                if (v0) { tmp64[d0] = k0; tmpmap[d0] = o0; }
+                if (v1) { tmp64[d1] = k1; tmpmap[d1] = o1; }
+                __builtin_amdgcn_s_waitcnt(0);
+                __builtin_amdgcn_wave_barrier();
+                if (v0) { k0 = tmp64[lane]; o0 = tmpmap[lane]; }
+                if (v1) { k1 = tmp64[lane + 64]; o1 = tmpmap[lane + 64]; }
+                __builtin_amdgcn_s_waitcnt(0);
+                __builtin_amdgcn_wave_barrier();
+                start = end;
+            }
+            // gather by origin (all lanes read before any lane writes)
+            uint64_t dg0 = 0, dg1 = 0;
+            int rm0 = 0, rm1 = 0;
+            if (v0) { dg0 = degree[o0]; rm0 = rowmap[o0]; }
+            if (v1) { dg1 = degree[o1]; rm1 = rowmap[o1]; }
+            __builtin_amdgcn_s_waitcnt(0);
+            __builtin_amdgcn_wave_barrier();
+            if (v0) { shift[lane] = k0; degree[lane] = dg0; rowmap[lane] = rm0; }
+            if (v1) { shift[lane + 64] = k1; degree[lane + 64] = dg1; rowmap[lane + 64] = rm1; }
+            // rows to sum: those that were not pivots at the previous step
+            int base = 0;
+            for (int h = 0; h < m; h += 64) {
+                const int r = h + lane;
+                const bool need = r < m && !was_piv[h == 0 ? rm0 : rm1];
+                const uint64_t bal = __ballot(need);
+                const int idx = base + __popcll(bal & below);
+                if (r < m) cidx[r] = need ? idx : -1;
+                if (need) comp[idx] = r;
+                base += __popcll(bal);
+            }
+            if (lane == 0) s_nc = base;
+        }
+        __syncthreads();
+        // ---- 2. this group's share of the discrepancy (order-basis.h:416-456) ----
+        const int nc = s_nc, slot = (int)(nbar & 1);
+        for (int o = tid; o < nc * n; o += kMbThreads) {
+            const int i = o / n, c = o - i * n, ph = rowmap[comp[i]];
+            AccSmall acc0, acc1;
+            for (int l = 0; l <= lim; ++l)
+                dot2(acc0, acc1, W + l * cst + ph * cw, Sg + (k - l) * cw * n + c, n, cols, f62);
+            const uint32_t v = sum2(acc0, acc1, p, mu);
+            if (G == 1) V[comp[i] * vs + c] = v;
+            else __hip_atomic_store(part + (slot * G + g) * mn + o, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (G > 1 && nc) grid_barrier(bar, ++nbar);
+        else __syncthreads();
+        // the summed discrepancy, its Montgomery form in V, an empty Z part; rows
+        // that were pivots at the previous step keep theirs: x.sigma_t has the same
+        // discrepancy (:446-456).  One row per wave, one column per lane.
+        for (int r = wave; r < m; r += kMbWaves) {
+            const int ph = rowmap[r];
+            const bool wp = was_piv[ph] != 0;
+            if (lane < n) {
+                uint32_t v;
+                if (wp) {
+                    v = D[ph * n + lane];
+                } else if (G == 1) {
+                    v = V[r * vs + lane];
+                } else {
+                    // plain loads, eight in flight: the grid barrier's acquire fence has
+                    // invalidated this CU's caches
+                    const uint32_t *q = part + slot * G * mn + cidx[r] * n + lane;
+                    uint64_t sum = 0;
+                    int h = 0;
+                    for (; h + 8 <= G; h += 8) {
+                        uint32_t t[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) t[j] = q[(h + j) * mn];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) sum += t[j];
+                    }
+                    for (; h < G; ++h) sum += q[h * mn];
+                    v = red64(sum, p, mu);
+                }
+                D[ph * n + lane] = v;
+                V[r * vs + lane] = mont(v, r2m, p, pinv);
+                V[r * vs + n + lane] = 0;
+            }
+        }
+        for (int i = tid; i < m; i += kMbThreads) pivot_of[i] = -1;
+        __syncthreads();
+        // ---- 3. row rank profile and combinations (the PLUQ of :470-488) ----
+        // lane j of a wave holds word j and word j+64 of [R_r | Z_r] for its rows
+        // r = wave + 16 u; every row below the pivot is updated, so the next pivot
+        // is each wave's first nonzero row, whose vector the wave stores in V
+        uint32_t x0[ROWS], x1[ROWS];
+        {
+            unsigned key = kNone;
+#pragma unroll
+            for (int u = 0; u < ROWS; ++u) {
+                const int r = wave + kMbWaves * u;
+                x0[u] = r < m && lane < n ? V[r * vs + lane] : 0u;
+                x1[u] = 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < ROWS; ++u) {
+                const int r = wave + kMbWaves * u;
+                const uint64_t bal = __ballot(r < m && lane < n && x0[u] != 0);
+                key = bal && key == kNone ? (unsigned)(r << 8 | __builtin_ctzll(bal)) : key;
+            }
+            if (lane == 0) cand[0][wave] = key;
+        }
+        __syncthreads();
+        int rank = 0;
+        uint32_t aprod = one;  // product of the pivots' entries so far: the scale of the pivot row
+        for (;;) {
+            const unsigned hit = __ockl_wfred_min_u32(lane < kMbWaves ? cand[rank & 1][lane] : kNone);
+            if (hit == kNone) break;
+            const int s = (int)(hit >> 8), c0 = (int)(hit & 255u);
+            // pivot vector: R_s, Z_s (rank words), -scale_s, then zeros; every row below s
+            // becomes a [R_r | Z_r] - b [R_s | Z_s | -scale_s], a = R_s[c0], b = R_r[c0]
+            const int wdt = n + rank + 1;
+            const bool two = wdt > 64;
+            const uint32_t sv0 = lane < wdt - 1 ? V[s * vs + lane] : 0u;
+            const uint32_t y0 = lane == wdt - 1 ? p - aprod : sv0;
+            uint32_t y1 = 0;
+            if (two) {
+                const uint32_t sv1 = lane + 64 < wdt - 1 ? V[s * vs + lane + 64] : 0u;
+                y1 = lane + 64 == wdt - 1 ? p - aprod : sv1;
+            }
+            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)y0, c0);
+            const uint32_t ny0 = p - y0, ny1 = p - y1;
+            // the rows' updates are independent: straight-line code (no branch between
+            // rows) lets their multiply chains overlap
+            auto update_rows = [&](auto both) {
+#pragma unroll
+                for (int u = 0; u < ROWS; ++u) {
+                    const int r = wave + kMbWaves * u;
+                    const bool upd = r > s && r < m;
+                    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
+                    const uint32_t t0 = amb1(a, x0[u], b, ny0, p, pinv);
+                    x0[u] = upd ? t0 : x0[u];
+                    if constexpr (decltype(both)::value) {
+                        const uint32_t t1 = amb1(a, x1[u], b, ny1, p, pinv);
+                        x1[u] = upd ? t1 : x1[u];
+                    }
+                }
+            };
+            if (two) update_rows(std::true_type{});
+            else update_rows(std::false_type{});
+            unsigned nkey = kNone;
+            int cu = -1;
+#pragma unroll
+            for (int u = 0; u < ROWS; ++u) {
+                const int r = wave + kMbWaves * u;
+                const uint64_t bal = __ballot(r > s && r < m && lane < n && x0[u] != 0);
+                const bool take = bal != 0 && nkey == kNone;
+                nkey = take ? (unsigned)(r << 8 | __builtin_ctzll(bal)) : nkey;
+                cu = take ? u : cu;
+            }
+            if (cu >= 0) {
+                uint32_t w0 = x0[0], w1 = x1[0];
+#pragma unroll
+                for (int u = 1; u < ROWS; ++u) {
+                    w0 = cu == u ? x0[u] : w0;
+                    w1 = cu == u ? x1[u] : w1;
+                }
+                const int r = wave + kMbWaves * cu;
+                if (lane < wdt) V[r * vs + lane] = w0;
+                if (two && lane + 64 < wdt) V[r * vs + lane + 64] = w1;
+            }
+            if (lane == 0) cand[(rank + 1) & 1][wave] = nkey;
+            aprod = mont(aprod, a, p, pinv);
+            if (tid == 0) {
+                pivot_of[s] = rank;
+                pivrow[rank] = s;
+                pref[rank + 1] = aprod;
+            }
+            ++rank;
+            __syncthreads();
+        }
+        // a non-pivot row's scale is the product of the entries of the pivots above it;
+        // its combination is Z / scale: plain inverse s^-1 = mont(montpow(scale, p-2), 1)
+        if (tid < 2 * 64) {
+            const uint64_t pm0 = __ballot(lane < m && pivot_of[lane] >= 0);
+            const uint64_t pm1 = __ballot(lane + 64 < m && pivot_of[lane + 64] >= 0);
+            const int r = tid;
+            if (r < m && pivot_of[r] < 0) {
+                const int cnt = wave == 0 ? __popcll(pm0 & below) : __popcll(pm0) + __popcll(pm1 & below);
+                uint32_t x = one, bp = pref[cnt];
+                for (uint32_t e = p - 2; e; e >>= 1) {
+                    if (e & 1) x = mont(x, bp, p, pinv);
+                    bp = mont(bp, bp, p, pinv);
+                }
+                rinv[r] = mont(x, 1u, p, pinv);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < ROWS; ++u) {
+            const int r = wave + kMbWaves * u;
+            if (r < m && pivot_of[r] < 0) {
+                const uint32_t inv = rinv[r];
+                if (lane >= n && lane < n + rank) V[r * vs + lane] = mont(x0[u], inv, p, pinv);
+                if (lane + 64 < n + rank) V[r * vs + lane + 64] = mont(x1[u], inv, p, pinv);
+            }
+        }
+        // Qt: pivot rows first, then the others, each in increasing order
+        static_assert(kMbMaxM <= 2 * 64 && kMbThreads >= 2 * 64, "two ballots cover every row");
+        if (tid < 2 * 64) {
+            const uint64_t b0 = __ballot(lane < m && pivot_of[lane] < 0);
+            const uint64_t b1 = __ballot(lane + 64 < m && pivot_of[lane + 64] < 0);
+            const int r = tid;
+            if (r < m) {
+                int pos = pivot_of[r];
+                if (pos < 0) pos = rank + (wave == 0 ? __popcll(b0 & below) : __popcll(b0) + __popcll(b1 & below));
+                oldrow[pos] = r;
+                tmpmap[pos] = rowmap[r];
+                tmp64[pos] = shift[r];
+                tmp64b[pos] = degree[r];
+                was_piv[rowmap[r]] = pivot_of[r] >= 0;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < m; i += kMbThreads) {
+            rowmap[i] = tmpmap[i];
+            shift[i] = tmp64[i];
+            degree[i] = tmp64b[i];
+        }
+        __syncthreads();
+        // ---- 4. this group's columns: non-pivot rows -= combination . pivot rows (:509-517) ----
+        {
+            const int nrows = m - rank, per_l = nrows * cw, total = (lim + 1) * per_l;
+            for (int o = tid; o < total; o += kMbThreads) {
+                const int l = o / per_l, rem = o - l * per_l, q = rem / cw, jj = rem - q * cw, si = rank + q;
+                const uint32_t *y = V + oldrow[si] * vs + n;
+                const uint32_t *w = W + l * cst + jj;
+                AccSmall acc0, acc1;
+                int t = 0;
+                for (; t + 3 < rank; t += 4) {
+                    const uint32_t w0 = w[rowmap[t] * cw], w1 = w[rowmap[t + 1] * cw], w2 = w[rowmap[t + 2] * cw],
+                                   w3 = w[rowmap[t + 3] * cw];
+                    acc0.add2(y[t], w0, y[t + 1], w1, f62);
+                    acc1.add2(y[t + 2], w2, y[t + 3], w3, f62);
+                }
+                for (; t < rank; ++t) acc0.add1(y[t], w[rowmap[t] * cw], f62);
+                const uint32_t sub = sum2(acc0, acc1, p, mu);
+                uint32_t *dst = W + l * cst + rowmap[si] * cw + jj;
+                const uint32_t v = *dst;
+                *dst = v >= sub ? v - sub : v + (p - sub);
+            }
+        }
+        __syncthreads();
+        // ---- pivot rows times x (:573-580); degrees and shifts (:546-566) ----
+        for (int o = tid; o < rank * cw; o += kMbThreads) {
+            const int t = o / cw, jj = o - t * cw;
+            uint32_t *col = W + rowmap[t] * cw + jj;
+            // eight coefficients at a time from the top: a block's writes reach only
+            // coefficients already read
+            for (int top = lim; top >= 0; top -= 8) {
+                uint32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (top - u >= 0) v[u] = col[(top - u) * cst];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (top - u >= 0) col[(top - u + 1) * cst] = v[u];
+            }
+            col[0] = 0;
+        }
+        if (wave == kMbWaves - 1) {
+            uint64_t dm = 0;
+            for (int i = lane; i < rank; i += 64) dm = degree[i] > dm ? degree[i] : dm;
+            dm = __ockl_wfred_max_u64(dm);
+            for (int i = lane; i < m; i += 64) {
+                if (i < rank) {
+                    degree[i]++;
+                    shift[i]++;
+                } else if (degree[i] < dm) {
+                    degree[i] = dm;
+                }
+            }
+            if (lane == 0) {
+                const uint64_t md = (uint64_t)maxdeg > dm + 1 ? (uint64_t)maxdeg : dm + 1;
+                s_maxdeg = (int)(md < (uint64_t)order ? md : (uint64_t)order);
+            }
+        }
+        __syncthreads();
+    }
+    // sigma.resize(max_degree+1), shift2 = shift (:592-597)
+    const int md = s_maxdeg;
+    const int cnt = out_count ? out_count : md + 1;
+    for (int o = tid; o < cnt * m * cols; o += kMbThreads) {
+        const int q = o / cols, l = q / m, jj = o - q * cols, i = q - l * m;
+        const uint32_t v = l <= md ? W[l * cst + rowmap[i] * cw + jj] : 0u;
+        out.base[view_index(out, (size_t)i * m + j0 + jj, l)] = v;
+    }
+    if (g == 0) {
+        for (int i = tid; i < m; i += kMbThreads) shift_io[i] = shift[i];
+        if (tid == 0) *deg_out = (uint64_t)md;
+    }
 }
 
 unsigned grid_for(size_t total)
@@ -580,19 +1058,21 @@ unsigned grid_for(size_t total)
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Device bytes of the three buffers each recursion level keeps live, summed
-// down the deepest path.
+// Device bytes of the buffers each recursion level keeps live (sigma1, serie2,
+// sigma2 and the two degree words), summed down the deepest path.
 size_t pm_level_bytes(size_t m, size_t n, size_t order)
 {
     if (order <= kMbThreshold) return 0;
     const size_t o1 = order >> 1, o2 = order - o1;
-    const size_t own = align256(m * m * (o1 + 1) * 4) + align256(m * n * o2 * 4) + align256(m * m * (o2 + 1) * 4);
+    const size_t own =
+        align256(m * m * (o1 + 1) * 4) + align256(m * n * o2 * 4) + align256(m * m * (o2 + 1) * 4) + align256(16);
     return own + std::max(pm_level_bytes(m, n, o1), pm_level_bytes(m, n, o2));
 }
 
 struct MbScratch {
-    uint32_t *S, *W;
+    uint32_t *S, *W, *part;
     uint64_t *shift, *deg;
+    unsigned *bar;  // grid barrier: arrivals, then the failure flag
 };
 
 struct PmRun {
@@ -626,20 +1106,72 @@ DevView polfirst_view(uint32_t *base, size_t rows, size_t cols, size_t stor, siz
     return v;
 }
 
+size_t mc_lds_words(size_t m, size_t n, size_t order, size_t cw)
+{
+    return cw * ((order + 1) * m + order * n) + m * n + m * (n + 1) + m * n;
+}
+
+// Column split of m_basis_cols_kernel: *G groups of *cw columns, or *G = 0 for
+// the one-workgroup kernel (orders above kMcMaxOrder, or LDS too small for one
+// column).  By default about m^2 n / 8192 groups (at most 16): the work each
+// group repeats (sort and elimination) against the work it shares out.
+// LBX_MBASIS_GROUPS=N asks for N groups, 0 for the one-workgroup kernel.
+void cols_split(size_t m, size_t n, size_t order, int *G, int *cw)
+{
+    *G = 0;
+    *cw = 0;
+    if (order > (size_t)kMcMaxOrder) return;
+    size_t target = std::min<size_t>(16, std::max<size_t>(1, m * m * n / 8192));
+    if (const char *e = getenv("LBX_MBASIS_GROUPS")) {
+        const long want = atol(e);
+        if (want <= 0) return;
+        target = (size_t)want;
+    }
+    size_t w = (m + target - 1) / target;
+    while (w > 0 && mc_lds_words(m, n, order, w) > (size_t)kMcLdsWords) --w;
+    if (!w) return;
+    const size_t groups = (m + w - 1) / w;
+    if (groups > (size_t)kMcMaxGroups) return;
+    *G = (int)groups;
+    *cw = (int)w;
+}
+
+// T = a_top . b_top (m x m coefficient matrices: coefficient ta of a, tb of b);
+// c[.][0] -= T and c[.][P] = T.  A cyclic product mod x^P - 1 of two operands
+// whose full product has P+1 coefficients holds coefficient P in coefficient
+// 0; this puts it back.  With fix = false only the degree word is written.
+// deg = d[0] + d[1] (the degrees of sigma2 and sigma1).
+__global__ void __launch_bounds__(256) wrap_fix_kernel(DevView a, size_t ta, DevView b, size_t tb, DevView c,
+                                                       size_t P, int m, uint32_t p, uint64_t mu, uint32_t r64,
+                                                       bool fix, const uint64_t *d, uint64_t *deg)
+{
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e == 0) *deg = d[0] + d[1];
+    if (!fix || e >= (size_t)m * m) return;
+    const size_t i = e / m, j = e - i * m;
+    Acc acc;
+    for (int q = 0; q < m; ++q)
+        acc.add(a.base[view_index(a, i * m + q, ta)], b.base[view_index(b, (size_t)q * m + j, tb)]);
+    const uint32_t t = acc.get(p, mu, r64);
+    uint32_t *c0 = c.base + view_index(c, e, 0);
+    const uint32_t v = *c0;
+    *c0 = v >= t ? v - t : v + (p - t);
+    c.base[view_index(c, e, P)] = t;
+}
+
 // M_Basis on a device view of the serie (coefficients 0..order-1) into sigma.
-lbx_status run_m_basis(PmRun &R, const DevView &serie, const DevView &sigma, size_t order, size_t *deg)
+// The degree goes to *deg on the device; out_count coefficients of sigma are
+// written (zero above the degree), or degree+1 when out_count is 0.
+lbx_status run_m_basis(PmRun &R, const DevView &serie, const DevView &sigma, size_t order, uint64_t *deg,
+                       size_t out_count)
 {
     const size_t m = R.m, n = R.n;
     const size_t g = serie.rows * serie.cols * order;
-    hipLaunchKernelGGL(gather_coeff_major, dim3(grid_for(g)), dim3(256), 0, R.s, serie, order, R.mb.S);
+    hipLaunchKernelGGL(gather_coeff_major, dim3(grid_for(g)), dim3(256), 0, R.s, serie, order, R.mb.S, R.mb.bar);
     LBX_TRY_HIP(hipGetLastError());
-    const uint64_t mu = ~(uint64_t)0 / R.p;
+    uint64_t mu = ~(uint64_t)0 / R.p;
     const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % R.p), r64 = (uint32_t)((uint64_t)r32 * r32 % R.p);
-    const uint32_t f62 = (uint32_t)(((uint64_t)1 << 62) % R.p);
-    // staged words: enough for every coefficient of a step, at most kMbStageWords
-    const size_t need = std::max((order + 1) * (m * m + m * n), m * m);
-    const int stage = (int)std::min(need, (size_t)kMbStageWords);
-    const size_t lds = (3 * m * n + (size_t)stage) * 4;
+    uint32_t f62 = (uint32_t)(((uint64_t)1 << 62) % R.p);
     const bool mont_ok = R.p < (1u << 31) && (R.p & 1);
     uint32_t pinv = 0, r2m = 0, one = 1;
     if (mont_ok) {
@@ -649,16 +1181,38 @@ lbx_status run_m_basis(PmRun &R, const DevView &serie, const DevView &sigma, siz
         one = r32;
         r2m = (uint32_t)((uint64_t)r32 * r32 % R.p);
     }
+    int G = 0, cw = 0;
+    if (mont_ok) cols_split(m, n, order, &G, &cw);
+    if (G > 0) {
+        const size_t lds = mc_lds_words(m, n, order, (size_t)cw) * 4;
+        // rows of the elimination per wave: m / 16, rounded up to 2, 4 or 8
+        const void *kern = m <= 2 * (size_t)kMbWaves   ? (const void *)m_basis_cols_kernel<2>
+                           : m <= 4 * (size_t)kMbWaves ? (const void *)m_basis_cols_kernel<4>
+                                                       : (const void *)m_basis_cols_kernel<8>;
+        LBX_TRY_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const uint32_t *S = R.mb.S;
+        DevView out = sigma;
+        uint64_t *shift_io = R.mb.shift;
+        uint32_t *part = R.mb.part;
+        unsigned *bar = R.mb.bar;
+        int order_i = (int)order, m_i = (int)m, n_i = (int)n, oc = (int)out_count;
+        uint32_t p = R.p;
+        void *args[] = {&S,   &out, &shift_io, &deg, &part, &bar,  &order_i, &m_i, &n_i, &cw,
+                        &oc,  &p,   &mu,       &f62, &pinv, &r2m, &one};
+        // every group waits for the others at each step: they must all be resident
+        LBX_TRY_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMbThreads), args, (unsigned)lds, R.s));
+        return LBX_OK;
+    }
+    // staged words: enough for every coefficient of a step, at most kMbStageWords
+    const size_t need = std::max((order + 1) * (m * m + m * n), m * m);
+    const int stage = (int)std::min(need, (size_t)kMbStageWords);
+    const size_t lds = (3 * m * n + (size_t)stage) * 4;
     auto kern = mont_ok ? m_basis_kernel<AccSmall, true>
                         : (R.p < (1u << 31) ? m_basis_kernel<AccSmall, false> : m_basis_kernel<AccWide, false>);
     LBX_TRY_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(1), dim3(kMbThreads), lds, R.s, R.mb.S, R.mb.W, sigma, R.mb.shift, R.mb.deg,
-                       (int)order, (int)m, (int)n, stage, R.p, mu, r64, f62, pinv, r2m, one);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kMbThreads), lds, R.s, R.mb.S, R.mb.W, sigma, R.mb.shift, deg, (int)order,
+                       (int)m, (int)n, stage, (int)out_count, R.p, mu, r64, f62, pinv, r2m, one);
     LBX_TRY_HIP(hipGetLastError());
-    uint64_t d = 0;
-    LBX_TRY_HIP(hipMemcpyAsync(&d, R.mb.deg, 8, hipMemcpyDeviceToHost, R.s));
-    LBX_TRY_HIP(hipStreamSynchronize(R.s));
-    *deg = (size_t)d;
     return LBX_OK;
 }
 
@@ -681,19 +1235,53 @@ lbx_status product(PmRun &R, const DevView &a, const DevView &b, const DevView &
     return mp_product(R.ctx, &R.p, 1, a, b, c, m, k, n, R.method, R.s);
 }
 
-lbx_status pm_basis(PmRun &R, const DevView &serie, const DevView &sigma, size_t order, size_t *deg)
+// sigma = sigma2.sigma1 (:297) on all stored coefficients: sigma2 has o2+1 and
+// sigma1 o1+1 (zero above their degrees), so sigma gets order+1 coefficients,
+// zero above d1+d2, without the degrees being read back.  When order is a
+// power of two and an FFT prime reaches it, the product is taken mod
+// x^order - 1 (the length the true degrees would need, since d1+d2 <= order)
+// and the one coefficient that wraps is put back.  *deg = d[0] + d[1].
+lbx_status sigma_product(PmRun &R, uint32_t *s2, size_t o2, uint32_t *s1, size_t o1, const DevView &sigma,
+                         size_t order, const uint64_t *d, uint64_t *deg)
 {
-    if (order <= kMbThreshold) return run_m_basis(R, serie, sigma, order, deg);
+    const size_t m = R.m;
+    const DevView a = polfirst_view(s2, m, m, o2 + 1, o2 + 1), b = polfirst_view(s1, m, m, o1 + 1, o1 + 1);
+    const unsigned Lp = log2_ceil(order);
+    const size_t P = (size_t)1 << Lp;
+    const uint64_t mu = ~(uint64_t)0 / R.p;
+    const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % R.p), r64 = (uint32_t)((uint64_t)r32 * r32 % R.p);
+    DevView c = sigma;
+    const bool wrap = P == order && R.p < (1u << 31) && (R.p - 1) % P == 0;
+    if (wrap) {
+        c.size = P;
+        LBX_TRY(fft_prime_product(R.ctx, R.p, Lp, a, b, c, m, m, m, P, R.method, R.s));
+    } else {
+        c.size = order + 1;
+        LBX_TRY(product(R, a, b, c, m, m, m));
+    }
+    hipLaunchKernelGGL(wrap_fix_kernel, dim3(wrap ? (unsigned)((m * m + 255) / 256) : 1u), dim3(256), 0, R.s, a, o2,
+                       b, o1, sigma, P, (int)m, R.p, mu, r64, wrap, d, deg);
+    return from_hip(hipGetLastError());
+}
+
+// PM_Basis (:219-326) over device buffers.  sigma gets order+1 coefficients,
+// zero above the degree (written to *deg on the device), except that a call
+// that is one base case writes out_count of them (degree+1 when 0).
+lbx_status pm_basis(PmRun &R, const DevView &serie, const DevView &sigma, size_t order, uint64_t *deg,
+                    size_t out_count)
+{
+    if (order <= kMbThreshold) return run_m_basis(R, serie, sigma, order, deg, out_count);
     const size_t m = R.m, n = R.n, o1 = order >> 1, o2 = order - o1;
     const size_t mark = R.used;
     uint32_t *s1 = R.take(m * m * (o1 + 1) * 4), *ser2 = R.take(m * n * o2 * 4), *s2 = R.take(m * m * (o2 + 1) * 4);
-    size_t d1 = 0, d2 = 0;
+    uint64_t *d12 = (uint64_t *)R.take(16);
     DevView serie1 = serie;
     serie1.size = o1;  // serie1.copy(serie, 0, ord1-1) (:258) as a view
-    LBX_TRY(pm_basis(R, serie1, polfirst_view(s1, m, m, o1 + 1, o1 + 1), o1, &d1));
+    LBX_TRY(pm_basis(R, serie1, polfirst_view(s1, m, m, o1 + 1, o1 + 1), o1, d12, o1 + 1));
     // serie2 = coefficients ord1..order-1 of sigma1.serie: midproductgen(serie2, sigma1, serie, true,
-    // ord1+1, ord1+ord2) (:274) with c.size() = ord2 > 4, the FFT middle product
-    const DevView a = polfirst_view(s1, m, m, o1 + 1, d1 + 1);
+    // ord1+1, ord1+ord2) (:274) with c.size() = ord2 > 4, the FFT middle product; sigma1 enters
+    // with its o1+1 stored coefficients, zero above its degree
+    const DevView a = polfirst_view(s1, m, m, o1 + 1, o1 + 1);
     DevView b = serie;
     b.size = order;
     const DevView c = polfirst_view(ser2, m, n, o2, o2);
@@ -703,14 +1291,11 @@ lbx_status pm_basis(PmRun &R, const DevView &serie, const DevView &sigma, size_t
         LBX_TRY(fft_prime_product(R.ctx, R.p, L, a, b, c, m, m, n, o2, R.method, R.s, &mid));
     else
         LBX_TRY(mp_product(R.ctx, &R.p, 1, a, b, c, m, m, n, R.method, R.s, &mid, L));
-    LBX_TRY(pm_basis(R, polfirst_view(ser2, m, n, o2, o2), polfirst_view(s2, m, m, o2 + 1, o2 + 1), o2, &d2));
-    // sigma = sigma2.sigma1 (:297), then resize(d1+d2+1)
-    DevView out = sigma;
-    out.size = d1 + d2 + 1;
-    LBX_TRY(product(R, polfirst_view(s2, m, m, o2 + 1, d2 + 1), polfirst_view(s1, m, m, o1 + 1, d1 + 1), out, m, m,
-                    m));
+    LBX_TRY(pm_basis(R, polfirst_view(ser2, m, n, o2, o2), polfirst_view(s2, m, m, o2 + 1, o2 + 1), o2, d12 + 1,
+                     o2 + 1));
+    // sigma = sigma2.sigma1 (:297); the resize to d1+d2+1 is the caller's
+    LBX_TRY(sigma_product(R, s2, o2, s1, o1, sigma, order, d12, deg));
     R.used = mark;
-    *deg = d1 + d2;
     return LBX_OK;
 }
 
@@ -740,38 +1325,47 @@ lbx_status order_basis(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_po
     StreamOrder stream_order(ctx, s);
     const size_t base_order = divide ? std::min(order, kMbThreshold) : order;
     const size_t b_S = align256(m * n * base_order * 4), b_W = align256((base_order + 1) * m * m * 4);
-    const size_t b_sh = align256(m * 8), b_deg = 256;
+    const size_t b_part = align256(2 * (size_t)kMcMaxGroups * m * n * 4);
+    const size_t b_sh = align256(m * 8), b_small = 256;
     const size_t arena = divide ? pm_level_bytes(m, n, order) : 0;
     void *scratch = nullptr;
-    LBX_TRY(ensure(ctx, "order_basis", b_S + b_W + b_sh + b_deg + arena, &scratch));
+    LBX_TRY(ensure(ctx, "order_basis", b_S + b_W + b_part + b_sh + 2 * b_small + arena, &scratch));
     uint8_t *base = (uint8_t *)scratch;
-    PmRun R{ctx, p, m, n, method, s, {}, base + b_S + b_W + b_sh + b_deg, 0, arena};
+    PmRun R{ctx, p, m, n, method, s, {}, base + b_S + b_W + b_part + b_sh + 2 * b_small, 0, arena};
     R.mb.S = (uint32_t *)base;
     R.mb.W = (uint32_t *)(base + b_S);
-    R.mb.shift = (uint64_t *)(base + b_S + b_W);
-    R.mb.deg = (uint64_t *)(base + b_S + b_W + b_sh);
+    R.mb.part = (uint32_t *)(base + b_S + b_W);
+    R.mb.shift = (uint64_t *)(base + b_S + b_W + b_part);
+    R.mb.deg = (uint64_t *)(base + b_S + b_W + b_part + b_sh);
+    R.mb.bar = (unsigned *)(base + b_S + b_W + b_part + b_sh + b_small);
 
     Staged ss, sg;
     lbx_status st = stage_in(ctx, "stage_a", serie, 4, true, &ss);
     if (st == LBX_OK) st = stage_in(ctx, "stage_c", sigma, 4, true, &sg);
     std::vector<uint64_t> sh(shift, shift + m);
-    size_t d = 0;
+    uint64_t d = 0;
+    unsigned failed = 0;
     if (st == LBX_OK) st = from_hip(hipMemcpyAsync(R.mb.shift, sh.data(), m * 8, hipMemcpyHostToDevice, s));
+    if (st == LBX_OK) st = from_hip(hipMemsetAsync(R.mb.bar, 0, 8, s));
     if (st == LBX_OK) {
         const DevView dser = devview(serie, ss.dev), dsig = devview(sigma, sg.dev);
-        st = divide ? pm_basis(R, dser, dsig, order, &d) : run_m_basis(R, dser, dsig, order, &d);
+        st = divide ? pm_basis(R, dser, dsig, order, R.mb.deg, 0) : run_m_basis(R, dser, dsig, order, R.mb.deg, 0);
     }
+    // the one read-back of the call: shift, degree and the grid barrier's failure flag
     if (st == LBX_OK) st = from_hip(hipMemcpyAsync(sh.data(), R.mb.shift, m * 8, hipMemcpyDeviceToHost, s));
+    if (st == LBX_OK) st = from_hip(hipMemcpyAsync(&d, R.mb.deg, 8, hipMemcpyDeviceToHost, s));
+    if (st == LBX_OK) st = from_hip(hipMemcpyAsync(&failed, R.mb.bar + 1, 4, hipMemcpyDeviceToHost, s));
     if (st == LBX_OK && sg.owned)
         st = from_hip(hipMemcpyAsync(sigma->data, sg.dev, storage_words(sigma) * 4, hipMemcpyDeviceToHost, s));
     const lbx_status sync = from_hip(hipStreamSynchronize(s));
     if (st == LBX_OK) st = sync;
+    if (st == LBX_OK && failed) st = LBX_ERR_HIP;  // a grid barrier gave up waiting
     release(ss);
     release(sg);
     if (st == LBX_OK) {
         for (size_t i = 0; i < m; ++i) shift[i] = (size_t)sh[i];
-        *degree = d;
-        sigma->size = d + 1;
+        *degree = (size_t)d;
+        sigma->size = (size_t)d + 1;
     }
     return st;
 }

@@ -83,6 +83,36 @@ def test_shapes_and_shifts(ctx, m, n, order):
         compare(ctx, "pm", FFT_P, m, n, serie, order, shift)
 
 
+@pytest.mark.parametrize("groups", ["0", "1", "3", "7", "16", "64"])
+def test_mbasis_column_groups(ctx, monkeypatch, groups):
+    """The base-case M_Basis with sigma's columns split over workgroups (LBX_MBASIS_GROUPS;
+    0 takes the one-workgroup kernel): 25 columns give ragged last groups (3 groups of 9,
+    7 of 4, 25 of 1).  Random and rank-deficient series, random shifts, and a PM_Basis whose
+    base cases take the same split."""
+    monkeypatch.setenv("LBX_MBASIS_GROUPS", groups)
+    rng = np.random.default_rng(300 + int(groups))
+    m, n, order = 25, 12, 32
+    serie = rand(rng, FFT_P, m * n * order)
+    shift = [int(x) for x in rng.integers(0, 3, m)]
+    compare(ctx, "m", FFT_P, m, n, serie, order, shift)
+    F = serie.reshape(m, n, order).astype(np.uint64)
+    F[:, 5] = (F[:, 1] * 7 + F[:, 2]) % FFT_P
+    F[3] = 0
+    F[:, :, :4] = 0
+    compare(ctx, "m", FFT_P, m, n, F.astype(np.uint32).reshape(-1), order, [0] * m)
+    compare(ctx, "pm", FFT_P, m, n, rand(rng, FFT_P, m * n * 80), 80, shift)
+    compare(ctx, "m", 1000000007, 9, 4, rand(rng, 1000000007, 9 * 4 * 20), 20, [0] * 9)
+
+
+@pytest.mark.parametrize("m,n,order", [(1, 1, 64), (3, 3, 128), (2, 1, 64), (64, 32, 64)])
+def test_pm_basis_full_degree(ctx, m, n, order):
+    """PM_Basis at power-of-two orders, where the product sigma2.sigma1 is taken mod
+    x^order - 1 and its top coefficient put back: with m <= n every row is a pivot at
+    every step, so the degree reaches the order and that coefficient is nonzero."""
+    rng = np.random.default_rng(order + m)
+    compare(ctx, "pm", FFT_P, m, n, rand(rng, FFT_P, m * n * order), order, [0] * m)
+
+
 def test_special_series(ctx):
     rng = np.random.default_rng(5)
     m, n, order = 6, 3, 50
