@@ -691,10 +691,9 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
     uint32_t *D = V + m * vs;
     __shared__ uint64_t shift[kMbMaxM], degree[kMbMaxM], tmp64[kMbMaxM], tmp64b[kMbMaxM];
     __shared__ int rowmap[kMbMaxM], tmpmap[kMbMaxM], oldrow[kMbMaxM], pivot_of[kMbMaxM], comp[kMbMaxM],
-        cidx[kMbMaxM], was_piv[kMbMaxM], qtab[kMbMaxM], pivrow[kMbMaxN];
-    __shared__ uint32_t pref[kMbMaxN + 1], rinv[kMbMaxM];
-    __shared__ unsigned cand[2][kMbWaves];  // each wave's candidate for the next pivot, row << 8 | column
-    __shared__ int s_maxdeg, s_nc;
+        cidx[kMbMaxM], was_piv[kMbMaxM], qtab[kMbMaxM], piv_s[kMbMaxN], piv_c[kMbMaxN];
+    __shared__ uint32_t pref[kMbMaxN + 1], rinv[kMbMaxM], piv_a[kMbMaxN];  // pivots: row, column, entry
+    __shared__ int s_maxdeg, s_nc, s_rank;
 
     for (int i = tid; i < (order + 1) * cst; i += kMbThreads) {
         const int q = i / cw, jj = i - q * cw, ph = q % m, l = q / m;
@@ -839,94 +838,86 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
         for (int i = tid; i < m; i += kMbThreads) pivot_of[i] = -1;
         __syncthreads();
         // ---- 3. row rank profile and combinations (the PLUQ of :470-488) ----
-        // lane j of a wave holds word j and word j+64 of [R_r | Z_r] for its rows
-        // r = wave + 16 u; every row below the pivot is updated, so the next pivot
-        // is each wave's first nonzero row, whose vector the wave stores in V
+        // Wave w holds rows r0 = w*ROWS .. r0+ROWS-1 of [R | Z] in registers (lane j: words
+        // j and j+64).  Blocks of rows go in order: at step w the waves from w on apply to
+        // their rows the pivots found in block w-1, then wave w runs through its own rows,
+        // each against the pivots before it, and stores the pivots' vectors.  Every row below
+        // a pivot meets it once, in pivot order, as in a pivot-by-pivot elimination; one
+        // barrier per block.  Pivot vector j: [R_s | Z_s (j words) | -scale_s | 0 ...];
+        // a row below it becomes a [R_r | Z_r] - b [pivot vector], a = R_s[c_j], b = R_r[c_j].
+        const int r0 = wave * ROWS;
         uint32_t x0[ROWS], x1[ROWS];
-        {
-            unsigned key = kNone;
 #pragma unroll
-            for (int u = 0; u < ROWS; ++u) {
-                const int r = wave + kMbWaves * u;
-                x0[u] = r < m && lane < n ? V[r * vs + lane] : 0u;
-                x1[u] = 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < ROWS; ++u) {
-                const int r = wave + kMbWaves * u;
-                const uint64_t bal = __ballot(r < m && lane < n && x0[u] != 0);
-                key = bal && key == kNone ? (unsigned)(r << 8 | __builtin_ctzll(bal)) : key;
-            }
-            if (lane == 0) cand[0][wave] = key;
+        for (int u = 0; u < ROWS; ++u) {
+            const int r = r0 + u;
+            x0[u] = r < m && lane < n ? V[r * vs + lane] : 0u;
+            x1[u] = 0u;
         }
-        __syncthreads();
-        int rank = 0;
-        uint32_t aprod = one;  // product of the pivots' entries so far: the scale of the pivot row
-        for (;;) {
-            const unsigned hit = __ockl_wfred_min_u32(lane < kMbWaves ? cand[rank & 1][lane] : kNone);
-            if (hit == kNone) break;
-            const int s = (int)(hit >> 8), c0 = (int)(hit & 255u);
-            // pivot vector: R_s, Z_s (rank words), -scale_s, then zeros; every row below s
-            // becomes a [R_r | Z_r] - b [R_s | Z_s | -scale_s], a = R_s[c0], b = R_r[c0]
-            const int wdt = n + rank + 1;
-            const bool two = wdt > 64;
-            const uint32_t sv0 = lane < wdt - 1 ? V[s * vs + lane] : 0u;
-            const uint32_t y0 = lane == wdt - 1 ? p - aprod : sv0;
-            uint32_t y1 = 0;
-            if (two) {
-                const uint32_t sv1 = lane + 64 < wdt - 1 ? V[s * vs + lane + 64] : 0u;
-                y1 = lane + 64 == wdt - 1 ? p - aprod : sv1;
-            }
-            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)y0, c0);
-            const uint32_t ny0 = p - y0, ny1 = p - y1;
-            // the rows' updates are independent: straight-line code (no branch between
-            // rows) lets their multiply chains overlap
-            auto update_rows = [&](auto both) {
+        int rank = 0, jb = 0;  // pivots found so far; those from jb on are still to be applied
+        for (int w = 0; w * ROWS < m; ++w) {
+            if (jb == rank && rank == n) break;  // no pivot left to find or to apply
+            if (wave >= w) {
+                for (int j = jb; j < rank; ++j) {
+                    const int s = piv_s[j], c0 = piv_c[j];
+                    const uint32_t a = piv_a[j];
+                    const int wdt = n + j + 1;
+                    const uint32_t y0 = lane < wdt ? V[s * vs + lane] : 0u, ny0 = p - y0;
+                    // the rows' updates are independent: straight-line code lets their
+                    // multiply chains overlap
+                    if (wdt > 64) {
+                        const uint32_t y1 = lane + 64 < wdt ? V[s * vs + lane + 64] : 0u, ny1 = p - y1;
 #pragma unroll
-                for (int u = 0; u < ROWS; ++u) {
-                    const int r = wave + kMbWaves * u;
-                    const bool upd = r > s && r < m;
-                    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
-                    const uint32_t t0 = amb1(a, x0[u], b, ny0, p, pinv);
-                    x0[u] = upd ? t0 : x0[u];
-                    if constexpr (decltype(both)::value) {
-                        const uint32_t t1 = amb1(a, x1[u], b, ny1, p, pinv);
-                        x1[u] = upd ? t1 : x1[u];
+                        for (int u = 0; u < ROWS; ++u) {
+                            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
+                            x0[u] = amb1(a, x0[u], b, ny0, p, pinv);
+                            x1[u] = amb1(a, x1[u], b, ny1, p, pinv);
+                        }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < ROWS; ++u) {
+                            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
+                            x0[u] = amb1(a, x0[u], b, ny0, p, pinv);
+                        }
                     }
                 }
-            };
-            if (two) update_rows(std::true_type{});
-            else update_rows(std::false_type{});
-            unsigned nkey = kNone;
-            int cu = -1;
-#pragma unroll
-            for (int u = 0; u < ROWS; ++u) {
-                const int r = wave + kMbWaves * u;
-                const uint64_t bal = __ballot(r > s && r < m && lane < n && x0[u] != 0);
-                const bool take = bal != 0 && nkey == kNone;
-                nkey = take ? (unsigned)(r << 8 | __builtin_ctzll(bal)) : nkey;
-                cu = take ? u : cu;
             }
-            if (cu >= 0) {
-                uint32_t w0 = x0[0], w1 = x1[0];
+            if (wave == w) {
+                uint32_t aprod = pref[rank];  // the scale of the next pivot row
+                int j = rank;
 #pragma unroll
-                for (int u = 1; u < ROWS; ++u) {
-                    w0 = cu == u ? x0[u] : w0;
-                    w1 = cu == u ? x1[u] : w1;
+                for (int u = 0; u < ROWS; ++u) {
+                    const int r = r0 + u;
+                    const uint64_t bal = __ballot(r < m && lane < n && x0[u] != 0);
+                    if (bal) {
+                        const int c0 = (int)__builtin_ctzll(bal), wdt = n + j + 1;
+                        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
+                        const uint32_t y0 = lane == wdt - 1 ? p - aprod : x0[u];
+                        const uint32_t y1 = lane + 64 == wdt - 1 ? p - aprod : x1[u];
+                        if (lane < wdt) V[r * vs + lane] = y0;
+                        if (lane + 64 < wdt) V[r * vs + lane + 64] = y1;
+                        if (lane == 0) {
+                            piv_s[j] = r;
+                            piv_c[j] = c0;
+                            piv_a[j] = a;
+                            pivot_of[r] = j;
+                        }
+                        const uint32_t ny0 = p - y0, ny1 = p - y1;
+#pragma unroll
+                        for (int v = u + 1; v < ROWS; ++v) {
+                            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[v], c0);
+                            x0[v] = amb1(a, x0[v], b, ny0, p, pinv);
+                            if (wdt > 64) x1[v] = amb1(a, x1[v], b, ny1, p, pinv);
+                        }
+                        aprod = mont(aprod, a, p, pinv);
+                        ++j;
+                        if (lane == 0) pref[j] = aprod;
+                    }
                 }
-                const int r = wave + kMbWaves * cu;
-                if (lane < wdt) V[r * vs + lane] = w0;
-                if (two && lane + 64 < wdt) V[r * vs + lane + 64] = w1;
+                if (lane == 0) s_rank = j;
             }
-            if (lane == 0) cand[(rank + 1) & 1][wave] = nkey;
-            aprod = mont(aprod, a, p, pinv);
-            if (tid == 0) {
-                pivot_of[s] = rank;
-                pivrow[rank] = s;
-                pref[rank + 1] = aprod;
-            }
-            ++rank;
             __syncthreads();
+            jb = rank;
+            rank = s_rank;
         }
         // a non-pivot row's scale is the product of the entries of the pivots above it;
         // its combination is Z / scale: plain inverse s^-1 = mont(montpow(scale, p-2), 1)
@@ -947,7 +938,7 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < ROWS; ++u) {
-            const int r = wave + kMbWaves * u;
+            const int r = r0 + u;
             if (r < m && pivot_of[r] < 0) {
                 const uint32_t inv = rinv[r];
                 if (lane >= n && lane < n + rank) V[r * vs + lane] = mont(x0[u], inv, p, pinv);
