@@ -52,7 +52,9 @@ typedef enum lbx_status {
     LBX_ERR_OUT_OF_MEMORY = 10,     /* device allocation failed */
     LBX_ERR_UNSUPPORTED = 11,       /* shape beyond what the library implements */
     LBX_ERR_NO_DEVICE = 12,         /* no gfx950 device at the requested index */
-    LBX_ERR_WORKSPACE_TOO_SMALL = 13 /* the call needs more device memory than its fixed workspace holds */
+    LBX_ERR_WORKSPACE_TOO_SMALL = 13, /* the call needs more device memory than its fixed workspace holds */
+    LBX_ERR_CHECK_FAILED = 14       /* the product check at a random point failed: the "error" report of
+                                       check_mul (matpoly-mult-fft.h:105-117) under CHECK_MATPOL_MUL */
 } lbx_status;
 
 typedef enum lbx_layout {
@@ -101,6 +103,24 @@ lbx_status lbx_trim(lbx_ctx *ctx);
  * multiprecision domain above, with the same result). */
 lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
                                const lbx_polmat *b, size_t max_rowdeg, int method, void *stream);
+
+/* check_mul(c, a, b, deg) (matpoly-mult-fft.h:87-163) at the point x instead
+ * of 1: a and b are evaluated over all their coefficients and c over its
+ * first deg (c->size when deg is 0) at x mod p, and *equal is set to 1 when
+ * C(x) = A(x).B(x) mod p, else to 0.  Any layout or view, host or device
+ * storage, 2 <= p < 2^32.  Returns when the answer is known. */
+lbx_status lbx_polymat_check_mul_u32(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a,
+                                     const lbx_polmat *b, size_t deg, uint32_t x, int *equal, void *stream);
+
+/* CHECK_MATPOL_MUL (polynomial-matrix-domain.h:72-74) as a run-time switch:
+ * with on != 0, every later lbx_polymat_mul_u32 on the context whose c
+ * receives the whole product (max_rowdeg 0 or >= a.size+b.size-2) is
+ * followed by lbx_polymat_check_mul_u32 at a pseudo-random point, and a
+ * mismatch returns LBX_ERR_CHECK_FAILED (c keeps the computed product).  The
+ * check waits for the product, so the call no longer returns before its
+ * work is done.  LBX_CHECK_MUL=1 in the environment turns it on for every
+ * new context. */
+lbx_status lbx_set_check(lbx_ctx *ctx, int on);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -18,6 +18,7 @@ STATUS_NAMES = {
     4: "LBX_ERR_NOT_PRIMITIVE_ROOT", 5: "LBX_ERR_SIZE_MISMATCH", 6: "LBX_ERR_NOT_ENOUGH_PRIMES",
     7: "LBX_ERR_INVALID_ARG", 8: "LBX_ERR_PRIME_TOO_LARGE", 9: "LBX_ERR_HIP", 10: "LBX_ERR_OUT_OF_MEMORY",
     11: "LBX_ERR_UNSUPPORTED", 12: "LBX_ERR_NO_DEVICE", 13: "LBX_ERR_WORKSPACE_TOO_SMALL",
+    14: "LBX_ERR_CHECK_FAILED",
 }
 POLFIRST, MATFIRST, MATROWFIRST = 0, 1, 2
 METHOD_AUTO, METHOD_PLAIN_U64, METHOD_INT8_MFMA = 0, 1, 2
@@ -28,7 +29,7 @@ EXPORTED = [
     "lbx_ntt_forward_u32", "lbx_ntt_inverse_u32", "lbx_polymat_mul_mp", "lbx_polymat_mul_batch",
     "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_order_basis_pm_u32",
     "lbx_order_basis_m_u32", "lbx_polymat_copy_u32", "lbx_profile_enable", "lbx_profile_get",
-    "lbx_polymat_mul_u32_noalloc", "lbx_batch_tuning",
+    "lbx_polymat_mul_u32_noalloc", "lbx_batch_tuning", "lbx_polymat_check_mul_u32", "lbx_set_check",
 ]
 
 
@@ -92,6 +93,8 @@ def lib():
         L.lbx_profile_get.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_float), i32]
         L.lbx_profile_get.restype = i32
         L.lbx_batch_tuning.argtypes = [vp]
+        L.lbx_polymat_check_mul_u32.argtypes = [vp, u32, PM, PM, PM, sz, u32, ctypes.POINTER(i32), vp]
+        L.lbx_set_check.argtypes = [vp, i32]
         L.lbx_batch_tuning.restype = ctypes.c_char_p
         for name in EXPORTED:
             if name not in ("lbx_strerror", "lbx_profile_get", "lbx_batch_tuning"):
@@ -127,6 +130,11 @@ class Context:
         """enable: False/0 off, True/1 stage times, 2 stage and per-kernel times."""
         check(lib().lbx_profile_enable(self.handle, int(enable)))
 
+    def set_check(self, on=True):
+        """CHECK_MATPOL_MUL as a switch: check every whole product at a random point
+        (a mismatch raises LBX_ERR_CHECK_FAILED)."""
+        check(lib().lbx_set_check(self.handle, 1 if on else 0))
+
     def stage_times(self):
         labels = (ctypes.c_char_p * 32)()
         ms = (ctypes.c_float * 32)()
@@ -147,10 +155,10 @@ def default_context():
 from .polymat import (PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain,  # noqa: E402
                       PolynomialMatrixFFTPrimeMulDomain, PolynomialMatrixNaiveMulDomain,
                       PolynomialMatrixThreePrimesFFTMulDomain, FFT, ntt_forward, ntt_inverse, mul_mp, mul_batch,
-                      OrderBasis)
+                      OrderBasis, check_mul)
 
 __all__ = ["Context", "LinboxError", "PolynomialMatrix", "PolynomialMatrixMulDomain", "PolynomialMatrixFFTMulDomain",
            "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain",
-           "PolynomialMatrixThreePrimesFFTMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis",
+           "PolynomialMatrixThreePrimesFFTMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis", "check_mul",
            "POLFIRST", "MATFIRST", "MATROWFIRST", "METHOD_AUTO", "METHOD_PLAIN_U64", "METHOD_INT8_MFMA", "lib",
            "build", "default_context", "EXPORTED"]

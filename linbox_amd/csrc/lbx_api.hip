@@ -589,6 +589,7 @@ lbx_status lbx_init(lbx_ctx **out, int device)
     }
     lbx_ctx *ctx = new lbx_ctx();
     ctx->device = device;
+    if (const char *e = std::getenv("LBX_CHECK_MUL")) ctx->check = std::atoi(e) != 0;
     *out = ctx;
     return LBX_OK;
 }
@@ -642,6 +643,7 @@ const char *lbx_strerror(lbx_status st)
     case LBX_ERR_UNSUPPORTED: return "unsupported size";
     case LBX_ERR_NO_DEVICE: return "no gfx950 device";
     case LBX_ERR_WORKSPACE_TOO_SMALL: return "the product needs more device memory than the fixed workspace holds";
+    case LBX_ERR_CHECK_FAILED: return "the product check at a random point failed";
     }
     return "unknown status";
 }
@@ -698,8 +700,18 @@ int lbx_profile_get(lbx_ctx *ctx, const char **labels, float *ms, int max)
     return n;
 }
 
+static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
+                          size_t max_rowdeg, int method, void *stream);
+
 lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
                                size_t max_rowdeg, int method, void *stream)
+{
+    const lbx_status st = mul_u32(ctx, p, c, a, b, max_rowdeg, method, stream);
+    return st == LBX_OK ? self_check(ctx, p, c, a, b, stream) : st;
+}
+
+static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
+                          size_t max_rowdeg, int method, void *stream)
 {
     if (!ctx || !a || !b || !c) return LBX_ERR_INVALID_ARG;
     LBX_TRY(check_view(a));

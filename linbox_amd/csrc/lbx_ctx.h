@@ -46,6 +46,8 @@ struct lbx_ctx {
     hipStream_t last_stream = nullptr;
     hipEvent_t last_done = nullptr;
     int profiling = 0;  // 0 off, 1 stages, 2 stages and kernels
+    bool check = false;           // check every whole product at a random point (lbx_set_check)
+    uint64_t check_state = 0x4C42582D43484B31ull;  // point generator state
     // recorded (label, start, end) event triples, resolved by lbx_profile_get
     std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> prof_pending;
     std::vector<std::string> prof_labels;
@@ -142,6 +144,12 @@ struct Staged {
 lbx_status stage_in(lbx_ctx *ctx, const char *slot, const lbx_polmat *src, size_t word_bytes, bool copy_in,
                     Staged *out);
 void release(Staged &s);
+// c = a.b at the point x over c's first deg coefficients (*equal); the
+// context's random-point check after a product (LBX_ERR_CHECK_FAILED)
+lbx_status check_product(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
+                         size_t deg, uint32_t x, int *equal, void *stream);
+lbx_status self_check(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
+                      void *stream);
 
 // Where one call's device buffers come from.  The context's workspace keeps
 // named buffers between calls (ensure) and caches twiddle tables; an arena

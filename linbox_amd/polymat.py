@@ -361,3 +361,17 @@ def mul_batch(jobs, ctx=None, method=METHOD_AUTO, stream=None):
     for i, j in enumerate(jobs):
         arr[i] = Job(j["p"], j["m"], j["k"], j["n"], j["sa"], j["sb"], j["a"], j["b"], j["c"])
     check(lib().lbx_polymat_mul_batch(ctx.handle, arr, len(jobs), method, stream))
+
+
+def check_mul(p, c, a, b, deg=0, x=1, ctx=None, stream=None):
+    """check_mul(c, a, b, deg) (matpoly-mult-fft.h:87-163) on the device at the point x
+    (the reference evaluates at 1): True when C(x) = A(x).B(x) mod p, with c taken over
+    its first deg coefficients (c.size() when 0)."""
+    ctx = ctx or default_context()
+    if a.coldim() != b.rowdim():
+        raise LinboxError(5, "a.coldim() != b.rowdim()")
+    pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+    eq = ctypes.c_int(0)
+    check(lib().lbx_polymat_check_mul_u32(ctx.handle, int(p), ctypes.byref(pc), ctypes.byref(pa), ctypes.byref(pb),
+                                          int(deg), int(x) % int(p), ctypes.byref(eq), stream))
+    return bool(eq.value)
