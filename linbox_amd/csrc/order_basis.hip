@@ -1136,14 +1136,27 @@ __global__ void __launch_bounds__(256) wrap_fix_kernel(DevView a, size_t ta, Dev
                                                        size_t P, int m, uint32_t p, uint64_t mu, uint32_t r64,
                                                        bool fix, const uint64_t *d, uint64_t *deg)
 {
-    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (e == 0) *deg = d[0] + d[1];
-    if (!fix || e >= (size_t)m * m) return;
-    const size_t i = e / m, j = e - i * m;
+    // one workgroup per row i of T, one thread per column j
+    __shared__ uint32_t arow[kMbMaxM];
+    const int i = blockIdx.x, j = threadIdx.x;
+    if (i == 0 && j == 0) *deg = d[0] + d[1];
+    if (!fix) return;
+    if (j < m) arow[j] = a.base[view_index(a, (size_t)i * m + j, ta)];
+    __syncthreads();
+    if (j >= m) return;
+    // the loads of column j are independent: eight in flight at a time
     Acc acc;
-    for (int q = 0; q < m; ++q)
-        acc.add(a.base[view_index(a, i * m + q, ta)], b.base[view_index(b, (size_t)q * m + j, tb)]);
+    int q = 0;
+    for (; q + 8 <= m; q += 8) {
+        uint32_t bv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) bv[u] = b.base[view_index(b, (size_t)(q + u) * m + j, tb)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc.add(arow[q + u], bv[u]);
+    }
+    for (; q < m; ++q) acc.add(arow[q], b.base[view_index(b, (size_t)q * m + j, tb)]);
     const uint32_t t = acc.get(p, mu, r64);
+    const size_t e = (size_t)i * m + j;
     uint32_t *c0 = c.base + view_index(c, e, 0);
     const uint32_t v = *c0;
     *c0 = v >= t ? v - t : v + (p - t);
@@ -1250,8 +1263,8 @@ lbx_status sigma_product(PmRun &R, uint32_t *s2, size_t o2, uint32_t *s1, size_t
         c.size = order + 1;
         LBX_TRY(product(R, a, b, c, m, m, m));
     }
-    hipLaunchKernelGGL(wrap_fix_kernel, dim3(wrap ? (unsigned)((m * m + 255) / 256) : 1u), dim3(256), 0, R.s, a, o2,
-                       b, o1, sigma, P, (int)m, R.p, mu, r64, wrap, d, deg);
+    hipLaunchKernelGGL(wrap_fix_kernel, dim3(wrap ? (unsigned)m : 1u), dim3(256), 0, R.s, a, o2, b, o1, sigma, P,
+                       (int)m, R.p, mu, r64, wrap, d, deg);
     return from_hip(hipGetLastError());
 }
 
