@@ -160,7 +160,7 @@ lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t
 
 /* PolynomialMatrixFFTMulDomain<Modular<integer>>::mul (matpoly-mult-fft-multiprecision.inl:611-630,
  * CRT in ZRing<integer>::mul_crtla :141-277) for a multiprecision modulus P
- * given as nlimbs (1..16) little-endian uint32 words; coefficients hold
+ * given as nlimbs (1..64) little-endian uint32 words; coefficients hold
  * nlimbs words each, in [0,P).  The integer product is computed modulo FFT
  * primes below 2^31 (RandomFFTPrime::generatePrimes order, random-fftprime.h:107-133)
  * covering 2 P^2 k min(sa,sb), recombined by CRT on the device, and reduced mod P.
@@ -175,6 +175,21 @@ lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t
 lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nlimbs, lbx_polmat *c,
                               const lbx_polmat *a, const lbx_polmat *b, size_t max_rowdeg, int method,
                               void *stream);
+
+/* PolynomialMatrixFFTMulDomain<Givaro::ZRing<integer>>::mul(c, a, b, max_rowdeg)
+ * (matpoly-mult-fft-multiprecision.inl:43-89, mul_crtla :141-277): the product
+ * over the integers.  Coefficients of a and b are ab_limbs (1..32) and those of
+ * c c_limbs (1..80) little-endian uint32 words each, two's complement.  As in
+ * the reference, maxA = 2^(largest bit size of a's coefficients), likewise
+ * maxB, and bound = 2 maxA maxB k min(sa,sb) (:78-86); the product is computed
+ * modulo FFT primes below 2^31 (generatePrimes order) whose product exceeds the
+ * bound and reconstructed in the symmetric range.  c_limbs words must hold
+ * maxA maxB k min(sa,sb) with its sign, else LBX_ERR_SIZE_MISMATCH; more than
+ * 96 primes gives LBX_ERR_NOT_ENOUGH_PRIMES.  The coefficient count s and the
+ * max_rowdeg wrap are those of lbx_polymat_mul_mp.  The call reads the inputs'
+ * bit sizes back before it queues the products. */
+lbx_status lbx_polymat_mul_zz(lbx_ctx *ctx, lbx_polmat *c, int c_limbs, const lbx_polmat *a, const lbx_polmat *b,
+                              int ab_limbs, size_t max_rowdeg, int method, void *stream);
 
 /* One product of a batch (device pointers, polfirst storage, whole matrices). */
 typedef struct lbx_job {

@@ -30,8 +30,8 @@
 
 namespace lbx {
 
-constexpr int kMaxPrimes = 64;
-constexpr int kMaxLimbs = 16;
+constexpr int kMaxPrimes = 160;
+constexpr int kMaxLimbs = 64;
 
 struct MpPlan {
     std::vector<uint32_t> primes;
@@ -726,6 +726,249 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
     return LBX_OK;
 }
 
+// ---- products over the integers: PolynomialMatrixFFTMulDomain<ZRing<integer>> ----
+// (matpoly-mult-fft-multiprecision.inl:43-89 with mul_crtla, :141-277).
+// Coefficients are nl-word (inputs) and cl-word (output) two's complement
+// integers.  As in the reference, maxA = 2^(largest bit size of a's
+// coefficients), likewise maxB, and bound = 2 maxA maxB k min(sa,sb) (:78-86);
+// the product is taken modulo FFT primes whose product M exceeds the bound and
+// reconstructed in the symmetric range.  The primes are odd, so the mixed-radix
+// digits of (M-1)/2 are all (p_i-1)/2: a value is negative (X > (M-1)/2) when
+// its Garner digits, read from the most significant, first exceed them.  The
+// output X - [X > (M-1)/2] M is formed modulo 2^(32 cl), which holds it.
+constexpr int kZzMaxIn = 32;      // words of an input coefficient
+constexpr int kZzMaxOut = 80;     // words of an output coefficient
+constexpr int kZzMaxPrimes = 96;
+
+namespace {
+
+// Constant block: primes, Shoup constants of 1, 2^32 mod p_i and its Shoup
+// constant, 2^(32 nl) mod p_i, Garner inverses (p_j mod p_i)^-1 with their
+// Shoup constants at 2(i r + j), then W_i = prod_{j<i} p_j mod 2^(32 cl) in cl
+// words each, then M mod 2^(32 cl).
+struct ZzLayout {
+    int r, cl;
+    __host__ __device__ int primes() const { return 0; }
+    __host__ __device__ int onep() const { return r; }
+    __host__ __device__ int r32() const { return 2 * r; }
+    __host__ __device__ int r32p() const { return 3 * r; }
+    __host__ __device__ int top() const { return 4 * r; }
+    __host__ __device__ int inv() const { return 5 * r; }
+    __host__ __device__ int W() const { return 5 * r + 2 * r * r; }
+    __host__ __device__ int M() const { return W() + r * cl; }
+    __host__ __device__ int words() const { return M() + cl; }
+};
+
+// Largest bit size of the absolute values of nl-word two's complement
+// coefficients (Givaro's bitsize, 0 for 0), max-accumulated into *out.
+__global__ void __launch_bounds__(256) zz_logmax_kernel(DevView in, int nl, size_t entries, size_t size,
+                                                        unsigned *__restrict__ out)
+{
+    const size_t total = entries * size;
+    unsigned best = 0;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        const uint32_t *src = in.base + elem_index(in, idx, size) * nl;
+        const bool neg = src[nl - 1] >> 31;
+        // |x|: the words themselves, or their two's complement negation
+        uint32_t carry = 1;
+        unsigned bits = 0;
+        for (int l = 0; l < nl; ++l) {
+            uint32_t w = src[l];
+            if (neg) {
+                const uint32_t t = ~w + carry;
+                carry = carry && t == 0;
+                w = t;
+            }
+            if (w) bits = 32 * l + 32 - __builtin_clz(w);
+        }
+        best = bits > best ? bits : best;
+    }
+    for (int off = 32; off; off >>= 1) {
+        const unsigned o = __shfl_xor(best, off);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0 && best) atomicMax(out, best);
+}
+
+// Residues mod every prime of nl-word two's complement coefficients: Horner over
+// the words as unsigned, minus 2^(32 nl) when the sign bit is set.
+template <int NLM>
+__global__ void __launch_bounds__(256) zz_residue_kernel(DevView in, int nl, size_t entries, size_t size,
+                                                         const uint32_t *__restrict__ K, int r,
+                                                         uint32_t *__restrict__ out)
+{
+    const size_t total = entries * size;
+    const ZzLayout lay{r, 0};
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        const uint32_t *src = in.base + elem_index(in, idx, size) * nl;
+        uint32_t limb[NLM];
+#pragma unroll
+        for (int l = 0; l < NLM; ++l) limb[l] = l < nl ? src[l] : 0u;
+        const bool neg = src[nl - 1] >> 31;
+        for (int i = 0; i < r; ++i) {
+            const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
+            const uint32_t c = K[lay.r32() + i], cp = K[lay.r32p() + i];
+            uint32_t v = 0;
+#pragma unroll
+            for (int l = NLM - 1; l >= 0; --l)
+                if (l < nl) v = add_mod(shoup_mul(v, c, cp, p), shoup_mul(limb[l], 1u, onep, p), p);
+            if (neg) v = sub_mod(v, K[lay.top() + i], p);
+            out[(size_t)i * total + idx] = v;
+        }
+    }
+}
+
+// Garner digits of each coefficient's r residues, the sign test against the
+// digits (p_i-1)/2, and X - [negative] M modulo 2^(32 cl) in cl words.
+__global__ void __launch_bounds__(128) zz_crt_kernel(const uint32_t *__restrict__ res, size_t entries, size_t size,
+                                                     const uint32_t *__restrict__ K, int r, int cl, DevView out)
+{
+    const size_t total = entries * size;
+    const ZzLayout lay{r, cl};
+    for (size_t idx = (size_t)blockIdx.x * 128 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 128) {
+        uint32_t v[kZzMaxPrimes];
+        uint32_t acc[kZzMaxOut];
+        for (int l = 0; l < cl; ++l) acc[l] = 0;
+        for (int i = 0; i < r; ++i) {
+            const uint32_t p = K[lay.primes() + i], onep = K[lay.onep() + i];
+            uint32_t t = res[(size_t)i * total + idx];
+            for (int j = 0; j < i; ++j) {
+                const uint32_t vj = shoup_mul(v[j], 1u, onep, p);
+                t = shoup_mul(sub_mod(t, vj, p), K[lay.inv() + 2 * (i * r + j)], K[lay.inv() + 2 * (i * r + j) + 1], p);
+            }
+            v[i] = t;
+            // acc += v_i W_i (mod 2^(32 cl))
+            const uint32_t *w = K + lay.W() + i * cl;
+            uint64_t carry = 0;
+            for (int l = 0; l < cl; ++l) {
+                const uint64_t s = (uint64_t)w[l] * t + acc[l] + carry;
+                acc[l] = (uint32_t)s;
+                carry = s >> 32;
+            }
+        }
+        bool neg = false;
+        for (int i = r - 1; i >= 0; --i) {
+            const uint32_t h = (K[lay.primes() + i] - 1) / 2;
+            if (v[i] != h) {
+                neg = v[i] > h;
+                break;
+            }
+        }
+        if (neg) {
+            const uint32_t *Mm = K + lay.M();
+            uint64_t borrow = 0;
+            for (int l = 0; l < cl; ++l) {
+                const uint64_t d = (uint64_t)acc[l] - Mm[l] - borrow;
+                acc[l] = (uint32_t)d;
+                borrow = (d >> 32) & 1;
+            }
+        }
+        uint32_t *dst = out.base + elem_index(out, idx, size) * cl;
+        for (int l = 0; l < cl; ++l) dst[l] = acc[l];
+    }
+}
+
+// Largest coefficient bit size of a view (one pass, one read-back).
+lbx_status zz_logmax(lbx_ctx *ctx, const DevView &v, int nl, unsigned *bits, hipStream_t s)
+{
+    void *w = nullptr;
+    LBX_TRY(ensure(ctx, "zz_logmax", 256, &w));
+    LBX_TRY_HIP(hipMemsetAsync(w, 0, 4, s));
+    const size_t entries = v.rows * v.cols;
+    hipLaunchKernelGGL(zz_logmax_kernel, dim3(grid_of(entries * v.size, 256)), dim3(256), 0, s, v, nl, entries, v.size,
+                       (unsigned *)w);
+    LBX_TRY_HIP(hipGetLastError());
+    LBX_TRY_HIP(hipMemcpyAsync(bits, w, 4, hipMemcpyDeviceToHost, s));
+    LBX_TRY_HIP(hipStreamSynchronize(s));
+    return LBX_OK;
+}
+
+}  // namespace
+
+lbx_status zz_product(lbx_ctx *ctx, int nl, int cl, const DevView &a, const DevView &b, const DevView &c, size_t m,
+                      size_t k, size_t n, size_t sc, int method, hipStream_t s)
+{
+    const size_t sa = a.size, sb = b.size;
+    const bool full = sc == sa + sb - 1;
+    unsigned lpts = 0;
+    size_t pts = 1;
+    while (pts < sc) {  // mul_crtla :151-154
+        pts <<= 1;
+        ++lpts;
+    }
+    unsigned la = 0, lb = 0;
+    LBX_TRY(zz_logmax(ctx, a, nl, &la, s));
+    LBX_TRY(zz_logmax(ctx, b, nl, &lb, s));
+    // bound = 2 maxA maxB k min(sa,sb), maxA = 2^la, maxB = 2^lb (:78-86);
+    // |coefficient| < bound/2 must fit cl words with its sign
+    Limbs bound{1};
+    for (unsigned i = 0; i < la + lb + 1; ++i) limbs_mul_small(bound, 2);
+    limbs_mul_small(bound, (uint32_t)k);
+    limbs_mul_small(bound, (uint32_t)(sa < sb ? sa : sb));
+    limbs_trim(bound);
+    size_t bits = 32 * (bound.size() - 1) + 32 - __builtin_clz(bound.back());  // bound < 2^bits
+    if (bits > (size_t)32 * cl) return LBX_ERR_SIZE_MISMATCH;  // bound/2 < 2^(bits-1) needs bits-1 bits and a sign
+    std::vector<uint32_t> primes;
+    if (!generate_primes((uint64_t)1 << 31, bound, lpts, primes) || (int)primes.size() > kZzMaxPrimes)
+        return LBX_ERR_NOT_ENOUGH_PRIMES;
+    const int r = (int)primes.size();
+    const ZzLayout lay{r, cl};
+    std::vector<uint32_t> K((size_t)lay.words(), 0);
+    Limbs W{1};  // prod_{j<i} p_j, kept below 2^(32 cl)
+    for (int i = 0; i < r; ++i) {
+        const uint32_t p = primes[i];
+        K[lay.primes() + i] = p;
+        K[lay.onep() + i] = shoup_pre(1, p);
+        const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p);
+        K[lay.r32() + i] = r32;
+        K[lay.r32p() + i] = shoup_pre(r32, p);
+        uint64_t top = 1;
+        for (int l = 0; l < nl; ++l) top = top * r32 % p;
+        K[lay.top() + i] = (uint32_t)top;
+        for (int j = 0; j < i; ++j) {
+            const uint32_t inv = invmod_h(primes[j] % p, p);
+            K[lay.inv() + 2 * (i * r + j)] = inv;
+            K[lay.inv() + 2 * (i * r + j) + 1] = shoup_pre(inv, p);
+        }
+        for (int l = 0; l < cl; ++l) K[lay.W() + i * cl + l] = l < (int)W.size() ? W[l] : 0;
+        limbs_mul_small(W, p);
+        if ((int)W.size() > cl) W.resize(cl);
+    }
+    for (int l = 0; l < cl; ++l) K[lay.M() + l] = l < (int)W.size() ? W[l] : 0;
+    void *dk, *ra, *rb, *rc;
+    LBX_TRY(ensure(ctx, "zz_const", K.size() * 4, &dk));
+    // the stream is idle here (zz_logmax waited for it): a blocking copy from K
+    LBX_TRY_HIP(hipMemcpy(dk, K.data(), K.size() * 4, hipMemcpyHostToDevice));
+    LBX_TRY(ensure(ctx, "mp_ra", (size_t)r * m * k * sa * 4, &ra));
+    LBX_TRY(ensure(ctx, "mp_rb", (size_t)r * k * n * sb * 4, &rb));
+    LBX_TRY(ensure(ctx, "mp_rc", (size_t)r * m * n * sc * 4, &rc));
+    auto residues = nl <= 8 ? zz_residue_kernel<8> : zz_residue_kernel<kZzMaxIn>;
+    hipLaunchKernelGGL(residues, dim3(grid_of(m * k * sa, 256)), dim3(256), 0, s, a, nl, m * k, sa,
+                       (const uint32_t *)dk, r, (uint32_t *)ra);
+    LBX_TRY_HIP(hipGetLastError());
+    hipLaunchKernelGGL(residues, dim3(grid_of(k * n * sb, 256)), dim3(256), 0, s, b, nl, k * n, sb,
+                       (const uint32_t *)dk, r, (uint32_t *)rb);
+    LBX_TRY_HIP(hipGetLastError());
+    lbx_status bst = LBX_ERR_UNSUPPORTED;
+    if (full && r >= kBatchMinPrimes)
+        bst = primes_as_batch(ctx, primes, (const uint32_t *)ra, m * k * sa, (const uint32_t *)rb, k * n * sb,
+                              (uint32_t *)rc, m, k, n, sa, sb, lpts, method, s);
+    if (bst != LBX_ERR_UNSUPPORTED) LBX_TRY(bst);
+    for (int i = 0; i < r && bst == LBX_ERR_UNSUPPORTED; ++i) {
+        DevView va{(uint32_t *)ra + (size_t)i * m * k * sa, LBX_POLFIRST, m, k, sa, 0, sa};
+        DevView vb{(uint32_t *)rb + (size_t)i * k * n * sb, LBX_POLFIRST, k, n, sb, 0, sb};
+        DevView vc{(uint32_t *)rc + (size_t)i * m * n * sc, LBX_POLFIRST, m, n, sc, 0, sc};
+        if (sa + sb <= 2 && sc == 1)
+            LBX_TRY(constant_product(ctx, primes[i], va, vb, vc, m, k, n, s));
+        else
+            LBX_TRY(fft_prime_product(ctx, primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
+    }
+    hipLaunchKernelGGL(zz_crt_kernel, dim3(grid_of(m * n * sc, 128)), dim3(128), 0, s, (const uint32_t *)rc, m * n,
+                       sc, (const uint32_t *)dk, r, cl, c);
+    LBX_TRY_HIP(hipGetLastError());
+    return LBX_OK;
+}
+
 }  // namespace lbx
 
 using namespace lbx;
@@ -771,6 +1014,52 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
                         method, s, nullptr, 0, sc);
     if (st == LBX_OK && sc_.owned) {
         hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * wb, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        st = from_hip(e);
+    }
+    if (host_io) (void)hipStreamSynchronize(s);
+    release(sa_);
+    release(sb_);
+    release(sc_);
+    if (st == LBX_OK) c->size = sc;
+    return st;
+}
+
+extern "C" lbx_status lbx_polymat_mul_zz(lbx_ctx *ctx, lbx_polmat *c, int c_limbs, const lbx_polmat *a,
+                                         const lbx_polmat *b, int ab_limbs, size_t max_rowdeg, int method,
+                                         void *stream)
+{
+    if (!ctx || !a || !b || !c) return LBX_ERR_INVALID_ARG;
+    if (ab_limbs < 1 || ab_limbs > kZzMaxIn || c_limbs < 1 || c_limbs > kZzMaxOut) return LBX_ERR_UNSUPPORTED;
+    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+    LBX_TRY(check_view(a));
+    LBX_TRY(check_view(b));
+    LBX_TRY(check_view(c));
+    if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
+    if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
+    const size_t m = a->rows, k = a->cols, n = b->cols;
+    const size_t sc = max_rowdeg ? max_rowdeg + 1 : a->size + b->size - 1;
+    if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
+    size_t pts = 1;
+    while (pts < sc) pts <<= 1;
+    if (a->size > pts || b->size > pts) return LBX_ERR_SIZE_MISMATCH;
+    if (k > 0xffffffffu || a->size > 0xffffffffu || sc > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DeviceGuard guard(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    StreamOrder stream_order(ctx, s);
+    const size_t wi = 4 * (size_t)ab_limbs, wo = 4 * (size_t)c_limbs;
+    Staged sa_, sb_, sc_;
+    lbx_status st = stage_in(ctx, "stage_a", a, wi, true, &sa_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wi, true, &sb_);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wo, !(c->shift == 0 && sc >= c->storage_size), &sc_);
+    const bool host_io = sa_.owned || sb_.owned || sc_.owned;
+    if (st == LBX_OK)
+        st = zz_product(ctx, ab_limbs, c_limbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k,
+                        n, sc, method, s);
+    if (st == LBX_OK && sc_.owned) {
+        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * wo, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         st = from_hip(e);
     }

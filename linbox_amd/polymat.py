@@ -354,6 +354,45 @@ def mul_mp(P, c, a, b, ctx=None, method=METHOD_AUTO, stream=None, max_rowdeg=0):
     return c
 
 
+def mul_zz(c, a, b, ctx=None, method=METHOD_AUTO, stream=None, max_rowdeg=0):
+    """PolynomialMatrixFFTMulDomain<ZRing<integer>>::mul (matpoly-mult-fft-multiprecision.inl:74-89):
+    the product over the integers.  a and b hold a.nlimbs words per coefficient, c
+    c.nlimbs, all two's complement; c.nlimbs must hold the product's bound."""
+    ctx = ctx or default_context()
+    if a.nlimbs != b.nlimbs:
+        raise LinboxError(7, "a and b must have the same limb count")
+    pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+    check(lib().lbx_polymat_mul_zz(ctx.handle, ctypes.byref(pc), c.nlimbs, ctypes.byref(pa), ctypes.byref(pb),
+                                   a.nlimbs, max_rowdeg, method, stream))
+    c._size = pc.size
+    return c
+
+
+def to_limbs(values, nlimbs):
+    """Signed Python ints -> uint32 array of nlimbs two's complement words each."""
+    out = np.zeros((len(values), nlimbs), dtype=np.uint32)
+    mask = (1 << (32 * nlimbs)) - 1
+    for i, v in enumerate(values):
+        u = int(v) & mask
+        for l in range(nlimbs):
+            out[i, l] = (u >> (32 * l)) & 0xFFFFFFFF
+    return out.reshape(-1)
+
+
+def from_limbs(words, nlimbs):
+    """uint32 array of nlimbs two's complement words each -> signed Python ints."""
+    w = np.asarray(words, dtype=np.uint32).reshape(-1, nlimbs)
+    out = []
+    for row in w:
+        u = 0
+        for l in range(nlimbs - 1, -1, -1):
+            u = (u << 32) | int(row[l])
+        if u >> (32 * nlimbs - 1):
+            u -= 1 << (32 * nlimbs)
+        out.append(u)
+    return out
+
+
 def mul_batch(jobs, ctx=None, method=METHOD_AUTO, stream=None):
     """jobs: list of dicts with p, m, k, n, sa, sb and device pointers a, b, c."""
     ctx = ctx or default_context()

@@ -125,10 +125,10 @@ def test_fft_prime_too_short_for_length(ctx):
     assert np.array_equal(C.to_polfirst().reshape(-1), ref)
 
 
-@pytest.mark.parametrize("bits", [320, 480, 512])
+@pytest.mark.parametrize("bits", [320, 480, 512, 1024, 2048])
 def test_crt_wide_moduli(ctx, bits):
-    """Moduli of 10, 15 and 16 limbs (22 to 35 FFT primes, so every recombination kernel
-    shape runs) against the product computed with Python integers."""
+    """Moduli of 10, 15, 16, 32 and 64 limbs (22 to 133 FFT primes, so every recombination
+    kernel shape runs) against the product computed with Python integers."""
     rng = np.random.default_rng(bits)
     P = (1 << (bits - 1)) | int(rng.integers(1, 1 << 62)) << 1 | 1
     nl = (P.bit_length() + 31) // 32
