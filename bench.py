@@ -161,6 +161,7 @@ def run_gpu(args):
         raise SystemExit("bench: the plain-form product is wrong (%s)" % plain_ok)
     da.free(), db.free(), dc.free()
     host = None if args.skip_host else host_leg(ctx, lbx, a, b)
+    stream.destroy()
     ctx.close()
     return ms_int8, wall_int8, stages_int8, ms_plain, stages_plain, check, host
 
@@ -293,6 +294,7 @@ def run_batch(steps=5):
     singles()
     stream.synchronize()
     ms_single = timed(singles, 1)
+    stream.destroy()
     ctx.close()
     print("batch: %.3f ms per 1024 products (graph), %.3f ms with one call per product" % (ms_batch, ms_single),
           file=sys.stderr, flush=True)
@@ -431,6 +433,7 @@ def run_configs(reps=10):
     out["copy_polfirst_to_matfirst_128x128x8192"] = {"ms": round(ms, 4), "gbs": round(gbs, 1),
                                                      "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 3)}
     del da, db
+    stream.destroy()
     ctx.close()
     print("configs: " + ", ".join("%s %.3f ms" % (k, v["ms"]) for k, v in out.items()), file=sys.stderr, flush=True)
     return out

@@ -1107,6 +1107,16 @@ size_t mc_lds_words(size_t m, size_t n, size_t order, size_t cw)
 // column).  By default about m^2 n / 8192 groups (at most 16): the work each
 // group repeats (sort and elimination) against the work it shares out.
 // LBX_MBASIS_GROUPS=N asks for N groups, 0 for the one-workgroup kernel.
+// The grid of m_basis_cols_kernel must fit on the device at once: one group
+// per CU (its LDS takes most of a CU's 160 KiB).
+lbx_status mc_check_groups(int G)
+{
+    int dev = 0, cus = 0;
+    LBX_TRY_HIP(hipGetDevice(&dev));
+    LBX_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return G <= cus ? LBX_OK : LBX_ERR_UNSUPPORTED;
+}
+
 void cols_split(size_t m, size_t n, size_t order, int *G, int *cw)
 {
     *G = 0;
@@ -1203,8 +1213,15 @@ lbx_status run_m_basis(PmRun &R, const DevView &serie, const DevView &sigma, siz
         uint32_t p = R.p;
         void *args[] = {&S,   &out, &shift_io, &deg, &part, &bar,  &order_i, &m_i, &n_i, &cw,
                         &oc,  &p,   &mu,       &f62, &pinv, &r2m, &one};
-        // every group waits for the others at each step: they must all be resident
-        LBX_TRY_HIP(hipLaunchCooperativeKernel(kern, dim3(G), dim3(kMbThreads), args, (unsigned)lds, R.s));
+        // Every group waits for the others at each step, so all G must become
+        // resident.  G <= 64 groups of one CU each fit on the device together,
+        // and groups still queued behind other streams' work are dispatched as
+        // that work drains (the barrier's poll limit bounds the wait), so an
+        // ordinary launch on the caller's stream serves.  A cooperative launch
+        // would go through a queue of its own, with a cross-queue hand-off
+        // before and after every base case.
+        LBX_TRY(mc_check_groups(G));
+        LBX_TRY_HIP(hipLaunchKernel(kern, dim3(G), dim3(kMbThreads), args, lds, R.s));
         return LBX_OK;
     }
     // staged words: enough for every coefficient of a step, at most kMbStageWords

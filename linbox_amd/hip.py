@@ -125,3 +125,17 @@ class Stream:
     @property
     def value(self):
         return self.handle.value
+
+    def destroy(self):
+        """Destroy the stream after waiting for its work.  A blocking stream
+        that outlives its use costs every later launch on the null stream."""
+        if self.handle.value:
+            runtime().hipStreamSynchronize(self.handle)
+            runtime().hipStreamDestroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
