@@ -87,7 +87,8 @@ typedef struct lbx_ctx lbx_ctx;
 lbx_status lbx_init(lbx_ctx **ctx, int device);
 lbx_status lbx_destroy(lbx_ctx *ctx);
 const char *lbx_strerror(lbx_status status);
-/* Release cached device workspace (kept between calls otherwise). */
+/* Release the cached device workspace and the pinned host buffers of host-operand
+ * copies (kept between calls otherwise). */
 lbx_status lbx_trim(lbx_ctx *ctx);
 
 /* PolynomialMatrixFFTMulDomain<Modular<uint32_t,uint64_t>>::mul(c,a,b,max_rowdeg)

@@ -128,6 +128,10 @@ class Context:
         except Exception:
             pass
 
+    def trim(self):
+        """Release the cached device workspace and the pinned host buffers (lbx_trim)."""
+        check(lib().lbx_trim(self.handle))
+
     def profile(self, enable=True):
         """enable: False/0 off, True/1 stage times, 2 stage and per-kernel times."""
         check(lib().lbx_profile_enable(self.handle, int(enable)))

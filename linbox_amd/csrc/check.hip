@@ -106,9 +106,9 @@ lbx_status check_product(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lb
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged sa, sb, sc;
-    lbx_status st = stage_in(ctx, "check_a", a, 4, true, &sa);
-    if (st == LBX_OK) st = stage_in(ctx, "check_b", b, 4, true, &sb);
-    if (st == LBX_OK) st = stage_in(ctx, "check_c", c, 4, true, &sc);
+    lbx_status st = stage_in(ctx, "check_a", a, 4, true, &sa, s);
+    if (st == LBX_OK) st = stage_in(ctx, "check_b", b, 4, true, &sb, s);
+    if (st == LBX_OK) st = stage_in(ctx, "check_c", c, 4, true, &sc, s);
     void *ws = nullptr;
     const size_t words = m * k + k * n + m * n;
     if (st == LBX_OK) st = ensure(ctx, "check_eval", words * 4 + 256, &ws);

@@ -183,13 +183,11 @@ extern "C" lbx_status lbx_polymat_copy_u32(lbx_ctx *ctx, lbx_polmat *dst, const 
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged ss, sd;
-    lbx_status st = stage_in(ctx, "stage_a", src, 4, true, &ss);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", dst, 4, true, &sd);  // the rest of dst's storage is kept
+    lbx_status st = stage_in(ctx, "stage_a", src, 4, true, &ss, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", dst, 4, true, &sd, s);  // the rest of dst's storage is kept
     if (st == LBX_OK) st = from_hip(launch_copy(devview(src, ss.dev), devview(dst, sd.dev), entries, cnt, beg, start, s));
     if (st == LBX_OK && sd.owned) {
-        hipError_t e = hipMemcpyAsync(dst->data, sd.dev, storage_words(dst) * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, dst->data, sd.dev, storage_words(dst) * 4, s);
     } else if (st == LBX_OK && ss.owned) {
         st = from_hip(hipStreamSynchronize(s));
     }

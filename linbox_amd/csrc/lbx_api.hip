@@ -549,7 +549,7 @@ lbx_status ArenaWorkspace::twiddle_table(uint32_t p, unsigned L, uint32_t w, con
 }
 
 lbx_status stage_in(lbx_ctx *ctx, const char *slot, const lbx_polmat *src, size_t word_bytes, bool copy_in,
-                    Staged *out)
+                    Staged *out, hipStream_t s)
 {
     out->v = *src;
     if (is_device_ptr(src->data)) {
@@ -559,7 +559,7 @@ lbx_status stage_in(lbx_ctx *ctx, const char *slot, const lbx_polmat *src, size_
     const size_t bytes = storage_words(src) * word_bytes;
     LBX_TRY(ensure(ctx, slot, bytes, &out->dev));
     out->owned = true;
-    if (copy_in) LBX_TRY_HIP(hipMemcpy(out->dev, src->data, bytes, hipMemcpyHostToDevice));
+    if (copy_in) LBX_TRY(host_to_device(ctx, out->dev, src->data, bytes, s));
     return LBX_OK;
 }
 
@@ -603,6 +603,7 @@ lbx_status lbx_trim(lbx_ctx *ctx)
     for (auto &kv : ctx->bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     ctx->bufs.clear();
+    stager_release(ctx);
     return LBX_OK;
 }
 
@@ -621,6 +622,7 @@ lbx_status lbx_destroy(lbx_ctx *ctx)
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
         mp_release(ctx);
         batch_release(ctx);
+        stager_release(ctx);
     }
     delete ctx;
     return LBX_OK;
@@ -748,9 +750,9 @@ static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_pol
     Staged sa_, sb_, sc_;
     // host storage of c is copied in only when the call leaves part of it as it is
     const bool c_in = !(c->shift == 0 && deg + 1 >= c->storage_size);
-    lbx_status st = stage_in(ctx, "stage_a", a, 4, true, &sa_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, 4, true, &sb_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, 4, c_in, &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", a, 4, true, &sa_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, 4, true, &sb_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, 4, c_in, &sc_, s);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK) {
         const DevView da = devview(a, sa_.dev), db = devview(b, sb_.dev), dc = devview(c, sc_.dev);
@@ -763,9 +765,7 @@ static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_pol
             st = fft_prime_product(ctx, p, L, da, db, dc, m, k, n, deg + 1, method, s);
     }
     if (st == LBX_OK && sc_.owned) {
-        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * 4, s);
     } else if (st == LBX_OK && host_io) {
         st = from_hip(hipStreamSynchronize(s));
     }
@@ -794,16 +794,14 @@ lbx_status lbx_polymat_mul_fft_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, size
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", &va, 4, true, &sa_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", &vb, 4, true, &sb_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", &vc, 4, false, &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", &va, 4, true, &sa_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", &vb, 4, true, &sb_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", &vc, 4, false, &sc_, s);
     if (st == LBX_OK)
         st = fft_prime_product(ctx, p, lpts, devview(&va, sa_.dev), devview(&vb, sb_.dev), devview(&vc, sc_.dev), m,
                                k, n, pts, method, s);
     if (st == LBX_OK && sc_.owned) {
-        hipError_t e = hipMemcpyAsync(c, sc_.dev, m * n * pts * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, c, sc_.dev, m * n * pts * 4, s);
     }
     if (sa_.owned || sb_.owned || sc_.owned) (void)hipStreamSynchronize(s);
     release(sa_);
@@ -836,15 +834,13 @@ static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t ro
     LBX_TRY(twiddles(ctx, p, lpts, w, &tw));
     lbx_polmat v{(void *)data, LBX_POLFIRST, count, 1, N, N, 0};
     Staged sd;
-    LBX_TRY(stage_in(ctx, "stage_a", &v, 4, true, &sd));
+    LBX_TRY(stage_in(ctx, "stage_a", &v, 4, true, &sd, s));
     const DevView dv = devview(&v, sd.dev);
     hipError_t e = inverse ? launch_ntt_inverse((uint32_t *)sd.dev, count, lpts, tw, p, dv, N, 1, s)
                            : launch_ntt_forward(dv, count, lpts, tw, p, (uint32_t *)sd.dev, s);
     lbx_status st = from_hip(e);
     if (st == LBX_OK && sd.owned) {
-        e = hipMemcpyAsync(data, sd.dev, count * N * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, data, sd.dev, count * N * 4, s);
     }
     if (sd.owned) (void)hipStreamSynchronize(s);
     release(sd);

@@ -19,8 +19,9 @@ struct Buffer {
     size_t bytes = 0;
 };
 
-struct MpPlan;     // cached CRT constants of the multiprecision route (mp.hip)
-struct BatchPlan;  // cached item lists and graph of a batch shape (batch.hip)
+struct MpPlan;      // cached CRT constants of the multiprecision route (mp.hip)
+struct BatchPlan;   // cached item lists and graph of a batch shape (batch.hip)
+struct HostStager;  // pinned buffers and copy threads for host operands (staging.hip)
 
 }  // namespace lbx
 
@@ -45,6 +46,7 @@ struct lbx_ctx {
     // an event recorded at its end (see StreamOrder)
     hipStream_t last_stream = nullptr;
     hipEvent_t last_done = nullptr;
+    lbx::HostStager *stager = nullptr;  // created by the first large host copy
     int profiling = 0;  // 0 off, 1 stages, 2 stages and kernels
     bool check = false;           // check every whole product at a random point (lbx_set_check)
     uint64_t check_state = 0x4C42582D43484B31ull;  // point generator state
@@ -141,9 +143,18 @@ struct Staged {
     void *dev = nullptr;
     bool owned = false;  // dev is a staging copy of host storage
 };
+// With copy_in the host storage is copied on stream s (queued ahead of the
+// call's kernels; the host memory may be reused on return).
 lbx_status stage_in(lbx_ctx *ctx, const char *slot, const lbx_polmat *src, size_t word_bytes, bool copy_in,
-                    Staged *out);
+                    Staged *out, hipStream_t s);
 void release(Staged &s);
+// bytes from host to device memory on stream s (returns once the host memory
+// has been read), and from device to host memory after the work queued on s
+// (returns once the host memory holds them); large copies go through the
+// context's pinned buffers.
+lbx_status host_to_device(lbx_ctx *ctx, void *dev, const void *host, size_t bytes, hipStream_t s);
+lbx_status device_to_host(lbx_ctx *ctx, void *host, const void *dev, size_t bytes, hipStream_t s);
+void stager_release(lbx_ctx *ctx);
 // c = a.b at the point x over c's first deg coefficients (*equal); the
 // context's random-point check after a product (LBX_ERR_CHECK_FAILED)
 lbx_status check_product(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,

@@ -109,15 +109,13 @@ lbx_status with_views(lbx_ctx *ctx, lbx_polmat *c, const lbx_polmat *a, const lb
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", a, 4, true, &sa_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, 4, true, &sb_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, 4, copy_c, &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", a, 4, true, &sa_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, 4, true, &sb_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, 4, copy_c, &sc_, s);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK) st = body(devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), s);
     if (st == LBX_OK && sc_.owned) {
-        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * 4, s);
     } else if (st == LBX_OK && host_io) {
         st = from_hip(hipStreamSynchronize(s));
     }

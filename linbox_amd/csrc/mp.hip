@@ -1005,17 +1005,15 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
     StreamOrder stream_order(ctx, s);
     const size_t wb = 4 * (size_t)nlimbs;
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", a, wb, true, &sa_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wb, true, &sb_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wb, !(c->shift == 0 && sc >= c->storage_size), &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", a, wb, true, &sa_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wb, true, &sb_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wb, !(c->shift == 0 && sc >= c->storage_size), &sc_, s);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK)
         st = mp_product(ctx, P, nlimbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k, n,
                         method, s, nullptr, 0, sc);
     if (st == LBX_OK && sc_.owned) {
-        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * wb, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * wb, s);
     }
     if (host_io) (void)hipStreamSynchronize(s);
     release(sa_);
@@ -1051,17 +1049,15 @@ extern "C" lbx_status lbx_polymat_mul_zz(lbx_ctx *ctx, lbx_polmat *c, int c_limb
     StreamOrder stream_order(ctx, s);
     const size_t wi = 4 * (size_t)ab_limbs, wo = 4 * (size_t)c_limbs;
     Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", a, wi, true, &sa_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wi, true, &sb_);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wo, !(c->shift == 0 && sc >= c->storage_size), &sc_);
+    lbx_status st = stage_in(ctx, "stage_a", a, wi, true, &sa_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wi, true, &sb_, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wo, !(c->shift == 0 && sc >= c->storage_size), &sc_, s);
     const bool host_io = sa_.owned || sb_.owned || sc_.owned;
     if (st == LBX_OK)
         st = zz_product(ctx, ab_limbs, c_limbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k,
                         n, sc, method, s);
     if (st == LBX_OK && sc_.owned) {
-        hipError_t e = hipMemcpyAsync(c->data, sc_.dev, storage_words(c) * wo, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        st = from_hip(e);
+        st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * wo, s);
     }
     if (host_io) (void)hipStreamSynchronize(s);
     release(sa_);

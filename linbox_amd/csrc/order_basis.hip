@@ -1361,8 +1361,8 @@ lbx_status order_basis(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_po
     R.mb.bar = (unsigned *)(base + b_S + b_W + b_part + b_sh + b_small);
 
     Staged ss, sg;
-    lbx_status st = stage_in(ctx, "stage_a", serie, 4, true, &ss);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", sigma, 4, true, &sg);
+    lbx_status st = stage_in(ctx, "stage_a", serie, 4, true, &ss, s);
+    if (st == LBX_OK) st = stage_in(ctx, "stage_c", sigma, 4, true, &sg, s);
     std::vector<uint64_t> sh(shift, shift + m);
     uint64_t d = 0;
     unsigned failed = 0;
@@ -1377,7 +1377,7 @@ lbx_status order_basis(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_po
     if (st == LBX_OK) st = from_hip(hipMemcpyAsync(&d, R.mb.deg, 8, hipMemcpyDeviceToHost, s));
     if (st == LBX_OK) st = from_hip(hipMemcpyAsync(&failed, R.mb.bar + 1, 4, hipMemcpyDeviceToHost, s));
     if (st == LBX_OK && sg.owned)
-        st = from_hip(hipMemcpyAsync(sigma->data, sg.dev, storage_words(sigma) * 4, hipMemcpyDeviceToHost, s));
+        st = device_to_host(ctx, sigma->data, sg.dev, storage_words(sigma) * 4, s);
     const lbx_status sync = from_hip(hipStreamSynchronize(s));
     if (st == LBX_OK) st = sync;
     if (st == LBX_OK && failed) st = LBX_ERR_HIP;  // a grid barrier gave up waiting
