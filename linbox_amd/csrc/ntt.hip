@@ -220,8 +220,8 @@ __device__ __forceinline__ void dif_block_stages(uint32_t *row, int sub, const u
     }
 }
 
-// Last BL DIF stages of block blockIdx.x for the 64 entries of limb row
-// blockIdx.z, inner chunk blockIdx.y; writes that 256-byte run of every
+// Last BL DIF stages of block blockIdx.z for the 64 entries of limb row
+// blockIdx.y, inner chunk blockIdx.x; writes that 256-byte run of every
 // point's block: four planes of 64 signed 8-bit digits (LIMBS), or the 64
 // centred values as int32 words (the point-major operands of the plain form).
 template <int L, int BL, bool LIMBS>
@@ -235,7 +235,9 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
     constexpr int P = B::P, T = B::T, CPR = P / 4;  // 16-byte chunks per block row
     extern __shared__ uint32_t sm[];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x, c = blockIdx.y, r = blockIdx.z;  // r: row within the chunk
+    // the chunk and row vary fastest: groups that run together fill adjacent
+    // 256-byte runs of the same points
+    const int c = blockIdx.x, r = blockIdx.y, b = blockIdx.z;  // r: row within the chunk
 #pragma unroll
     for (int u = 0; u < 4; ++u) {  // CPR lanes per entry row read its 4P-byte block run
         const int id = tid + T * u;
@@ -294,8 +296,10 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
     }
 }
 
-// First BL DIT stages of block blockIdx.x for entries 64*blockIdx.y .. +63,
-// read from the point-major result C[t][e]; writes polfirst scratch.
+// First BL DIT stages of block blockIdx.y for entries 64*blockIdx.x .. +63,
+// read from the point-major result C[t][e]; writes polfirst scratch.  The
+// entry group varies fastest, so groups that run together read adjacent
+// 256-byte runs of the same points.
 template <int L, int BL>
 __global__ void __launch_bounds__(Blk<BL>::T) dit_head_kernel(const uint32_t *__restrict__ Cm, size_t entries,
                                                               size_t e_first, size_t e_count,
@@ -306,8 +310,8 @@ __global__ void __launch_bounds__(Blk<BL>::T) dit_head_kernel(const uint32_t *__
     constexpr int N = 1 << L, P = B::P, T = B::T, TPE = B::TPE, CPR = P / 4;
     extern __shared__ uint32_t sm[];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x;
-    const size_t e0 = (size_t)blockIdx.y * 64;  // relative to e_first
+    const int b = blockIdx.y;
+    const size_t e0 = (size_t)blockIdx.x * 64;  // relative to e_first
     if (((entries | e_first) & 3) == 0 && e0 + 64 <= e_count) {
         // 16 lanes read one point's 64 consecutive entries as 16-byte chunks
 #pragma unroll 2
@@ -797,7 +801,7 @@ hipError_t launch_dif_tail_limbs(const uint32_t *X, uint8_t *Y, size_t rows_tota
         !split_ok(L, BL))
         return hipErrorInvalidValue;
     const size_t block_bytes = rpad * kchunks * 256;
-    dim3 grid((unsigned)((size_t)1 << (L - BL)), (unsigned)kchunks, (unsigned)rows_chunk);
+    dim3 grid((unsigned)kchunks, (unsigned)rows_chunk, (unsigned)((size_t)1 << (L - BL)));
     switch (L * 16 + BL) {
 #define LBX_CASE(l, bl)                                                                                            \
     case l * 16 + bl:                                                                                              \
@@ -816,7 +820,7 @@ hipError_t launch_dit_head(const uint32_t *Cm, size_t entries, size_t e_first, s
     if (e_count == 0) return hipSuccess;
     const size_t groups = (e_count + 63) / 64;
     if (groups > 65535 || !split_ok(L, BL) || e_first + e_count > entries) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((size_t)1 << (L - BL)), (unsigned)groups);
+    dim3 grid((unsigned)groups, (unsigned)((size_t)1 << (L - BL)));
     switch (L * 16 + BL) {
 #define LBX_CASE(l, bl) \
     case l * 16 + bl: return dit_head_launch<l, bl>(grid, Cm, entries, e_first, e_count, Y, tw, p, s);
