@@ -693,7 +693,7 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
     __shared__ int rowmap[kMbMaxM], tmpmap[kMbMaxM], oldrow[kMbMaxM], pivot_of[kMbMaxM], comp[kMbMaxM],
         cidx[kMbMaxM], was_piv[kMbMaxM], qtab[kMbMaxM], piv_s[kMbMaxN], piv_c[kMbMaxN];
     __shared__ uint32_t pref[kMbMaxN + 1], rinv[kMbMaxM], piv_a[kMbMaxN];  // pivots: row, column, entry
-    __shared__ int s_maxdeg, s_nc, s_rank;
+    __shared__ int s_maxdeg, s_nc, s_npub, s_turn;
 
     for (int i = tid; i < (order + 1) * cst; i += kMbThreads) {
         const int q = i / cw, jj = i - q * cw, ph = q % m, l = q / m;
@@ -836,15 +836,23 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
             }
         }
         for (int i = tid; i < m; i += kMbThreads) pivot_of[i] = -1;
+        if (tid == 0) {
+            s_npub = 0;
+            s_turn = 0;
+        }
         __syncthreads();
         // ---- 3. row rank profile and combinations (the PLUQ of :470-488) ----
         // Wave w holds rows r0 = w*ROWS .. r0+ROWS-1 of [R | Z] in registers (lane j: words
-        // j and j+64).  Blocks of rows go in order: at step w the waves from w on apply to
-        // their rows the pivots found in block w-1, then wave w runs through its own rows,
-        // each against the pivots before it, and stores the pivots' vectors.  Every row below
-        // a pivot meets it once, in pivot order, as in a pivot-by-pivot elimination; one
-        // barrier per block.  Pivot vector j: [R_s | Z_s (j words) | -scale_s | 0 ...];
-        // a row below it becomes a [R_r | Z_r] - b [pivot vector], a = R_s[c_j], b = R_r[c_j].
+        // j and j+64).  Waves take turns in order: on its turn wave w runs through its own
+        // rows, each against the pivots before it, publishes each pivot's vector as it finds
+        // it (s_npub) and then hands the turn on (s_turn).  Until its turn a wave applies to
+        // its rows every pivot already published, so the waves below apply a block's pivots
+        // while it is still being eliminated.  Every row below a pivot meets it once, in pivot
+        // order, as in a pivot-by-pivot elimination.  Pivot vector j: [R_s | Z_s (j words) |
+        // -scale_s | 0 ...]; a row below it becomes a [R_r | Z_r] - b [pivot vector],
+        // a = R_s[c_j], b = R_r[c_j].  The counters are workgroup-scope release/acquire
+        // words in LDS; a wait that runs past its poll limit raises the failure flag, and
+        // every later wait in the call then gives up at once.
         const int r0 = wave * ROWS;
         uint32_t x0[ROWS], x1[ROWS];
 #pragma unroll
@@ -853,37 +861,57 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
             x0[u] = r < m && lane < n ? V[r * vs + lane] : 0u;
             x1[u] = 0u;
         }
-        int rank = 0, jb = 0;  // pivots found so far; those from jb on are still to be applied
-        for (int w = 0; w * ROWS < m; ++w) {
-            if (jb == rank && rank == n) break;  // no pivot left to find or to apply
-            if (wave >= w) {
-                for (int j = jb; j < rank; ++j) {
-                    const int s = piv_s[j], c0 = piv_c[j];
-                    const uint32_t a = piv_a[j];
-                    const int wdt = n + j + 1;
-                    const uint32_t y0 = lane < wdt ? V[s * vs + lane] : 0u, ny0 = p - y0;
-                    // the rows' updates are independent: straight-line code lets their
-                    // multiply chains overlap
-                    if (wdt > 64) {
-                        const uint32_t y1 = lane + 64 < wdt ? V[s * vs + lane + 64] : 0u, ny1 = p - y1;
+        // applies pivots [applied, upto) to this wave's rows
+        int applied = 0;
+        auto apply_to = [&](int upto) {
+            for (; applied < upto; ++applied) {
+                const int j = applied, s = piv_s[j], c0 = piv_c[j];
+                const uint32_t a = piv_a[j];
+                const int wdt = n + j + 1;
+                const uint32_t y0 = lane < wdt ? V[s * vs + lane] : 0u, ny0 = p - y0;
+                // the rows' updates are independent: straight-line code lets their
+                // multiply chains overlap
+                if (wdt > 64) {
+                    const uint32_t y1 = lane + 64 < wdt ? V[s * vs + lane + 64] : 0u, ny1 = p - y1;
 #pragma unroll
-                        for (int u = 0; u < ROWS; ++u) {
-                            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
-                            x0[u] = amb1(a, x0[u], b, ny0, p, pinv);
-                            x1[u] = amb1(a, x1[u], b, ny1, p, pinv);
-                        }
-                    } else {
+                    for (int u = 0; u < ROWS; ++u) {
+                        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
+                        x0[u] = amb1(a, x0[u], b, ny0, p, pinv);
+                        x1[u] = amb1(a, x1[u], b, ny1, p, pinv);
+                    }
+                } else {
 #pragma unroll
-                        for (int u = 0; u < ROWS; ++u) {
-                            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
-                            x0[u] = amb1(a, x0[u], b, ny0, p, pinv);
-                        }
+                    for (int u = 0; u < ROWS; ++u) {
+                        const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)x0[u], c0);
+                        x0[u] = amb1(a, x0[u], b, ny0, p, pinv);
                     }
                 }
             }
-            if (wave == w) {
-                uint32_t aprod = pref[rank];  // the scale of the next pivot row
-                int j = rank;
+        };
+        if (r0 < m) {
+            unsigned polls = 0;
+            bool mine = false;
+            for (;;) {
+                // the turn first: once it is this wave's, every pivot of the waves
+                // before it is published and counted below
+                const int turn = __hip_atomic_load(&s_turn, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const int pub = __hip_atomic_load(&s_npub, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                apply_to(pub);
+                if (pub == n) break;  // every pivot found: these rows hold no new one
+                if (turn >= wave) {
+                    mine = true;
+                    break;
+                }
+                if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (++polls == kMcSpinLimit) {
+                    __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (mine) {
+                int j = applied;
+                uint32_t aprod = pref[j];  // the scale of the next pivot row
 #pragma unroll
                 for (int u = 0; u < ROWS; ++u) {
                     const int r = r0 + u;
@@ -895,11 +923,14 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
                         const uint32_t y1 = lane + 64 == wdt - 1 ? p - aprod : x1[u];
                         if (lane < wdt) V[r * vs + lane] = y0;
                         if (lane + 64 < wdt) V[r * vs + lane + 64] = y1;
+                        const uint32_t next = mont(aprod, a, p, pinv);
                         if (lane == 0) {
                             piv_s[j] = r;
                             piv_c[j] = c0;
                             piv_a[j] = a;
                             pivot_of[r] = j;
+                            pref[j + 1] = next;
+                            __hip_atomic_store(&s_npub, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                         const uint32_t ny0 = p - y0, ny1 = p - y1;
 #pragma unroll
@@ -908,17 +939,16 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
                             x0[v] = amb1(a, x0[v], b, ny0, p, pinv);
                             if (wdt > 64) x1[v] = amb1(a, x1[v], b, ny1, p, pinv);
                         }
-                        aprod = mont(aprod, a, p, pinv);
+                        aprod = next;
                         ++j;
-                        if (lane == 0) pref[j] = aprod;
                     }
                 }
-                if (lane == 0) s_rank = j;
+                applied = j;
+                if (lane == 0) __hip_atomic_store(&s_turn, wave + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            __syncthreads();
-            jb = rank;
-            rank = s_rank;
         }
+        __syncthreads();
+        const int rank = s_npub;
         // a non-pivot row's scale is the product of the entries of the pivots above it;
         // its combination is Z / scale: plain inverse s^-1 = mont(montpow(scale, p-2), 1)
         if (tid < 2 * 64) {
