@@ -708,7 +708,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "polymat_mul 128x128.128x128 deg 8191 p=2013265921 (length-16384 NTT)",
                    "m": M, "k": K, "n": N, "degree": SA - 1, "p": P, "ntt_points": pts,
-                   "pointwise": "int8_mfma", "parallelism": "single_gpu"},
+                   "pointwise": "int8_mfma", "parallelism": "single_gpu",
+                   "inputs": "a then b in polfirst order, uniform in [0,p) from numpy's PCG64 seeded "
+                             "0x4C494E424F58 + 2 (other legs: + 4 batch, + 6 order basis, + 7 configs)"},
         "plain_u64": {"value": round(1000.0 / ms_plain, 3), "ms_per_step": round(ms_plain, 4),
                       "stages_ms": {k: round(v, 4) for k, v in stages_plain if not k.startswith("kernel ")}},
         "int8_vs_plain": round(ms_plain / ms_int8, 3),
