@@ -625,98 +625,105 @@ using namespace lbx;
 
 extern "C" const char *lbx_batch_tuning(lbx_ctx *ctx)
 {
-    if (!ctx) return "{}";
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    static const char *tiles[3] = {"2x2", "4x4", "8x8"};
-    std::string js = "{\"plain_tile\": {";
-    const char *cls[3] = {"max(m,n)<=4", "max(m,n)<=16", "max(m,n)>16"};
-    for (int c = 0; c < 3; ++c) {
-        js += std::string(c ? ", " : "") + "\"" + cls[c] + "\": ";
-        js += ctx->tile_tuned[c] < 0 ? std::string("null") : std::string("\"") + tiles[ctx->tile_tuned[c]] + "\"";
-    }
-    js += "}, \"int8_capable\": {";
-    const char *icl[2] = {"min(m,k,n) in [16,32)", "min(m,k,n)>=32"};
-    for (int c = 0; c < 2; ++c) {
-        js += std::string(c ? ", " : "") + "\"" + icl[c] + "\": ";
-        if (ctx->form_tuned[c] < 0) {
-            js += "null";
-            continue;
+    // no C++ exception may reach a C caller (only a failed allocation throws here)
+    try {
+        if (!ctx) return "{}";
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        static const char *tiles[3] = {"2x2", "4x4", "8x8"};
+        std::string js = "{\"plain_tile\": {";
+        const char *cls[3] = {"max(m,n)<=4", "max(m,n)<=16", "max(m,n)>16"};
+        for (int c = 0; c < 3; ++c) {
+            js += std::string(c ? ", " : "") + "\"" + cls[c] + "\": ";
+            js += ctx->tile_tuned[c] < 0 ? std::string("null") : std::string("\"") + tiles[ctx->tile_tuned[c]] + "\"";
         }
-        char buf[256];
-        std::snprintf(buf, sizeof buf,
-                      "{\"form\": \"%s\", \"points_per_mfma_item\": %d, \"plain_ms\": %.4f, \"int8_ms\": %.4f}",
-                      ctx->form_tuned[c] ? "int8_mfma" : "plain_u64", ctx->ppi_tuned[c], ctx->form_ms[c][0],
-                      ctx->form_ms[c][1]);
-        js += buf;
+        js += "}, \"int8_capable\": {";
+        const char *icl[2] = {"min(m,k,n) in [16,32)", "min(m,k,n)>=32"};
+        for (int c = 0; c < 2; ++c) {
+            js += std::string(c ? ", " : "") + "\"" + icl[c] + "\": ";
+            if (ctx->form_tuned[c] < 0) {
+                js += "null";
+                continue;
+            }
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "{\"form\": \"%s\", \"points_per_mfma_item\": %d, \"plain_ms\": %.4f, \"int8_ms\": %.4f}",
+                          ctx->form_tuned[c] ? "int8_mfma" : "plain_u64", ctx->ppi_tuned[c], ctx->form_ms[c][0],
+                          ctx->form_ms[c][1]);
+            js += buf;
+        }
+        js += "}}";
+        ctx->tuning_json = js;
+        return ctx->tuning_json.c_str();
+    } catch (...) {
+        return "{}";
     }
-    js += "}}";
-    ctx->tuning_json = js;
-    return ctx->tuning_json.c_str();
 }
 
 extern "C" lbx_status lbx_polymat_mul_batch(lbx_ctx *ctx, const lbx_job *jobs, size_t njobs, int method, void *stream)
 {
-    if (!ctx || (!jobs && njobs)) return LBX_ERR_INVALID_ARG;
-    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
-    for (size_t i = 0; i < njobs; ++i) {
-        const lbx_job &j = jobs[i];
-        if (!j.a || !j.b || !j.c || !j.m || !j.k || !j.n || !j.sa || !j.sb || j.p < 2) return LBX_ERR_INVALID_ARG;
-    }
-    if (!njobs) return LBX_OK;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    // operands must be device memory; pointers already seen by the cached plan
-    // of this shape are not looked up again
-    auto pointers_ok = [&](const BatchPlan *plan, size_t j, const lbx_job &x) {
-        if (plan && j < plan->jobs.size() && plan->jobs[j].a == x.a && plan->jobs[j].b == x.b &&
-            plan->jobs[j].c == x.c)
-            return true;
-        return is_device_ptr(x.a) && is_device_ptr(x.b) && is_device_ptr(x.c);
-    };
-    DeviceGuard guard(ctx->device);
-    hipStream_t s = (hipStream_t)stream;
-    StreamOrder stream_order(ctx, s);
-    std::vector<lbx_job> bj;
-    std::vector<unsigned> Ls;
-    std::vector<size_t> rest;
-    for (size_t i = 0; i < njobs; ++i) {
-        unsigned L;
-        if (batchable(jobs[i], &L)) {
-            bj.push_back(jobs[i]);
-            Ls.push_back(L);
-        } else {
-            rest.push_back(i);
+    return guarded([&]() -> lbx_status {
+        if (!ctx || (!jobs && njobs)) return LBX_ERR_INVALID_ARG;
+        if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+        for (size_t i = 0; i < njobs; ++i) {
+            const lbx_job &j = jobs[i];
+            if (!j.a || !j.b || !j.c || !j.m || !j.k || !j.n || !j.sa || !j.sb || j.p < 2) return LBX_ERR_INVALID_ARG;
         }
-    }
-    if (!bj.empty()) {
-        const std::string sig = signature(bj);
-        auto it = ctx->batch_plans.find(sig);
-        const BatchPlan *plan = it != ctx->batch_plans.end() ? it->second : nullptr;
-        for (size_t j = 0; j < bj.size(); ++j)
-            if (!pointers_ok(plan, j, bj[j])) return LBX_ERR_INVALID_ARG;
-        LBX_TRY(run_batch_plan(ctx, bj, Ls, s));
-    }
-    for (size_t i : rest) {  // outside the batch kernels' range: single-product routes
-        const lbx_job &j = jobs[i];
-        if (!pointers_ok(nullptr, 0, j)) return LBX_ERR_INVALID_ARG;
-        lbx_polmat a{(void *)j.a, LBX_POLFIRST, j.m, j.k, j.sa, j.sa, 0};
-        lbx_polmat b{(void *)j.b, LBX_POLFIRST, j.k, j.n, j.sb, j.sb, 0};
-        lbx_polmat c{(void *)j.c, LBX_POLFIRST, j.m, j.n, j.sa + j.sb - 1, j.sa + j.sb - 1, 0};
-        const size_t deg = j.sa + j.sb - 2;
-        const unsigned L = log2_len(deg);
-        const bool single = j.sa + j.sb > 2 && j.p < (1u << 31) && L <= (unsigned)kMaxLog &&
-                            (j.p - 1) % ((uint32_t)1 << L) == 0;
-        if (single) {
-            LBX_TRY(fft_prime_product(ctx, j.p, L, devview(&a, (void *)j.a), devview(&b, (void *)j.b),
-                                      devview(&c, j.c), j.m, j.k, j.n, deg + 1, method, s));
-        } else if (j.sa + j.sb <= 2 && j.p < (1u << 31)) {
-            LBX_TRY(constant_product(ctx, j.p, devview(&a, (void *)j.a), devview(&b, (void *)j.b), devview(&c, j.c),
-                                     j.m, j.k, j.n, s));
-        } else {
-            ctx->mu.unlock();  // the multiprecision entry point takes the lock itself
-            lbx_status st = lbx_polymat_mul_mp(ctx, &j.p, 1, &c, &a, &b, 0, method, stream);
-            ctx->mu.lock();
-            LBX_TRY(st);
+        if (!njobs) return LBX_OK;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        // operands must be device memory; pointers already seen by the cached plan
+        // of this shape are not looked up again
+        auto pointers_ok = [&](const BatchPlan *plan, size_t j, const lbx_job &x) {
+            if (plan && j < plan->jobs.size() && plan->jobs[j].a == x.a && plan->jobs[j].b == x.b &&
+                plan->jobs[j].c == x.c)
+                return true;
+            return is_device_ptr(x.a) && is_device_ptr(x.b) && is_device_ptr(x.c);
+        };
+        DeviceGuard guard(ctx->device);
+        hipStream_t s = (hipStream_t)stream;
+        StreamOrder stream_order(ctx, s);
+        std::vector<lbx_job> bj;
+        std::vector<unsigned> Ls;
+        std::vector<size_t> rest;
+        for (size_t i = 0; i < njobs; ++i) {
+            unsigned L;
+            if (batchable(jobs[i], &L)) {
+                bj.push_back(jobs[i]);
+                Ls.push_back(L);
+            } else {
+                rest.push_back(i);
+            }
         }
-    }
-    return LBX_OK;
+        if (!bj.empty()) {
+            const std::string sig = signature(bj);
+            auto it = ctx->batch_plans.find(sig);
+            const BatchPlan *plan = it != ctx->batch_plans.end() ? it->second : nullptr;
+            for (size_t j = 0; j < bj.size(); ++j)
+                if (!pointers_ok(plan, j, bj[j])) return LBX_ERR_INVALID_ARG;
+            LBX_TRY(run_batch_plan(ctx, bj, Ls, s));
+        }
+        for (size_t i : rest) {  // outside the batch kernels' range: single-product routes
+            const lbx_job &j = jobs[i];
+            if (!pointers_ok(nullptr, 0, j)) return LBX_ERR_INVALID_ARG;
+            lbx_polmat a{(void *)j.a, LBX_POLFIRST, j.m, j.k, j.sa, j.sa, 0};
+            lbx_polmat b{(void *)j.b, LBX_POLFIRST, j.k, j.n, j.sb, j.sb, 0};
+            lbx_polmat c{(void *)j.c, LBX_POLFIRST, j.m, j.n, j.sa + j.sb - 1, j.sa + j.sb - 1, 0};
+            const size_t deg = j.sa + j.sb - 2;
+            const unsigned L = log2_len(deg);
+            const bool single = j.sa + j.sb > 2 && j.p < (1u << 31) && L <= (unsigned)kMaxLog &&
+                                (j.p - 1) % ((uint32_t)1 << L) == 0;
+            if (single) {
+                LBX_TRY(fft_prime_product(ctx, j.p, L, devview(&a, (void *)j.a), devview(&b, (void *)j.b),
+                                          devview(&c, j.c), j.m, j.k, j.n, deg + 1, method, s));
+            } else if (j.sa + j.sb <= 2 && j.p < (1u << 31)) {
+                LBX_TRY(constant_product(ctx, j.p, devview(&a, (void *)j.a), devview(&b, (void *)j.b), devview(&c, j.c),
+                                         j.m, j.k, j.n, s));
+            } else {
+                ctx->mu.unlock();  // the multiprecision entry point takes the lock itself
+                lbx_status st = lbx_polymat_mul_mp(ctx, &j.p, 1, &c, &a, &b, 0, method, stream);
+                ctx->mu.lock();
+                LBX_TRY(st);
+            }
+        }
+        return LBX_OK;
+    });
 }

@@ -164,13 +164,17 @@ extern "C" lbx_status lbx_polymat_check_mul_u32(lbx_ctx *ctx, uint32_t p, const 
                                                 const lbx_polmat *b, size_t deg, uint32_t x, int *equal,
                                                 void *stream)
 {
-    return check_product(ctx, p, c, a, b, deg, x, equal, stream);
+    return guarded([&]() -> lbx_status {
+        return check_product(ctx, p, c, a, b, deg, x, equal, stream);
+    });
 }
 
 extern "C" lbx_status lbx_set_check(lbx_ctx *ctx, int on)
 {
-    if (!ctx) return LBX_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->check = on != 0;
-    return LBX_OK;
+    return guarded([&]() -> lbx_status {
+        if (!ctx) return LBX_ERR_INVALID_ARG;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->check = on != 0;
+        return LBX_OK;
+    });
 }

@@ -168,31 +168,33 @@ using namespace lbx;
 extern "C" lbx_status lbx_polymat_copy_u32(lbx_ctx *ctx, lbx_polmat *dst, const lbx_polmat *src, size_t beg,
                                            size_t end, size_t start, void *stream)
 {
-    if (!ctx || !dst || !src) return LBX_ERR_INVALID_ARG;
-    LBX_TRY(check_view(src));
-    LBX_TRY(check_view(dst));
-    if (src->rows != dst->rows || src->cols != dst->cols) return LBX_ERR_SIZE_MISMATCH;
-    if (beg > end || end >= src->size) return LBX_ERR_INVALID_ARG;
-    const size_t cnt = end - beg + 1;
-    if (start > dst->size || cnt > dst->size - start) return LBX_ERR_SIZE_MISMATCH;  // _size >= end-beg+1
-    if (src->data == dst->data) return LBX_ERR_INVALID_ARG;                          // overlapping storage
-    const size_t entries = src->rows * src->cols;
-    if (entries >= ((size_t)1 << 32)) return LBX_ERR_UNSUPPORTED;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    DeviceGuard guard(ctx->device);
-    hipStream_t s = (hipStream_t)stream;
-    StreamOrder stream_order(ctx, s);
-    Staged ss, sd;
-    lbx_status st = stage_in(ctx, "stage_a", src, 4, true, &ss, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", dst, 4, true, &sd, s);  // the rest of dst's storage is kept
-    if (st == LBX_OK) st = from_hip(launch_copy(devview(src, ss.dev), devview(dst, sd.dev), entries, cnt, beg, start, s));
-    if (st == LBX_OK && sd.owned) {
-        st = device_to_host(ctx, dst->data, sd.dev, storage_words(dst) * 4, s);
-    } else if (st == LBX_OK && ss.owned) {
-        st = from_hip(hipStreamSynchronize(s));
-    }
-    if (ss.owned || sd.owned) (void)hipStreamSynchronize(s);
-    release(ss);
-    release(sd);
-    return st;
+    return guarded([&]() -> lbx_status {
+        if (!ctx || !dst || !src) return LBX_ERR_INVALID_ARG;
+        LBX_TRY(check_view(src));
+        LBX_TRY(check_view(dst));
+        if (src->rows != dst->rows || src->cols != dst->cols) return LBX_ERR_SIZE_MISMATCH;
+        if (beg > end || end >= src->size) return LBX_ERR_INVALID_ARG;
+        const size_t cnt = end - beg + 1;
+        if (start > dst->size || cnt > dst->size - start) return LBX_ERR_SIZE_MISMATCH;  // _size >= end-beg+1
+        if (src->data == dst->data) return LBX_ERR_INVALID_ARG;                          // overlapping storage
+        const size_t entries = src->rows * src->cols;
+        if (entries >= ((size_t)1 << 32)) return LBX_ERR_UNSUPPORTED;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        DeviceGuard guard(ctx->device);
+        hipStream_t s = (hipStream_t)stream;
+        StreamOrder stream_order(ctx, s);
+        Staged ss, sd;
+        lbx_status st = stage_in(ctx, "stage_a", src, 4, true, &ss, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_c", dst, 4, true, &sd, s);  // the rest of dst's storage is kept
+        if (st == LBX_OK) st = from_hip(launch_copy(devview(src, ss.dev), devview(dst, sd.dev), entries, cnt, beg, start, s));
+        if (st == LBX_OK && sd.owned) {
+            st = device_to_host(ctx, dst->data, sd.dev, storage_words(dst) * 4, s);
+        } else if (st == LBX_OK && ss.owned) {
+            st = from_hip(hipStreamSynchronize(s));
+        }
+        if (ss.owned || sd.owned) (void)hipStreamSynchronize(s);
+        release(ss);
+        release(sd);
+        return st;
+    });
 }

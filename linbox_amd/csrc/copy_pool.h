@@ -16,9 +16,15 @@ class CopyPool {
 public:
     static constexpr size_t kSplitMin = 1u << 20;  // smaller copies stay on the calling thread
 
-    explicit CopyPool(int workers)
+    // Starts up to `workers` threads; if the system refuses one, the pool keeps
+    // those already running (copies are split over however many there are).
+    explicit CopyPool(int workers) noexcept
     {
-        for (int i = 0; i < workers; ++i) threads_.emplace_back([this, i] { work(i + 1); });
+        try {
+            threads_.reserve(workers > 0 ? (size_t)workers : 0);
+            for (int i = 0; i < workers; ++i) threads_.emplace_back([this, i] { work(i + 1); });
+        } catch (...) {
+        }
     }
     ~CopyPool()
     {

@@ -576,56 +576,62 @@ extern "C" {
 
 lbx_status lbx_init(lbx_ctx **out, int device)
 {
-    if (!out) return LBX_ERR_INVALID_ARG;
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return LBX_ERR_NO_DEVICE;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return LBX_ERR_NO_DEVICE;
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return LBX_ERR_NO_DEVICE;
-    {
-        DeviceGuard g(device);
-        const lbx_status st = noalloc_attach(device);
-        if (st != LBX_OK) return st;
-    }
-    lbx_ctx *ctx = new lbx_ctx();
-    ctx->device = device;
-    if (const char *e = std::getenv("LBX_CHECK_MUL")) ctx->check = std::atoi(e) != 0;
-    *out = ctx;
-    return LBX_OK;
+    return guarded([&]() -> lbx_status {
+        if (!out) return LBX_ERR_INVALID_ARG;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return LBX_ERR_NO_DEVICE;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess) return LBX_ERR_NO_DEVICE;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return LBX_ERR_NO_DEVICE;
+        {
+            DeviceGuard g(device);
+            const lbx_status st = noalloc_attach(device);
+            if (st != LBX_OK) return st;
+        }
+        lbx_ctx *ctx = new lbx_ctx();
+        ctx->device = device;
+        if (const char *e = std::getenv("LBX_CHECK_MUL")) ctx->check = std::atoi(e) != 0;
+        *out = ctx;
+        return LBX_OK;
+    });
 }
 
 lbx_status lbx_trim(lbx_ctx *ctx)
 {
-    if (!ctx) return LBX_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    DeviceGuard g(ctx->device);
-    (void)hipDeviceSynchronize();
-    for (auto &kv : ctx->bufs)
-        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
-    ctx->bufs.clear();
-    stager_release(ctx);
-    return LBX_OK;
-}
-
-lbx_status lbx_destroy(lbx_ctx *ctx)
-{
-    if (!ctx) return LBX_ERR_INVALID_ARG;
-    {
+    return guarded([&]() -> lbx_status {
+        if (!ctx) return LBX_ERR_INVALID_ARG;
         std::lock_guard<std::mutex> lock(ctx->mu);
         DeviceGuard g(ctx->device);
         (void)hipDeviceSynchronize();
         for (auto &kv : ctx->bufs)
             if (kv.second.ptr) (void)hipFree(kv.second.ptr);
-        for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
-        for (auto ev : ctx->events) (void)hipEventDestroy(ev);
-        if (ctx->last_done) (void)hipEventDestroy(ctx->last_done);
-        if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
-        mp_release(ctx);
-        batch_release(ctx);
+        ctx->bufs.clear();
         stager_release(ctx);
-    }
-    delete ctx;
-    return LBX_OK;
+        return LBX_OK;
+    });
+}
+
+lbx_status lbx_destroy(lbx_ctx *ctx)
+{
+    return guarded([&]() -> lbx_status {
+        if (!ctx) return LBX_ERR_INVALID_ARG;
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            DeviceGuard g(ctx->device);
+            (void)hipDeviceSynchronize();
+            for (auto &kv : ctx->bufs)
+                if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+            for (auto &kv : ctx->twiddles) (void)hipFree(kv.second);
+            for (auto ev : ctx->events) (void)hipEventDestroy(ev);
+            if (ctx->last_done) (void)hipEventDestroy(ctx->last_done);
+            if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+            mp_release(ctx);
+            batch_release(ctx);
+            stager_release(ctx);
+        }
+        delete ctx;
+        return LBX_OK;
+    });
 }
 
 const char *lbx_strerror(lbx_status st)
@@ -652,54 +658,61 @@ const char *lbx_strerror(lbx_status st)
 
 lbx_status lbx_profile_enable(lbx_ctx *ctx, int enable)
 {
-    if (!ctx) return LBX_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->profiling = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
-    return LBX_OK;
+    return guarded([&]() -> lbx_status {
+        if (!ctx) return LBX_ERR_INVALID_ARG;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->profiling = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
+        return LBX_OK;
+    });
 }
 
 int lbx_profile_get(lbx_ctx *ctx, const char **labels, float *ms, int max)
 {
-    if (!ctx) return 0;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    DeviceGuard guard(ctx->device);
-    // average per label over every call recorded since the previous get
-    std::vector<std::string> order;
-    std::map<std::string, std::pair<double, int>> acc;
-    std::vector<hipEvent_t> used;
-    for (auto &rec : ctx->prof_pending) {
-        const std::string &label = std::get<0>(rec);
-        hipEvent_t a = std::get<1>(rec), b = std::get<2>(rec);
-        (void)hipEventSynchronize(b);
-        float t = 0;
-        (void)hipEventElapsedTime(&t, a, b);
-        if (!acc.count(label)) order.push_back(label);
-        acc[label].first += t;
-        acc[label].second += 1;
-        used.push_back(a);
-        used.push_back(b);
-    }
-    std::sort(used.begin(), used.end());
-    used.erase(std::unique(used.begin(), used.end()), used.end());
-    for (auto ev : used) (void)hipEventDestroy(ev);
-    ctx->prof_pending.clear();
-    if (!order.empty()) {
-        ctx->prof_labels.clear();
-        ctx->prof_ms.clear();
-        for (auto &l : order) {
-            ctx->prof_labels.push_back(l);
-            ctx->prof_ms.push_back((float)(acc[l].first / acc[l].second));
+    // no C++ exception may reach a C caller (only a failed allocation throws here)
+    try {
+        if (!ctx) return 0;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        DeviceGuard guard(ctx->device);
+        // average per label over every call recorded since the previous get
+        std::vector<std::string> order;
+        std::map<std::string, std::pair<double, int>> acc;
+        std::vector<hipEvent_t> used;
+        for (auto &rec : ctx->prof_pending) {
+            const std::string &label = std::get<0>(rec);
+            hipEvent_t a = std::get<1>(rec), b = std::get<2>(rec);
+            (void)hipEventSynchronize(b);
+            float t = 0;
+            (void)hipEventElapsedTime(&t, a, b);
+            if (!acc.count(label)) order.push_back(label);
+            acc[label].first += t;
+            acc[label].second += 1;
+            used.push_back(a);
+            used.push_back(b);
         }
+        std::sort(used.begin(), used.end());
+        used.erase(std::unique(used.begin(), used.end()), used.end());
+        for (auto ev : used) (void)hipEventDestroy(ev);
+        ctx->prof_pending.clear();
+        if (!order.empty()) {
+            ctx->prof_labels.clear();
+            ctx->prof_ms.clear();
+            for (auto &l : order) {
+                ctx->prof_labels.push_back(l);
+                ctx->prof_ms.push_back((float)(acc[l].first / acc[l].second));
+            }
+        }
+        int n = (int)ctx->prof_ms.size();
+        if (n > max) n = max;
+        ctx->prof_label_ptrs.clear();
+        for (auto &l : ctx->prof_labels) ctx->prof_label_ptrs.push_back(l.c_str());
+        for (int i = 0; i < n; ++i) {
+            if (labels) labels[i] = ctx->prof_label_ptrs[i];
+            if (ms) ms[i] = ctx->prof_ms[i];
+        }
+        return n;
+    } catch (...) {
+        return 0;
     }
-    int n = (int)ctx->prof_ms.size();
-    if (n > max) n = max;
-    ctx->prof_label_ptrs.clear();
-    for (auto &l : ctx->prof_labels) ctx->prof_label_ptrs.push_back(l.c_str());
-    for (int i = 0; i < n; ++i) {
-        if (labels) labels[i] = ctx->prof_label_ptrs[i];
-        if (ms) ms[i] = ctx->prof_ms[i];
-    }
-    return n;
 }
 
 static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
@@ -708,8 +721,10 @@ static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_pol
 lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
                                size_t max_rowdeg, int method, void *stream)
 {
-    const lbx_status st = mul_u32(ctx, p, c, a, b, max_rowdeg, method, stream);
-    return st == LBX_OK ? self_check(ctx, p, c, a, b, stream) : st;
+    return guarded([&]() -> lbx_status {
+        const lbx_status st = mul_u32(ctx, p, c, a, b, max_rowdeg, method, stream);
+        return st == LBX_OK ? self_check(ctx, p, c, a, b, stream) : st;
+    });
 }
 
 static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
@@ -780,34 +795,36 @@ static lbx_status mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_pol
 lbx_status lbx_polymat_mul_fft_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, size_t m, size_t k, size_t n,
                                    const uint32_t *a, const uint32_t *b, uint32_t *c, int method, void *stream)
 {
-    if (!ctx || !a || !b || !c || !m || !k || !n) return LBX_ERR_INVALID_ARG;
-    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
-    const size_t pts = (size_t)1 << lpts;
-    if (p < 2 || p >= (1u << 31)) return LBX_ERR_PRIME_TOO_LARGE;
-    if ((p - 1) % pts != 0) return LBX_ERR_BAD_FFT_PRIME;
-    if (lpts == 0) return LBX_ERR_K_NOT_POSITIVE;
-    lbx_polmat va{(void *)a, LBX_POLFIRST, m, k, pts, pts, 0};
-    lbx_polmat vb{(void *)b, LBX_POLFIRST, k, n, pts, pts, 0};
-    lbx_polmat vc{(void *)c, LBX_POLFIRST, m, n, pts, pts, 0};
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    DeviceGuard guard(ctx->device);
-    hipStream_t s = (hipStream_t)stream;
-    StreamOrder stream_order(ctx, s);
-    Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", &va, 4, true, &sa_, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", &vb, 4, true, &sb_, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", &vc, 4, false, &sc_, s);
-    if (st == LBX_OK)
-        st = fft_prime_product(ctx, p, lpts, devview(&va, sa_.dev), devview(&vb, sb_.dev), devview(&vc, sc_.dev), m,
-                               k, n, pts, method, s);
-    if (st == LBX_OK && sc_.owned) {
-        st = device_to_host(ctx, c, sc_.dev, m * n * pts * 4, s);
-    }
-    if (sa_.owned || sb_.owned || sc_.owned) (void)hipStreamSynchronize(s);
-    release(sa_);
-    release(sb_);
-    release(sc_);
-    return st;
+    return guarded([&]() -> lbx_status {
+        if (!ctx || !a || !b || !c || !m || !k || !n) return LBX_ERR_INVALID_ARG;
+        if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+        const size_t pts = (size_t)1 << lpts;
+        if (p < 2 || p >= (1u << 31)) return LBX_ERR_PRIME_TOO_LARGE;
+        if ((p - 1) % pts != 0) return LBX_ERR_BAD_FFT_PRIME;
+        if (lpts == 0) return LBX_ERR_K_NOT_POSITIVE;
+        lbx_polmat va{(void *)a, LBX_POLFIRST, m, k, pts, pts, 0};
+        lbx_polmat vb{(void *)b, LBX_POLFIRST, k, n, pts, pts, 0};
+        lbx_polmat vc{(void *)c, LBX_POLFIRST, m, n, pts, pts, 0};
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        DeviceGuard guard(ctx->device);
+        hipStream_t s = (hipStream_t)stream;
+        StreamOrder stream_order(ctx, s);
+        Staged sa_, sb_, sc_;
+        lbx_status st = stage_in(ctx, "stage_a", &va, 4, true, &sa_, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_b", &vb, 4, true, &sb_, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_c", &vc, 4, false, &sc_, s);
+        if (st == LBX_OK)
+            st = fft_prime_product(ctx, p, lpts, devview(&va, sa_.dev), devview(&vb, sb_.dev), devview(&vc, sc_.dev), m,
+                                   k, n, pts, method, s);
+        if (st == LBX_OK && sc_.owned) {
+            st = device_to_host(ctx, c, sc_.dev, m * n * pts * 4, s);
+        }
+        if (sa_.owned || sb_.owned || sc_.owned) (void)hipStreamSynchronize(s);
+        release(sa_);
+        release(sb_);
+        release(sc_);
+        return st;
+    });
 }
 
 static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data, size_t count,
@@ -850,13 +867,17 @@ static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t ro
 lbx_status lbx_ntt_forward_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data, size_t count,
                                uint32_t *root_used, void *stream)
 {
-    return ntt_entry(ctx, p, lpts, root, data, count, root_used, stream, false);
+    return guarded([&]() -> lbx_status {
+        return ntt_entry(ctx, p, lpts, root, data, count, root_used, stream, false);
+    });
 }
 
 lbx_status lbx_ntt_inverse_u32(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t root, uint32_t *data, size_t count,
                                uint32_t *root_used, void *stream)
 {
-    return ntt_entry(ctx, p, lpts, root, data, count, root_used, stream, true);
+    return guarded([&]() -> lbx_status {
+        return ntt_entry(ctx, p, lpts, root, data, count, root_used, stream, true);
+    });
 }
 
 }  // extern "C"

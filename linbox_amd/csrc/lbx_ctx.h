@@ -57,6 +57,21 @@ struct lbx_ctx {
     std::vector<const char *> prof_label_ptrs;
 };
 
+namespace lbx {
+// Runs the body of a C ABI entry point: no C++ exception may reach a C caller.
+// The library throws nothing itself; the standard library throws only when an
+// allocation fails (bad_alloc, length_error), which is reported as such.
+template <class F>
+lbx_status guarded(F &&body) noexcept
+{
+    try {
+        return body();
+    } catch (...) {
+        return LBX_ERR_OUT_OF_MEMORY;
+    }
+}
+}  // namespace lbx
+
 #define LBX_TRY_HIP(expr)                                 \
     do {                                                  \
         hipError_t e_ = (expr);                           \

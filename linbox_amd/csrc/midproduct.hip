@@ -145,44 +145,48 @@ extern "C" lbx_status lbx_polymat_midproduct_u32(lbx_ctx *ctx, uint32_t p, lbx_p
                                                  const lbx_polmat *b, int small_left, size_t n0, size_t n1,
                                                  int method, void *stream)
 {
-    LBX_TRY(check_args(ctx, p, c, a, b, method));
-    MidShape sh;
-    LBX_TRY(mid_shape(c, n0, n1, &sh));
-    const size_t small = small_left ? a->size : b->size, big = small_left ? b->size : a->size;
-    // wordsize-fast.inl:214-218; sizes beyond pts would overrun the padded
-    // copies there, and a small operand longer than hdeg is not reversed whole
-    if (sh.deg < sh.hdeg || c->size < sh.deg - sh.hdeg || big >= sh.hdeg + sh.deg) return LBX_ERR_SIZE_MISMATCH;
-    if (small > sh.hdeg || big > sh.pts || c->size > sh.pts || sh.hdeg > sh.pts) return LBX_ERR_SIZE_MISMATCH;
-    if (sh.L == 0) return LBX_ERR_K_NOT_POSITIVE;  // FFT(F, 0): "k must be positive" (fft.h:136-139)
-    if (sh.L > (unsigned)kMaxLog) return LBX_ERR_UNSUPPORTED;
-    const MidSpec mid{small_left != 0, sh.hdeg};
-    const bool single_prime = p < (1u << 31) && (p - 1) % sh.pts == 0;
-    // c is copied in too: a view's storage outside the window must come back unchanged
-    return with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
-                                                      hipStream_t s) -> lbx_status {
-        if (single_prime)
-            return fft_prime_product(ctx, p, sh.L, da, db, dc, a->rows, a->cols, b->cols, c->size, method, s, &mid);
-        return mp_product(ctx, &p, 1, da, db, dc, a->rows, a->cols, b->cols, method, s, &mid, sh.L);
+    return guarded([&]() -> lbx_status {
+        LBX_TRY(check_args(ctx, p, c, a, b, method));
+        MidShape sh;
+        LBX_TRY(mid_shape(c, n0, n1, &sh));
+        const size_t small = small_left ? a->size : b->size, big = small_left ? b->size : a->size;
+        // wordsize-fast.inl:214-218; sizes beyond pts would overrun the padded
+        // copies there, and a small operand longer than hdeg is not reversed whole
+        if (sh.deg < sh.hdeg || c->size < sh.deg - sh.hdeg || big >= sh.hdeg + sh.deg) return LBX_ERR_SIZE_MISMATCH;
+        if (small > sh.hdeg || big > sh.pts || c->size > sh.pts || sh.hdeg > sh.pts) return LBX_ERR_SIZE_MISMATCH;
+        if (sh.L == 0) return LBX_ERR_K_NOT_POSITIVE;  // FFT(F, 0): "k must be positive" (fft.h:136-139)
+        if (sh.L > (unsigned)kMaxLog) return LBX_ERR_UNSUPPORTED;
+        const MidSpec mid{small_left != 0, sh.hdeg};
+        const bool single_prime = p < (1u << 31) && (p - 1) % sh.pts == 0;
+        // c is copied in too: a view's storage outside the window must come back unchanged
+        return with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
+                                                          hipStream_t s) -> lbx_status {
+            if (single_prime)
+                return fft_prime_product(ctx, p, sh.L, da, db, dc, a->rows, a->cols, b->cols, c->size, method, s, &mid);
+            return mp_product(ctx, &p, 1, da, db, dc, a->rows, a->cols, b->cols, method, s, &mid, sh.L);
+        });
     });
 }
 
 extern "C" lbx_status lbx_polymat_midproduct_naive_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lbx_polmat *a,
                                                        const lbx_polmat *b, size_t n0, size_t n1, void *stream)
 {
-    LBX_TRY(check_args(ctx, p, c, a, b, 0));
-    MidShape sh;
-    LBX_TRY(mid_shape(c, n0, n1, &sh));
-    const size_t top = std::min(a->size + b->size - 1, sh.deg);
-    if (top < sh.hdeg) return LBX_OK;  // empty range: nothing written (naive.h:92)
-    const size_t count = top - sh.hdeg + 1;
-    if (count > c->size) return LBX_ERR_SIZE_MISMATCH;
-    if (a->cols > 0x7fffffffu) return LBX_ERR_UNSUPPORTED;
-    const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p), r64 = (uint32_t)((uint64_t)r32 * r32 % p);
-    return with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
-                                                      hipStream_t s) -> lbx_status {
-        const size_t total = a->rows * b->cols * count;
-        hipLaunchKernelGGL(midproduct_naive_kernel, dim3(grid_for(total)), dim3(256), 0, s, da, db, dc,
-                           (int)a->cols, sh.hdeg, count, p, r32, r64);
-        return from_hip(hipGetLastError());
+    return guarded([&]() -> lbx_status {
+        LBX_TRY(check_args(ctx, p, c, a, b, 0));
+        MidShape sh;
+        LBX_TRY(mid_shape(c, n0, n1, &sh));
+        const size_t top = std::min(a->size + b->size - 1, sh.deg);
+        if (top < sh.hdeg) return LBX_OK;  // empty range: nothing written (naive.h:92)
+        const size_t count = top - sh.hdeg + 1;
+        if (count > c->size) return LBX_ERR_SIZE_MISMATCH;
+        if (a->cols > 0x7fffffffu) return LBX_ERR_UNSUPPORTED;
+        const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p), r64 = (uint32_t)((uint64_t)r32 * r32 % p);
+        return with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
+                                                          hipStream_t s) -> lbx_status {
+            const size_t total = a->rows * b->cols * count;
+            hipLaunchKernelGGL(midproduct_naive_kernel, dim3(grid_for(total)), dim3(256), 0, s, da, db, dc,
+                               (int)a->cols, sh.hdeg, count, p, r32, r64);
+            return from_hip(hipGetLastError());
+        });
     });
 }

@@ -977,92 +977,96 @@ extern "C" lbx_status lbx_polymat_mul_mp(lbx_ctx *ctx, const uint32_t *P, int nl
                                          const lbx_polmat *a, const lbx_polmat *b, size_t max_rowdeg, int method,
                                          void *stream)
 {
-    if (!ctx || !P || !a || !b || !c) return LBX_ERR_INVALID_ARG;
-    if (nlimbs < 1 || nlimbs > kMaxLimbs) return LBX_ERR_PRIME_TOO_LARGE;
-    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
-    LBX_TRY(check_view(a));
-    LBX_TRY(check_view(b));
-    LBX_TRY(check_view(c));
-    bool nonzero = false, big = false;
-    for (int l = 0; l < nlimbs; ++l) {
-        nonzero |= P[l] != 0;
-        if (l > 0) big |= P[l] != 0;
-    }
-    if (!nonzero || (!big && P[0] < 2)) return LBX_ERR_INVALID_ARG;
-    if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
-    if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
-    const size_t m = a->rows, k = a->cols, n = b->cols;
-    const size_t sc = max_rowdeg ? max_rowdeg + 1 : a->size + b->size - 1;
-    if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
-    size_t pts = 1;
-    while (pts < sc) pts <<= 1;
-    if (a->size > pts || b->size > pts) return LBX_ERR_SIZE_MISMATCH;  // the padded copies hold pts coefficients
-    if (k > 0xffffffffu || a->size > 0xffffffffu || sc > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
+    return guarded([&]() -> lbx_status {
+        if (!ctx || !P || !a || !b || !c) return LBX_ERR_INVALID_ARG;
+        if (nlimbs < 1 || nlimbs > kMaxLimbs) return LBX_ERR_PRIME_TOO_LARGE;
+        if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+        LBX_TRY(check_view(a));
+        LBX_TRY(check_view(b));
+        LBX_TRY(check_view(c));
+        bool nonzero = false, big = false;
+        for (int l = 0; l < nlimbs; ++l) {
+            nonzero |= P[l] != 0;
+            if (l > 0) big |= P[l] != 0;
+        }
+        if (!nonzero || (!big && P[0] < 2)) return LBX_ERR_INVALID_ARG;
+        if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
+        if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
+        const size_t m = a->rows, k = a->cols, n = b->cols;
+        const size_t sc = max_rowdeg ? max_rowdeg + 1 : a->size + b->size - 1;
+        if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
+        size_t pts = 1;
+        while (pts < sc) pts <<= 1;
+        if (a->size > pts || b->size > pts) return LBX_ERR_SIZE_MISMATCH;  // the padded copies hold pts coefficients
+        if (k > 0xffffffffu || a->size > 0xffffffffu || sc > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
 
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    DeviceGuard guard(ctx->device);
-    hipStream_t s = (hipStream_t)stream;
-    StreamOrder stream_order(ctx, s);
-    const size_t wb = 4 * (size_t)nlimbs;
-    Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", a, wb, true, &sa_, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wb, true, &sb_, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wb, !(c->shift == 0 && sc >= c->storage_size), &sc_, s);
-    const bool host_io = sa_.owned || sb_.owned || sc_.owned;
-    if (st == LBX_OK)
-        st = mp_product(ctx, P, nlimbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k, n,
-                        method, s, nullptr, 0, sc);
-    if (st == LBX_OK && sc_.owned) {
-        st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * wb, s);
-    }
-    if (host_io) (void)hipStreamSynchronize(s);
-    release(sa_);
-    release(sb_);
-    release(sc_);
-    if (st == LBX_OK) c->size = sc;
-    return st;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        DeviceGuard guard(ctx->device);
+        hipStream_t s = (hipStream_t)stream;
+        StreamOrder stream_order(ctx, s);
+        const size_t wb = 4 * (size_t)nlimbs;
+        Staged sa_, sb_, sc_;
+        lbx_status st = stage_in(ctx, "stage_a", a, wb, true, &sa_, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wb, true, &sb_, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wb, !(c->shift == 0 && sc >= c->storage_size), &sc_, s);
+        const bool host_io = sa_.owned || sb_.owned || sc_.owned;
+        if (st == LBX_OK)
+            st = mp_product(ctx, P, nlimbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k, n,
+                            method, s, nullptr, 0, sc);
+        if (st == LBX_OK && sc_.owned) {
+            st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * wb, s);
+        }
+        if (host_io) (void)hipStreamSynchronize(s);
+        release(sa_);
+        release(sb_);
+        release(sc_);
+        if (st == LBX_OK) c->size = sc;
+        return st;
+    });
 }
 
 extern "C" lbx_status lbx_polymat_mul_zz(lbx_ctx *ctx, lbx_polmat *c, int c_limbs, const lbx_polmat *a,
                                          const lbx_polmat *b, int ab_limbs, size_t max_rowdeg, int method,
                                          void *stream)
 {
-    if (!ctx || !a || !b || !c) return LBX_ERR_INVALID_ARG;
-    if (ab_limbs < 1 || ab_limbs > kZzMaxIn || c_limbs < 1 || c_limbs > kZzMaxOut) return LBX_ERR_UNSUPPORTED;
-    if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
-    LBX_TRY(check_view(a));
-    LBX_TRY(check_view(b));
-    LBX_TRY(check_view(c));
-    if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
-    if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
-    const size_t m = a->rows, k = a->cols, n = b->cols;
-    const size_t sc = max_rowdeg ? max_rowdeg + 1 : a->size + b->size - 1;
-    if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
-    size_t pts = 1;
-    while (pts < sc) pts <<= 1;
-    if (a->size > pts || b->size > pts) return LBX_ERR_SIZE_MISMATCH;
-    if (k > 0xffffffffu || a->size > 0xffffffffu || sc > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
+    return guarded([&]() -> lbx_status {
+        if (!ctx || !a || !b || !c) return LBX_ERR_INVALID_ARG;
+        if (ab_limbs < 1 || ab_limbs > kZzMaxIn || c_limbs < 1 || c_limbs > kZzMaxOut) return LBX_ERR_UNSUPPORTED;
+        if (method < 0 || method > 2) return LBX_ERR_INVALID_ARG;
+        LBX_TRY(check_view(a));
+        LBX_TRY(check_view(b));
+        LBX_TRY(check_view(c));
+        if (a->cols != b->rows || c->rows != a->rows || c->cols != b->cols) return LBX_ERR_SIZE_MISMATCH;
+        if (!a->size || !b->size) return LBX_ERR_INVALID_ARG;
+        const size_t m = a->rows, k = a->cols, n = b->cols;
+        const size_t sc = max_rowdeg ? max_rowdeg + 1 : a->size + b->size - 1;
+        if (c->size < sc) return LBX_ERR_SIZE_MISMATCH;
+        size_t pts = 1;
+        while (pts < sc) pts <<= 1;
+        if (a->size > pts || b->size > pts) return LBX_ERR_SIZE_MISMATCH;
+        if (k > 0xffffffffu || a->size > 0xffffffffu || sc > 0xffffffffu) return LBX_ERR_UNSUPPORTED;
 
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    DeviceGuard guard(ctx->device);
-    hipStream_t s = (hipStream_t)stream;
-    StreamOrder stream_order(ctx, s);
-    const size_t wi = 4 * (size_t)ab_limbs, wo = 4 * (size_t)c_limbs;
-    Staged sa_, sb_, sc_;
-    lbx_status st = stage_in(ctx, "stage_a", a, wi, true, &sa_, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wi, true, &sb_, s);
-    if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wo, !(c->shift == 0 && sc >= c->storage_size), &sc_, s);
-    const bool host_io = sa_.owned || sb_.owned || sc_.owned;
-    if (st == LBX_OK)
-        st = zz_product(ctx, ab_limbs, c_limbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k,
-                        n, sc, method, s);
-    if (st == LBX_OK && sc_.owned) {
-        st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * wo, s);
-    }
-    if (host_io) (void)hipStreamSynchronize(s);
-    release(sa_);
-    release(sb_);
-    release(sc_);
-    if (st == LBX_OK) c->size = sc;
-    return st;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        DeviceGuard guard(ctx->device);
+        hipStream_t s = (hipStream_t)stream;
+        StreamOrder stream_order(ctx, s);
+        const size_t wi = 4 * (size_t)ab_limbs, wo = 4 * (size_t)c_limbs;
+        Staged sa_, sb_, sc_;
+        lbx_status st = stage_in(ctx, "stage_a", a, wi, true, &sa_, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_b", b, wi, true, &sb_, s);
+        if (st == LBX_OK) st = stage_in(ctx, "stage_c", c, wo, !(c->shift == 0 && sc >= c->storage_size), &sc_, s);
+        const bool host_io = sa_.owned || sb_.owned || sc_.owned;
+        if (st == LBX_OK)
+            st = zz_product(ctx, ab_limbs, c_limbs, devview(a, sa_.dev), devview(b, sb_.dev), devview(c, sc_.dev), m, k,
+                            n, sc, method, s);
+        if (st == LBX_OK && sc_.owned) {
+            st = device_to_host(ctx, c->data, sc_.dev, storage_words(c) * wo, s);
+        }
+        if (host_io) (void)hipStreamSynchronize(s);
+        release(sa_);
+        release(sb_);
+        release(sc_);
+        if (st == LBX_OK) c->size = sc;
+        return st;
+    });
 }

@@ -1429,11 +1429,15 @@ using namespace lbx;
 extern "C" lbx_status lbx_order_basis_pm_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_polmat *serie,
                                              size_t order, size_t *shift, size_t *degree, int method, void *stream)
 {
-    return order_basis(ctx, p, sigma, serie, order, shift, degree, method, stream, true);
+    return guarded([&]() -> lbx_status {
+        return order_basis(ctx, p, sigma, serie, order, shift, degree, method, stream, true);
+    });
 }
 
 extern "C" lbx_status lbx_order_basis_m_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *sigma, const lbx_polmat *serie,
                                             size_t order, size_t *shift, size_t *degree, void *stream)
 {
-    return order_basis(ctx, p, sigma, serie, order, shift, degree, LBX_METHOD_AUTO, stream, false);
+    return guarded([&]() -> lbx_status {
+        return order_basis(ctx, p, sigma, serie, order, shift, degree, LBX_METHOD_AUTO, stream, false);
+    });
 }

@@ -9,6 +9,7 @@
 // while the DMA engine moves the previous chunk, so the two overlap.  The DMAs
 // are queued on the call's stream, behind the work the call already waits for.
 #include <cstdlib>
+#include <new>
 
 #include "copy_pool.h"
 #include "lbx_ctx.h"
@@ -44,7 +45,8 @@ int copy_threads()
 lbx_status get_stager(lbx_ctx *ctx, HostStager **out)
 {
     if (!ctx->stager) {
-        HostStager *hs = new HostStager();
+        HostStager *hs = new (std::nothrow) HostStager();
+        if (!hs) return LBX_ERR_OUT_OF_MEMORY;
         hipError_t e = hipHostMalloc((void **)&hs->pinned, kChunk * kSlots, hipHostMallocDefault);
         for (int i = 0; i < kSlots && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&hs->ev[i], hipEventDisableTiming);
         if (e != hipSuccess) {
@@ -54,7 +56,15 @@ lbx_status get_stager(lbx_ctx *ctx, HostStager **out)
             delete hs;
             return from_hip(e);
         }
-        hs->pool = new CopyPool(copy_threads() - 1);
+        // no exception leaves the C ABI: a pool the system grants fewer threads
+        // splits its copies over those
+        hs->pool = new (std::nothrow) CopyPool(copy_threads() - 1);
+        if (!hs->pool) {
+            for (auto ev : hs->ev) (void)hipEventDestroy(ev);
+            (void)hipHostFree(hs->pinned);
+            delete hs;
+            return LBX_ERR_OUT_OF_MEMORY;
+        }
         ctx->stager = hs;
     }
     *out = ctx->stager;
