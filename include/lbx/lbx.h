@@ -49,7 +49,7 @@ typedef enum lbx_status {
     LBX_ERR_INVALID_ARG = 7,        /* null pointer, zero dimension, unknown layout or method */
     LBX_ERR_PRIME_TOO_LARGE = 8,    /* modulus outside the supported range (three-primes.inl:57-64 analogue) */
     LBX_ERR_HIP = 9,                /* a HIP runtime call failed */
-    LBX_ERR_OUT_OF_MEMORY = 10,     /* device allocation failed */
+    LBX_ERR_OUT_OF_MEMORY = 10,     /* a device, pinned or host allocation failed */
     LBX_ERR_UNSUPPORTED = 11,       /* shape beyond what the library implements */
     LBX_ERR_NO_DEVICE = 12,         /* no gfx950 device at the requested index */
     LBX_ERR_WORKSPACE_TOO_SMALL = 13, /* the call needs more device memory than its fixed workspace holds */
