@@ -78,17 +78,7 @@ bool batchable(const lbx_job &j, unsigned *Lout)
     return true;
 }
 
-int cus()
-{
-    static int n = 0;
-    if (!n) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
-                                                     hipSuccess || n <= 0)
-            n = 256;
-    }
-    return n;
-}
+int cus() { return device_cus(); }
 
 // Class of an INT8-capable job (every dimension >= 16, one exact int32 chunk
 // of inner products): 0 for a smallest dimension below 32, 1 above; -1 when

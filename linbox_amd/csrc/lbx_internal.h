@@ -84,6 +84,9 @@ struct EntryMap {
 // inverse tail = DIT stages [BL, L) per entry, written to entries
 //                out_first + row of the view.
 constexpr unsigned kSplitBL = 6;
+// Compute units of the device current at the first call (256 on MI355X; 256 if
+// the query fails), determined once per process.
+int device_cus();
 hipError_t launch_ntt_forward_head(const DevView &in, size_t entries, const EntryMap &map, unsigned L, unsigned BL,
                                    const uint2 *tw, uint32_t p, uint32_t *out, hipStream_t s);
 // With limbs = false the tail writes the centred values (int32) instead of

@@ -69,17 +69,6 @@ hipError_t fwd_launch(const DevView &in, size_t entries, const EntryMap &map, co
     return hipGetLastError();
 }
 
-int device_cus()
-{
-    static int n = 0;
-    if (!n) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
-                                                     hipSuccess || n <= 0)
-            n = 256;
-    }
-    return n;
-}
 
 // Forward head: the persistent form (grid of two groups per CU) for polfirst
 // inputs of 2^12..2^14 points, the one-group-per-polynomial form otherwise (at
@@ -858,6 +847,19 @@ hipError_t launch_batch_ntt(unsigned L, const BatchJob *jobs, const NttItem *ite
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+int device_cus()
+{
+    // initialised once, thread-safely, on first use
+    static const int n = [] {
+        int dev = 0, c = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                     hipSuccess || c <= 0)
+            c = 256;
+        return c;
+    }();
+    return n;
 }
 
 }  // namespace lbx

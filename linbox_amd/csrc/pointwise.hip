@@ -702,13 +702,7 @@ hipError_t launch_pointwise_mfma(const uint8_t *Al, const uint8_t *Bl, uint32_t 
     const size_t mpad = rpad_of(m), npad = rpad_of(n), kchunks = kpad_of(k) / 64;
     if (npts > 0x7fffffff) return hipErrorInvalidValue;
     const size_t a_bytes = limb_block_bytes(m, k);
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
+    const int cus = device_cus();
     if ((mpad == 32 || mpad == 64 || mpad == 128) && kchunks <= 2 && npad * kchunks * 256 <= 65536) {
         const size_t lds = npad * kchunks * 256;
         const size_t grid = npts < (size_t)cus * 2 ? npts : (size_t)cus * 2;
@@ -738,13 +732,7 @@ template <int RI, int RJ>
 static hipError_t pm_launch(const uint32_t *Aw, const uint32_t *Bw, uint32_t *Cmat, size_t m, size_t n, size_t mpad,
                             size_t npad, size_t kpad, size_t npts, uint32_t p, const PlainSConst &pc, hipStream_t s)
 {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
+    const int cus = device_cus();
     const size_t tiles = ((mpad + 16 * RI - 1) / (16 * RI)) * ((npad + 16 * RJ - 1) / (16 * RJ));
     const size_t work = npts * tiles;
     const size_t grid = work < (size_t)cus * 2 ? work : (size_t)cus * 2;

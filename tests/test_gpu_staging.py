@@ -141,3 +141,44 @@ def test_layout_copy_host(ctx):
     say("staging layout copy", A.data.nbytes)
     B.copy(A, ctx=ctx)
     assert np.array_equal(B.to_polfirst(), A.to_polfirst())
+
+
+def test_two_threads_two_contexts():
+    """Two host threads, each with its own context, run host-operand products at
+    the same time (ctypes releases the interpreter lock during the calls): each
+    context's pinned buffers and copy threads serve only its own calls."""
+    import threading
+    m, k, n, sa, sb = 8, 8, 8, 40000, 30000  # 10 / 7.7 MiB operands, 17.9 MiB products
+    rng = np.random.default_rng(21)
+    cases = [(rand(rng, m * k * sa), rand(rng, k * n * sb)) for _ in range(2)]
+    refs = []
+    c0 = lbx.Context(0)
+    try:
+        for a, b in cases:
+            refs.append(device_product(c0, m, k, n, a, sa, b, sb, lbx.METHOD_AUTO))
+    finally:
+        c0.close()
+    say("staging two threads")
+    results, errors = [None, None], []
+
+    def run(i):
+        try:
+            c = lbx.Context(0)
+            try:
+                a, b = cases[i]
+                outs = [host_product(c, m, k, n, a, sa, b, sb, lbx.METHOD_AUTO) for _ in range(3)]
+                results[i] = outs
+            finally:
+                c.close()
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=200)
+    assert not errors, errors
+    for i in range(2):
+        for out in results[i]:
+            assert np.array_equal(out, refs[i])
