@@ -6,7 +6,9 @@ runs.  Products: any prime below 2^32 (FFT primes, word primes that take the CRT
 route, primes of too small 2-valuation), both forms, all layouts.  Middle
 products: both operand orders and explicit windows.  Order bases: M_Basis and
 PM_Basis with random shifts.  Layout conversions: random layouts, windows,
-offsets and views on either side.
+offsets and views on either side.  Integer products (1..32-word operands,
+random signed bit sizes, forms, spare output words, cyclic max_rowdeg) and
+multiword moduli of 2..64 limbs against Python integers.
 """
 import sys
 
@@ -153,3 +155,106 @@ def test_random_copies(ctx):
         want = (ddata if dview else ddata[:, :ds]).copy()
         want[:, d0 + start:d0 + start + cnt] = sdata[:, s0 + beg:s0 + end + 1]
         assert np.array_equal(D.to_polfirst(), want), case
+
+
+def naive_int(A, B, m, k, n, sa, sb):
+    """Product of integer polynomial matrices (polfirst lists of Python ints)."""
+    sc = sa + sb - 1
+    C = [0] * (m * n * sc)
+    for i in range(m):
+        for j in range(n):
+            for q in range(k):
+                a = A[(i * k + q) * sa:(i * k + q + 1) * sa]
+                b = B[(q * n + j) * sb:(q * n + j + 1) * sb]
+                base = (i * n + j) * sc
+                for u, x in enumerate(a):
+                    if x:
+                        for v, y in enumerate(b):
+                            C[base + u + v] += x * y
+    return C
+
+
+def test_random_integer_products(ctx):
+    """lbx_polymat_mul_zz on random shapes, word counts (1..32 in), signed bit sizes,
+    forms, spare output words and cyclic max_rowdeg wraps, against Python integers."""
+    import random
+    from linbox_amd.polymat import mul_zz, to_limbs, from_limbs
+    rnd = random.Random(20261203)
+    methods = [lbx.METHOD_AUTO, lbx.METHOD_PLAIN_U64, lbx.METHOD_INT8_MFMA]
+    for case in range(60):
+        m, k, n = rnd.randint(1, 5), rnd.randint(1, 6), rnd.randint(1, 5)
+        sa, sb = rnd.randint(1, 40), rnd.randint(1, 40)
+        if sa + sb <= 2:
+            sb = 2
+        wi = rnd.choice([1, 2, 3, 5, 8, 16, 32])
+        bits = rnd.randint(1, 32 * wi - 1)
+        A = [rnd.randrange(-(1 << bits) + 1, 1 << bits) for _ in range(m * k * sa)]
+        B = [rnd.randrange(-(1 << bits) + 1, 1 << bits) for _ in range(k * n * sb)]
+        la = max(abs(x).bit_length() for x in A)
+        lb = max(abs(x).bit_length() for x in B)
+        need = ((2 * (1 << la) * (1 << lb) * k * min(sa, sb)).bit_length() + 31) // 32
+        wo = min(80, need + rnd.choice([0, 0, 1, 3]))
+        full = naive_int(A, B, m, k, n, sa, sb)
+        # a cyclic max_rowdeg product needs operands no longer than its transform
+        # (the reference pads each operand into 2^ceil(log2(max_rowdeg+1)) words)
+        mr = 0
+        lo = max(sa, sb) - 1
+        if rnd.random() < 0.25 and 1 <= lo <= sa + sb - 3:
+            mr = rnd.randint(lo, sa + sb - 3)
+        method = rnd.choice(methods)
+        say("random zz", case, m, k, n, sa, sb, wi, bits, wo, mr, method)
+        a = PolynomialMatrix(m, k, sa, data=to_limbs(A, wi), nlimbs=wi)
+        b = PolynomialMatrix(k, n, sb, data=to_limbs(B, wi), nlimbs=wi)
+        sc = mr + 1 if mr else sa + sb - 1
+        c = PolynomialMatrix(m, n, sc, nlimbs=wo)
+        mul_zz(c, a, b, ctx=ctx, method=method, max_rowdeg=mr)
+        got = from_limbs(c.data, wo)
+        if not mr:
+            assert got == full, case
+        else:
+            pts = 1
+            while pts < sc:
+                pts <<= 1
+            want = []
+            for e in range(m * n):
+                row = [0] * max(pts, sc)
+                for t in range(sa + sb - 1):
+                    row[t % pts] += full[e * (sa + sb - 1) + t]
+                want += row[:sc]
+            assert got == want, case
+    # operands longer than the transform a max_rowdeg implies are refused
+    a = PolynomialMatrix(2, 2, 9, data=to_limbs([1] * 36, 1), nlimbs=1)
+    b = PolynomialMatrix(2, 2, 3, data=to_limbs([1] * 12, 1), nlimbs=1)
+    c = PolynomialMatrix(2, 2, 3, nlimbs=2)
+    with pytest.raises(lbx.LinboxError) as e:
+        mul_zz(c, a, b, ctx=ctx, max_rowdeg=2)
+    assert e.value.status == 5
+
+
+def test_random_wide_moduli(ctx):
+    """lbx_polymat_mul_mp for random moduli of 2..64 limbs (odd or even, any top word)
+    on small random shapes, against the product reduced with Python integers."""
+    import random
+    rnd = random.Random(20261204)
+    for case in range(40):
+        nl = rnd.randint(2, 64)
+        P = rnd.randrange(1 << (32 * nl - 32), 1 << (32 * nl))
+        if P < 3:
+            P = 3
+        m, k, n = rnd.randint(1, 4), rnd.randint(1, 5), rnd.randint(1, 4)
+        sa, sb = rnd.randint(1, 24), rnd.randint(2, 24)
+        A = [rnd.randrange(P) for _ in range(m * k * sa)]
+        B = [rnd.randrange(P) for _ in range(k * n * sb)]
+        method = rnd.choice([lbx.METHOD_AUTO, lbx.METHOD_PLAIN_U64, lbx.METHOD_INT8_MFMA])
+        say("random mp", case, nl, P.bit_length(), m, k, n, sa, sb, method)
+
+        def words(vals):
+            return np.array([(v >> (32 * i)) & 0xFFFFFFFF for v in vals for i in range(nl)], dtype=np.uint32)
+        a = PolynomialMatrix(m, k, sa, data=words(A), nlimbs=nl)
+        b = PolynomialMatrix(k, n, sb, data=words(B), nlimbs=nl)
+        c = PolynomialMatrix(m, n, sa + sb - 1, nlimbs=nl)
+        lbx.mul_mp(P, c, a, b, ctx=ctx, method=method)
+        w = c.data.reshape(-1, nl)
+        got = [sum(int(w[r, i]) << (32 * i) for i in range(nl)) for r in range(w.shape[0])]
+        want = [x % P for x in naive_int(A, B, m, k, n, sa, sb)]
+        assert got == want, case
