@@ -113,14 +113,26 @@ lbx_status lbx_polymat_mul_u32(lbx_ctx *ctx, uint32_t p, lbx_polmat *c, const lb
 lbx_status lbx_polymat_check_mul_u32(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a,
                                      const lbx_polmat *b, size_t deg, uint32_t x, int *equal, void *stream);
 
-/* CHECK_MATPOL_MUL (polynomial-matrix-domain.h:72-74) as a run-time switch:
- * with on != 0, every later lbx_polymat_mul_u32 on the context whose c
- * receives the whole product (max_rowdeg 0 or >= a.size+b.size-2) is
- * followed by lbx_polymat_check_mul_u32 at a pseudo-random point, and a
- * mismatch returns LBX_ERR_CHECK_FAILED (c keeps the computed product).  The
- * check waits for the product, so the call no longer returns before its
- * work is done.  LBX_CHECK_MUL=1 in the environment turns it on for every
- * new context. */
+/* check_midproduct (matpoly-mult-fft.h:168-262), by random projections
+ * instead of evaluation at 1: with random r (m words) and v (n words),
+ * r^T c v against sum over q of the scalar middle products of (r^T a)_q and
+ * (b v)_q, coefficients 0..min(c->size, deg-hdeg+1)-1 of c (hdeg = n0 ? n0 :
+ * c->size, deg = n1 ? n1 : 2*hdeg-1) against coefficients hdeg-1.. of a.b.
+ * A wrong c passes with probability at most 2/p.  *equal = 1 when they
+ * agree.  Any p < 2^32; waits for the check. */
+lbx_status lbx_polymat_check_midproduct_u32(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a,
+                                           const lbx_polmat *b, size_t n0, size_t n1, int *equal, void *stream);
+
+/* CHECK_MATPOL_MUL and CHECK_MATPOL_MIDP (polynomial-matrix-domain.h:72-74,
+ * 88-90) as a run-time switch: with on != 0, every later lbx_polymat_mul_u32
+ * on the context whose c receives the whole product (max_rowdeg 0 or >=
+ * a.size+b.size-2) is followed by lbx_polymat_check_mul_u32 at a
+ * pseudo-random point, and every lbx_polymat_midproduct_u32 and
+ * lbx_polymat_midproduct_naive_u32 by lbx_polymat_check_midproduct_u32 over
+ * the coefficients it computes; a mismatch returns LBX_ERR_CHECK_FAILED (c
+ * keeps the computed result).  The check waits for the work it checks, so the
+ * call no longer returns before it is done.  LBX_CHECK_MUL=1 in the
+ * environment turns it on for every new context. */
 lbx_status lbx_set_check(lbx_ctx *ctx, int on);
 
 /* No counterpart in the reference.  For callers that cannot allocate device

@@ -68,8 +68,9 @@ inline lbx_ctx *context()
             lbx_ctx *c = nullptr;
             const lbx_status st = lbx_init(&c, 0);
             if (st != LBX_OK) throw LinBox::LinboxError(lbx_strerror(st));
-#ifdef CHECK_MATPOL_MUL
-            // the reference's product check (polynomial-matrix-domain.h:72-74) on the device
+#if defined(CHECK_MATPOL_MUL) || defined(CHECK_MATPOL_MIDP)
+            // the reference's product and middle-product checks
+            // (polynomial-matrix-domain.h:72-74,88-90) on the device
             lbx_set_check(c, 1);
 #endif
             context_slot() = c;

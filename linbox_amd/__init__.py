@@ -30,6 +30,7 @@ EXPORTED = [
     "lbx_polymat_midproduct_u32", "lbx_polymat_midproduct_naive_u32", "lbx_order_basis_pm_u32",
     "lbx_order_basis_m_u32", "lbx_polymat_copy_u32", "lbx_profile_enable", "lbx_profile_get",
     "lbx_polymat_mul_u32_noalloc", "lbx_batch_tuning", "lbx_polymat_check_mul_u32", "lbx_set_check",
+    "lbx_polymat_check_midproduct_u32",
     "lbx_polymat_mul_zz",
 ]
 
@@ -95,6 +96,7 @@ def lib():
         L.lbx_profile_get.restype = i32
         L.lbx_batch_tuning.argtypes = [vp]
         L.lbx_polymat_check_mul_u32.argtypes = [vp, u32, PM, PM, PM, sz, u32, ctypes.POINTER(i32), vp]
+        L.lbx_polymat_check_midproduct_u32.argtypes = [vp, u32, PM, PM, PM, sz, sz, ctypes.POINTER(i32), vp]
         L.lbx_set_check.argtypes = [vp, i32]
         L.lbx_polymat_mul_zz.argtypes = [vp, PM, i32, PM, PM, i32, sz, i32, vp]
         L.lbx_batch_tuning.restype = ctypes.c_char_p
@@ -161,10 +163,10 @@ def default_context():
 from .polymat import (PolynomialMatrix, PolynomialMatrixMulDomain, PolynomialMatrixFFTMulDomain,  # noqa: E402
                       PolynomialMatrixFFTPrimeMulDomain, PolynomialMatrixNaiveMulDomain,
                       PolynomialMatrixThreePrimesFFTMulDomain, FFT, ntt_forward, ntt_inverse, mul_mp, mul_batch,
-                      OrderBasis, check_mul, mul_zz)
+                      OrderBasis, check_mul, check_midproduct, mul_zz)
 
 __all__ = ["Context", "LinboxError", "PolynomialMatrix", "PolynomialMatrixMulDomain", "PolynomialMatrixFFTMulDomain",
            "PolynomialMatrixFFTPrimeMulDomain", "PolynomialMatrixNaiveMulDomain",
-           "PolynomialMatrixThreePrimesFFTMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis", "check_mul", "mul_zz",
+           "PolynomialMatrixThreePrimesFFTMulDomain", "FFT", "ntt_forward", "ntt_inverse", "mul_mp", "mul_batch", "OrderBasis", "check_mul", "check_midproduct", "mul_zz",
            "POLFIRST", "MATFIRST", "MATROWFIRST", "METHOD_AUTO", "METHOD_PLAIN_U64", "METHOD_INT8_MFMA", "lib",
            "build", "default_context", "EXPORTED"]

@@ -176,6 +176,13 @@ lbx_status check_product(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lb
                          size_t deg, uint32_t x, int *equal, void *stream);
 lbx_status self_check(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
                       void *stream);
+// count coefficients of c against coefficients hdeg-1.. of a.b (zero beyond
+// its degree), by random projections of rows and columns (*equal); the
+// context's check after a middle product (LBX_ERR_CHECK_FAILED)
+lbx_status check_midproduct(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a, const lbx_polmat *b,
+                            size_t hdeg, size_t count, int *equal, void *stream);
+lbx_status self_check_midproduct(lbx_ctx *ctx, uint32_t p, const lbx_polmat *c, const lbx_polmat *a,
+                                 const lbx_polmat *b, size_t hdeg, size_t count, void *stream);
 
 // Where one call's device buffers come from.  The context's workspace keeps
 // named buffers between calls (ensure) and caches twiddle tables; an arena

@@ -159,12 +159,15 @@ extern "C" lbx_status lbx_polymat_midproduct_u32(lbx_ctx *ctx, uint32_t p, lbx_p
         const MidSpec mid{small_left != 0, sh.hdeg};
         const bool single_prime = p < (1u << 31) && (p - 1) % sh.pts == 0;
         // c is copied in too: a view's storage outside the window must come back unchanged
-        return with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
-                                                          hipStream_t s) -> lbx_status {
+        LBX_TRY(with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
+                                                           hipStream_t s) -> lbx_status {
             if (single_prime)
                 return fft_prime_product(ctx, p, sh.L, da, db, dc, a->rows, a->cols, b->cols, c->size, method, s, &mid);
             return mp_product(ctx, &p, 1, da, db, dc, a->rows, a->cols, b->cols, method, s, &mid, sh.L);
-        });
+        }));
+        // CHECK_MATPOL_MIDP (polynomial-matrix-domain.h:88-90): the middle
+        // coefficients, not those that wrap around past deg-hdeg
+        return self_check_midproduct(ctx, p, c, a, b, sh.hdeg, std::min(c->size, sh.deg - sh.hdeg + 1), stream);
     });
 }
 
@@ -181,12 +184,13 @@ extern "C" lbx_status lbx_polymat_midproduct_naive_u32(lbx_ctx *ctx, uint32_t p,
         if (count > c->size) return LBX_ERR_SIZE_MISMATCH;
         if (a->cols > 0x7fffffffu) return LBX_ERR_UNSUPPORTED;
         const uint32_t r32 = (uint32_t)(((uint64_t)1 << 32) % p), r64 = (uint32_t)((uint64_t)r32 * r32 % p);
-        return with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
-                                                          hipStream_t s) -> lbx_status {
+        LBX_TRY(with_views(ctx, c, a, b, true, stream, [&](const DevView &da, const DevView &db, const DevView &dc,
+                                                           hipStream_t s) -> lbx_status {
             const size_t total = a->rows * b->cols * count;
             hipLaunchKernelGGL(midproduct_naive_kernel, dim3(grid_for(total)), dim3(256), 0, s, da, db, dc,
                                (int)a->cols, sh.hdeg, count, p, r32, r64);
             return from_hip(hipGetLastError());
-        });
+        }));
+        return self_check_midproduct(ctx, p, c, a, b, sh.hdeg, count, stream);  // CHECK_MATPOL_MIDP
     });
 }

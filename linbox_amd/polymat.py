@@ -402,6 +402,21 @@ def mul_batch(jobs, ctx=None, method=METHOD_AUTO, stream=None):
     check(lib().lbx_polymat_mul_batch(ctx.handle, arr, len(jobs), method, stream))
 
 
+def check_midproduct(p, c, a, b, n0=0, n1=0, ctx=None, stream=None):
+    """check_midproduct(c, a, b, smallLeft, n0, n1) (matpoly-mult-fft.h:168-262) on the
+    device by random row and column projections: True when coefficients
+    0..min(c.size(), deg-hdeg+1)-1 of c equal coefficients hdeg-1.. of a.b mod p
+    (hdeg = n0 or c.size(), deg = n1 or 2 hdeg - 1)."""
+    ctx = ctx or default_context()
+    if a.coldim() != b.rowdim():
+        raise LinboxError(5, "a.coldim() != b.rowdim()")
+    pc, pa, pb = c.polmat(), a.polmat(), b.polmat()
+    eq = ctypes.c_int(0)
+    check(lib().lbx_polymat_check_midproduct_u32(ctx.handle, int(p), ctypes.byref(pc), ctypes.byref(pa),
+                                                 ctypes.byref(pb), int(n0), int(n1), ctypes.byref(eq), stream))
+    return bool(eq.value)
+
+
 def check_mul(p, c, a, b, deg=0, x=1, ctx=None, stream=None):
     """check_mul(c, a, b, deg) (matpoly-mult-fft.h:87-163) on the device at the point x
     (the reference evaluates at 1): True when C(x) = A(x).B(x) mod p, with c taken over

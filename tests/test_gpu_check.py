@@ -11,6 +11,12 @@ lbx_set_check / LBX_CHECK_MUL, the run-time form of CHECK_MATPOL_MUL
 * With the switch on, products of every route (single prime in both
   pointwise forms, few primes, constant matrices) pass, a truncated product
   (max_rowdeg) is left unchecked, and the switch can be turned off again.
+* The middle-product check (lbx_polymat_check_midproduct_u32, check_midproduct
+  of matpoly-mult-fft.h:168-262 by random row and column projections, and the
+  CHECK_MATPOL_MIDP switch): it accepts the middle products of the FFT, CRT
+  and naive routes, both operand orders and explicit windows, rejects any
+  checked coefficient changed (first, last, in between, any entry), and
+  ignores the coefficients past deg-hdeg that wrap around.
 """
 import os
 import subprocess
@@ -20,7 +26,8 @@ import numpy as np
 import pytest
 
 import linbox_amd as lbx
-from linbox_amd import PolynomialMatrix, PolynomialMatrixFFTMulDomain, check_mul
+from linbox_amd import (PolynomialMatrix, PolynomialMatrixFFTMulDomain, PolynomialMatrixNaiveMulDomain, check_mul,
+                        check_midproduct)
 from linbox_amd.hip import DeviceBuffer
 from oracle import oracle as orc
 from test_gpu_parity import ctx, rand, say  # noqa: F401
@@ -122,3 +129,73 @@ def test_environment_switch():
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "checked ok 59" in r.stdout
+
+
+# (m, k, n, small operand size, other operand size, c.size, small_left, n0, n1)
+MID_CASES = [
+    (3, 4, 2, 40, 79, 40, True, 0, 0),     # hdeg = deg-hdeg+1 = c.size
+    (2, 5, 3, 40, 79, 40, False, 0, 0),    # b the small operand
+    (4, 3, 4, 17, 64, 32, True, 17, 48),   # PM_Basis window: hdeg = n0, deg = n1
+    (2, 2, 2, 12, 16, 16, True, 12, 16),   # c.size > deg-hdeg+1: coefficients past it wrap, unchecked
+    (6, 8, 5, 300, 700, 300, True, 0, 0),  # split transforms (2^10 points)
+]
+
+
+def _mid(ctx, p, case, naive=False):
+    m, k, n, s_small, s_big, sc, small_left, n0, n1 = case
+    rng = np.random.default_rng(sum(case[:6]) + p % 97)
+    sa, sb = (s_small, s_big) if small_left else (s_big, s_small)
+    A = PolynomialMatrix.from_polfirst(rand(rng, p, m * k * sa).reshape(m * k, sa), m, k)
+    B = PolynomialMatrix.from_polfirst(rand(rng, p, k * n * sb).reshape(k * n, sb), k, n)
+    C = PolynomialMatrix(m, n, sc)
+    if naive:
+        PolynomialMatrixNaiveMulDomain(p, ctx).midproduct(C, A, B, small_left, n0, n1)
+    else:
+        PolynomialMatrixFFTMulDomain(p, ctx).midproduct(C, A, B, small_left, n0, n1)
+    hdeg = n0 or sc
+    deg = n1 or 2 * hdeg - 1
+    return A, B, C, min(sc, deg - hdeg + 1)
+
+
+@pytest.mark.parametrize("p", [998244353, 2013265921, 1000000007, 4294967291])
+def test_midproduct_check_accepts_and_rejects(ctx, p):
+    rng = np.random.default_rng(p % 1009)
+    for case in MID_CASES:
+        say("midproduct check", p, case)
+        A, B, C, count = _mid(ctx, p, case)
+        n0, n1 = case[7], case[8]
+        assert check_midproduct(p, C, A, B, n0, n1, ctx=ctx)
+        m, n, sc = case[0], case[2], case[5]
+        for t in sorted({0, count - 1, int(rng.integers(0, count))}):
+            e = int(rng.integers(0, m * n))
+            i, j = divmod(e, n)
+            v = C.get(i, j, t)
+            C.ref_set(i, j, t, (v + 1 + int(rng.integers(0, p - 1))) % p)
+            assert not check_midproduct(p, C, A, B, n0, n1, ctx=ctx), (case, t, e)
+            C.ref_set(i, j, t, v)
+        if sc > count:  # a wrapped coefficient is outside the middle product
+            C.ref_set(0, 0, sc - 1, (C.get(0, 0, sc - 1) + 1) % p)
+            assert check_midproduct(p, C, A, B, n0, n1, ctx=ctx)
+
+
+def test_midproduct_check_naive_route(ctx):
+    p = 998244353
+    for case in [(3, 4, 2, 3, 6, 3, True, 0, 0), (2, 3, 2, 4, 9, 4, True, 4, 7)]:
+        say("midproduct check naive", case)
+        A, B, C, count = _mid(ctx, p, case, naive=True)
+        assert check_midproduct(p, C, A, B, case[7], case[8], ctx=ctx)
+        C.ref_set(1, 1, 0, (C.get(1, 1, 0) + 5) % p)
+        assert not check_midproduct(p, C, A, B, case[7], case[8], ctx=ctx)
+
+
+def test_midproduct_switch(ctx):
+    """With the switch on, middle products of every route run their check and pass."""
+    ctx.set_check(True)
+    try:
+        for p in (998244353, 1000000007, 4294967291):
+            for case in MID_CASES[:3]:
+                say("midproduct switch", p, case)
+                _mid(ctx, p, case)
+        _mid(ctx, 998244353, (3, 4, 2, 3, 6, 3, True, 0, 0), naive=True)
+    finally:
+        ctx.set_check(False)
