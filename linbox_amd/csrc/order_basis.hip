@@ -820,6 +820,13 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
                     const uint32_t *q = part + slot * G * mn + cidx[r] * n + lane;
                     uint64_t sum = 0;
                     int h = 0;
+                    for (; h + 16 <= G; h += 16) {
+                        uint32_t t[16];
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) t[j] = q[(h + j) * mn];
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) sum += t[j];
+                    }
                     for (; h + 8 <= G; h += 8) {
                         uint32_t t[8];
 #pragma unroll
