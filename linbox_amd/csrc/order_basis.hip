@@ -632,7 +632,8 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned count)
         const unsigned target = count * gridDim.x;
         unsigned polls = 0;
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            // the failure flag only matters once a limit ran out: read it now and then
+            if ((polls & 255) == 255 && __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
             if (++polls == kMcSpinLimit) {
                 __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -909,7 +910,10 @@ __global__ void __launch_bounds__(kMbThreads) m_basis_cols_kernel(
                     mine = true;
                     break;
                 }
-                if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                // the failure flag (global memory) only matters once a poll limit
+                // ran out somewhere: read it now and then, not on every poll
+                if ((polls & 255) == 255 && __hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    break;
                 if (++polls == kMcSpinLimit) {
                     __hip_atomic_store(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
