@@ -252,6 +252,15 @@ static size_t chunk_bytes()
     return (size_t)2 << 30;
 }
 
+// Whether several chunks of one direction overlap over two streams (the
+// default) or run one after the other on the caller's stream
+// (LBX_CHUNK_PIPE=0: tuning and tests).
+static bool chunk_pipeline()
+{
+    const char *e = std::getenv("LBX_CHUNK_PIPE");
+    return !(e && !std::strcmp(e, "0"));
+}
+
 // Two-stream chunk pipeline: chunk c's producer runs on the caller's stream s,
 // its consumer on ctx->aux after the producer's event; the producer of chunk
 // c+2 (same scratch slot) waits for the consumer of chunk c.  Events 2c and
@@ -378,8 +387,9 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
             rpc = std::max(mpad, npad);
             cchunk = m * n;
         }
-        fwd_pipe = (mpad + rpc - 1) / rpc > 1 || (npad + rpc - 1) / rpc > 1;
-        inv_pipe = m * n > cchunk;
+        const bool pipe_ok = chunk_pipeline();
+        fwd_pipe = pipe_ok && ((mpad + rpc - 1) / rpc > 1 || (npad + rpc - 1) / rpc > 1);
+        inv_pipe = pipe_ok && m * n > cchunk;
         // a slot holds the largest chunk either direction uses; one slot suffices
         // when nothing is pipelined
         slot_words = std::max(std::min(rpc, std::max(m, n)) * k, std::min(cchunk, m * n)) * N;

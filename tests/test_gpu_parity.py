@@ -148,18 +148,21 @@ def test_mul_every_transform_length(ctx, p):
             assert np.array_equal(got.reshape(-1), ref), (L, method)
 
 
+@pytest.mark.parametrize("pipe", ["1", "0"])
 @pytest.mark.parametrize("chunk_bytes", [4096, 1 << 20])
-def test_mul_chunked_pipeline(ctx, chunk_bytes, monkeypatch):
-    """Tiny intermediate chunks force many head/tail chunks, both scratch slots,
-    padding rows inside chunks and the two-stream hand-offs."""
+def test_mul_chunked_pipeline(ctx, chunk_bytes, pipe, monkeypatch):
+    """Tiny intermediate chunks force many head/tail chunks and padding rows
+    inside chunks: with the two-stream hand-offs over both scratch slots, and
+    (LBX_CHUNK_PIPE=0) in order on one stream through a single slot."""
     monkeypatch.setenv("LBX_CHUNK_BYTES", str(chunk_bytes))
+    monkeypatch.setenv("LBX_CHUNK_PIPE", pipe)
     for (p, m, k, n, sa, sb) in [(2013265921, 37, 20, 45, 300, 200), (998244353, 64, 64, 33, 129, 128),
                                  (469762049, 5, 130, 7, 1000, 1100)]:
         rng = np.random.default_rng(m * n + chunk_bytes)
         a, b = rand(rng, p, m * k * sa), rand(rng, p, k * n * sb)
         st, ref = orc.mul_fft_prime(p, m, k, n, a, sa, b, sb)
         assert st == 0
-        say("chunked", chunk_bytes, p, m, k, n, sa, sb)
+        say("chunked", chunk_bytes, pipe, p, m, k, n, sa, sb)
         got = gpu_mul(ctx, p, m, k, n, a, sa, b, sb, lbx.METHOD_INT8_MFMA)
         assert np.array_equal(got.reshape(-1), ref)
 
