@@ -55,7 +55,13 @@ LBX_HD uint32_t shoup_mul(uint32_t x, uint32_t w, uint32_t wp, uint32_t p)
 LBX_HD uint32_t shoup_pre(uint32_t w, uint32_t p) { return (uint32_t)(((uint64_t)w << 32) / p); }
 
 LBX_HD uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p) { return csub(a + b, p); }
-LBX_HD uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) { return csub(a + (p - b), p); }
+// a, b < p < 2^31: a - b wraps to above p exactly when a < b, and then
+// a - b + p (which wraps back) is the reduced value (sub, add, min).
+LBX_HD uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p)
+{
+    const uint32_t d = a - b, e = d + p;
+    return e < d ? e : d;
+}
 
 // Balanced signed base-256 digits of the centred representative of x:
 // c = x or x-p, |c| < p/2 < 2^30, and c = sum_s d_s 2^(8s) with d_s in

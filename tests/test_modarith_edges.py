@@ -58,6 +58,19 @@ def test_csub_and_add_sub(lib):
                 assert lib.lbx_test_sub_mod(a, b, p) == (a - b) % p
 
 
+def test_add_sub_random_and_largest_word(lib):
+    """add_mod and sub_mod on random canonical pairs, and for 2^31 - 1, the
+    largest modulus their contract (a, b < p < 2^31) allows, where the wrapped
+    difference and the sum come closest to 2^32."""
+    rng = random.Random(7)
+    for p in PRIMES + [(1 << 31) - 1]:
+        pairs = [(a, b) for a in edges(p) for b in edges(p)]
+        pairs += [(rng.randrange(p), rng.randrange(p)) for _ in range(2000)]
+        for a, b in pairs:
+            assert lib.lbx_test_add_mod(a, b, p) == (a + b) % p, (p, a, b)
+            assert lib.lbx_test_sub_mod(a, b, p) == (a - b) % p, (p, a, b)
+
+
 def test_shoup_products(lib):
     rng = random.Random(1)
     for p in PRIMES:
