@@ -453,8 +453,8 @@ lbx_status build_plan(lbx_ctx *ctx, const std::vector<lbx_job> &jobs, const std:
         J.chat = w + ((x.m * x.k + x.k * x.n) << L);
         uint32_t root;
         if ((st = primitive_root(x.p, L, &root)) != LBX_OK) return fail(st);
-        if ((st = twiddles(ctx, x.p, L, root, &J.twf)) != LBX_OK) return fail(st);
-        if ((st = twiddles(ctx, x.p, L, invroot(x.p, L, root), &J.twi)) != LBX_OK) return fail(st);
+        if ((st = twiddles(ctx, x.p, L, root, kTwMont, &J.twf)) != LBX_OK) return fail(st);
+        if ((st = twiddles(ctx, x.p, L, invroot(x.p, L, root), kTwShoup, &J.twi)) != LBX_OK) return fail(st);
         const uint32_t inv_n = invmod_h((uint32_t)(((uint64_t)1 << L) % x.p), x.p);
         J.pc = plain_const(x.p, inv_n);
         J.form = 0;

@@ -37,10 +37,12 @@ __global__ void __launch_bounds__(256) fill_kernel(DevView out, size_t entries, 
 }
 
 // pow_w table of init_powers (fft-integral.inl:1614-1650) built on the
-// device: entry idx in [N-2h, N-h) is w^((idx-(N-2h)) N/(2h)), with its Shoup
-// constant, the same words the host-built cached table holds.
-__global__ void __launch_bounds__(256) build_twiddles_kernel(uint2 *tw, uint32_t p, unsigned L, uint32_t w)
+// device: entry idx in [N-2h, N-h) is w^((idx-(N-2h)) N/(2h)), as the pair of
+// words `fmt` asks for, the same words the host-built cached table holds.
+__global__ void __launch_bounds__(256) build_twiddles_kernel(uint2 *tw, uint32_t p, unsigned L, uint32_t w,
+                                                             TwFormat fmt)
 {
+    const uint32_t pinv = inv_mod32(p);
     const size_t n = (size_t)1 << L;
     const size_t count = n > 1 ? n - 1 : 1;
     for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < count; idx += (size_t)gridDim.x * 256) {
@@ -56,7 +58,12 @@ __global__ void __launch_bounds__(256) build_twiddles_kernel(uint2 *tw, uint32_t
             base = base * base % p;
             e >>= 1;
         }
-        tw[idx] = make_uint2((uint32_t)acc, (uint32_t)((acc << 32) / p));
+        if (fmt == kTwMont) {
+            const uint32_t wm = mont_wm((uint32_t)acc, p);
+            tw[idx] = make_uint2(wm, mont_wq(wm, pinv));
+        } else {
+            tw[idx] = make_uint2((uint32_t)acc, (uint32_t)((acc << 32) / p));
+        }
     }
 }
 
@@ -88,10 +95,10 @@ hipError_t launch_fill(const DevView &out, size_t entries, size_t t0, size_t cou
     return hipGetLastError();
 }
 
-hipError_t launch_build_twiddles(uint2 *tw, uint32_t p, unsigned L, uint32_t w, hipStream_t s)
+hipError_t launch_build_twiddles(uint2 *tw, uint32_t p, unsigned L, uint32_t w, TwFormat fmt, hipStream_t s)
 {
     const size_t n = (size_t)1 << L;
-    hipLaunchKernelGGL(build_twiddles_kernel, dim3(grid_for(n > 1 ? n - 1 : 1)), dim3(256), 0, s, tw, p, L, w);
+    hipLaunchKernelGGL(build_twiddles_kernel, dim3(grid_for(n > 1 ? n - 1 : 1)), dim3(256), 0, s, tw, p, L, w, fmt);
     return hipGetLastError();
 }
 

@@ -100,10 +100,10 @@ uint32_t invroot(uint32_t p, unsigned L, uint32_t w)
 }
 
 // Twiddle table in the pow_w layout of init_powers (fft-integral.inl:1614-1650),
-// cached per (p, L, root).
-lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint2 **out)
+// cached per (p, L, root, format).
+lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, TwFormat fmt, const uint2 **out)
 {
-    auto key = std::make_tuple(p, L, w);
+    auto key = std::make_tuple(p, L, w, (int)fmt);
     auto it = ctx->twiddles.find(key);
     if (it != ctx->twiddles.end()) {
         *out = it->second;
@@ -117,7 +117,15 @@ lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint
     for (size_t kk = 2; kk <= n / 2; kk <<= 1)
         for (size_t i = 0; i < n / 2; i += kk, idx++) pw[idx] = pw[i];
     std::vector<uint2> tw(pw.size());
-    for (size_t i = 0; i < pw.size(); ++i) tw[i] = make_uint2(pw[i], shoup_pre(pw[i], p));
+    const uint32_t pinv = inv_mod32(p);
+    for (size_t i = 0; i < pw.size(); ++i) {
+        if (fmt == kTwMont) {
+            const uint32_t wm = mont_wm(pw[i], p);
+            tw[i] = make_uint2(wm, mont_wq(wm, pinv));
+        } else {
+            tw[i] = make_uint2(pw[i], shoup_pre(pw[i], p));
+        }
+    }
     uint2 *d = nullptr;
     LBX_TRY_HIP(hipMalloc(&d, sizeof(uint2) * tw.size()));
     LBX_TRY_HIP(hipMemcpy(d, tw.data(), sizeof(uint2) * tw.size(), hipMemcpyHostToDevice));
@@ -349,18 +357,20 @@ lbx_status fft_prime_product(lbx_ctx *ctx, uint32_t p, unsigned L, const DevView
     uint32_t w;
     LBX_TRY(primitive_root(p, L, &w));
     const uint32_t wi = invroot(p, L, w);
-    const uint2 *twf, *twi;
-    LBX_TRY(ws.twiddle_table(p, L, w, &twf, s));
-    LBX_TRY(ws.twiddle_table(p, L, wi, &twi, s));
     // product: both operands forward with w, inverse with w^-1; midproduct
     // (wordsize-fast.inl:277-313): small operand with w, other with w^-1,
-    // inverse with w, small operand reversed over its first hdeg coefficients
+    // inverse with w, small operand reversed over its first hdeg coefficients.
+    // Forward transforms read Montgomery tables, inverse ones Shoup tables.
+    const uint2 *twf, *twc;
+    LBX_TRY(ws.twiddle_table(p, L, w, kTwMont, &twf, s));
+    LBX_TRY(ws.twiddle_table(p, L, mid ? w : wi, kTwShoup, &twc, s));
     DevView a = a_in, b = b_in;
-    const uint2 *twa = twf, *twb = twf, *twc = twi;
+    const uint2 *twa = twf, *twb = twf;
     void *rev = nullptr;
     if (mid) {
-        twc = twf;
-        (mid->small_left ? twb : twa) = twi;
+        const uint2 *twfi;
+        LBX_TRY(ws.twiddle_table(p, L, wi, kTwMont, &twfi, s));
+        (mid->small_left ? twb : twa) = twfi;
         const size_t se = mid->small_left ? m * k : k * n;
         LBX_TRY(ws.get("mid_rev", se * mid->hdeg * 4, &rev));
     }
@@ -548,12 +558,13 @@ lbx_status ArenaWorkspace::get(const char *, size_t bytes, void **out)
     return LBX_OK;
 }
 
-lbx_status ArenaWorkspace::twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t s)
+lbx_status ArenaWorkspace::twiddle_table(uint32_t p, unsigned L, uint32_t w, TwFormat fmt, const uint2 **out,
+                                         hipStream_t s)
 {
     const size_t n = (size_t)1 << L;
     void *d;
     LBX_TRY(get("twiddles", sizeof(uint2) * (n > 1 ? n - 1 : 1), &d));
-    LBX_TRY_HIP(launch_build_twiddles((uint2 *)d, p, L, w, s));
+    LBX_TRY_HIP(launch_build_twiddles((uint2 *)d, p, L, w, fmt, s));
     *out = (const uint2 *)d;
     return LBX_OK;
 }
@@ -858,7 +869,7 @@ static lbx_status ntt_entry(lbx_ctx *ctx, uint32_t p, unsigned lpts, uint32_t ro
     hipStream_t s = (hipStream_t)stream;
     StreamOrder stream_order(ctx, s);
     const uint2 *tw;
-    LBX_TRY(twiddles(ctx, p, lpts, w, &tw));
+    LBX_TRY(twiddles(ctx, p, lpts, w, inverse ? kTwShoup : kTwMont, &tw));
     lbx_polmat v{(void *)data, LBX_POLFIRST, count, 1, N, N, 0};
     Staged sd;
     LBX_TRY(stage_in(ctx, "stage_a", &v, 4, true, &sd, s));

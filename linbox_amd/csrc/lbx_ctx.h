@@ -29,7 +29,7 @@ struct lbx_ctx {
     int device = 0;
     std::mutex mu;
     std::map<std::string, lbx::Buffer> bufs;
-    std::map<std::tuple<uint32_t, unsigned, uint32_t>, uint2 *> twiddles;
+    std::map<std::tuple<uint32_t, unsigned, uint32_t, int>, uint2 *> twiddles;
     std::map<std::string, lbx::MpPlan *> mp_plans;
     std::map<std::string, lbx::BatchPlan *> batch_plans;
     int tile_tuned[3] = {-1, -1, -1};  // batch pointwise tile per size class, -1 = not tuned yet
@@ -138,10 +138,11 @@ private:
 };
 
 // Root of unity of order 2^k (fft-utils.h:83-104), its inverse (fft.h:119-130)
-// and the cached device twiddle table of a root (fft-integral.inl:1614-1650).
+// and the cached device twiddle table of a root (fft-integral.inl:1614-1650)
+// in the format of the transforms that read it.
 lbx_status primitive_root(uint32_t p, unsigned k, uint32_t *w_out);
 uint32_t invroot(uint32_t p, unsigned L, uint32_t w);
-lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, const uint2 **out);
+lbx_status twiddles(lbx_ctx *ctx, uint32_t p, unsigned L, uint32_t w, TwFormat fmt, const uint2 **out);
 
 // Named device workspace, grown on demand and kept between calls.
 lbx_status ensure(lbx_ctx *ctx, const char *name, size_t bytes, void **out);
@@ -194,7 +195,8 @@ class Workspace {
 public:
     virtual ~Workspace() = default;
     virtual lbx_status get(const char *name, size_t bytes, void **out) = 0;
-    virtual lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t s) = 0;
+    virtual lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, TwFormat fmt, const uint2 **out,
+                                     hipStream_t s) = 0;
     // chunked head/tail pipelining on a second stream (and its events)
     virtual bool may_pipeline() const = 0;
     // stage and kernel timing events (lbx_profile_enable)
@@ -205,9 +207,9 @@ class CtxWorkspace final : public Workspace {
 public:
     explicit CtxWorkspace(lbx_ctx *ctx) : ctx_(ctx) {}
     lbx_status get(const char *name, size_t bytes, void **out) override { return ensure(ctx_, name, bytes, out); }
-    lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t) override
+    lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, TwFormat fmt, const uint2 **out, hipStream_t) override
     {
-        return twiddles(ctx_, p, L, w, out);
+        return twiddles(ctx_, p, L, w, fmt, out);
     }
     bool may_pipeline() const override { return true; }
     bool timing() const override { return ctx_->profiling != 0; }
@@ -220,7 +222,8 @@ class ArenaWorkspace final : public Workspace {
 public:
     ArenaWorkspace(void *base, size_t cap) : base_((uint8_t *)base), cap_(cap) {}
     lbx_status get(const char *, size_t bytes, void **out) override;
-    lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, const uint2 **out, hipStream_t s) override;
+    lbx_status twiddle_table(uint32_t p, unsigned L, uint32_t w, TwFormat fmt, const uint2 **out,
+                             hipStream_t s) override;
     bool may_pipeline() const override { return false; }
     bool timing() const override { return false; }
     size_t used() const { return used_; }

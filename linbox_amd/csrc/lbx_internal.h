@@ -135,7 +135,7 @@ struct BatchJob {
     const uint32_t *a, *b;
     uint32_t *c;
     uint32_t *ahat, *bhat, *chat;  // m*k, k*n, m*n rows of 2^L words
-    const uint2 *twf, *twi;        // twiddles of the root and of its inverse
+    const uint2 *twf, *twi;        // root: forward (Montgomery) table; its inverse: inverse (Shoup) table
     PlainConst pc;                 // plain epilogue constants with scale 2^-L
     uint8_t *alimb, *blimb;        // INT8: per-point limb blocks of A and B^T (limb_block_bytes each)
     uint32_t *cmat;                // INT8: point-major product [t][i*n+j]
@@ -194,9 +194,13 @@ hipError_t launch_gather(const DevView &in, size_t entries, size_t count, uint32
 // out[e*hdeg + i] = in(e, hdeg-1-i) (0 past the view's size), i < hdeg: the
 // reversal of the small midproduct operand (wordsize-fast.inl:228-235).
 hipError_t launch_reverse_gather(const DevView &in, size_t entries, size_t hdeg, uint32_t *out, hipStream_t s);
-// The pow_w twiddle table of root w of order 2^L (N-1 entries of {w^e, Shoup
-// constant}) computed on the device, identical to the host-built one.
-hipError_t launch_build_twiddles(uint2 *tw, uint32_t p, unsigned L, uint32_t w, hipStream_t s);
+// Word pairs of a twiddle table entry w^e: the inverse (DIT) transforms take
+// {w^e, Shoup constant floor(w^e 2^32 / p)}, the forward (DIF) ones the signed
+// Montgomery pair {w^e 2^32 mod p, that times p^-1 mod 2^32} (modarith.h).
+enum TwFormat : int { kTwShoup = 0, kTwMont = 1 };
+// The pow_w twiddle table of root w of order 2^L (N-1 entries) computed on
+// the device, identical to the host-built one of the same format.
+hipError_t launch_build_twiddles(uint2 *tw, uint32_t p, unsigned L, uint32_t w, TwFormat fmt, hipStream_t s);
 // view(e, t) = value for t0 <= t < t0+count (a graph-safe fill, no memset node).
 hipError_t launch_fill(const DevView &out, size_t entries, size_t t0, size_t count, uint32_t value, hipStream_t s);
 // view(e, t) = in[e*stride + t] for t < count.

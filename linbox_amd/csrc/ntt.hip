@@ -152,12 +152,13 @@ struct Blk {
     static constexpr size_t LDS_BYTES = (size_t)64 * STRIDE * 4;
 };
 
-// DIF butterfly on canonical words; tw index 0 of every span table is w^0 = 1.
+// DIF butterfly on canonical words with Montgomery twiddle pairs; tw index 0
+// of every span table is w^0 = 1.
 __device__ __forceinline__ void dif_bfly(uint32_t &A, uint32_t &B, const uint2 t, bool trivial, uint32_t p)
 {
     const uint32_t a = A, b = B;
     A = add_mod(a, b, p);
-    B = trivial ? sub_mod(a, b, p) : shoup_mul(a + (p - b), t.x, t.y, p);
+    B = trivial ? sub_mod(a, b, p) : mont_mul((int32_t)(a - b), t.x, t.y, p);
 }
 
 __device__ __forceinline__ void dit_bfly(uint32_t &A, uint32_t &B, const uint2 t, bool trivial, uint32_t p)
@@ -510,7 +511,7 @@ __global__ void __launch_bounds__(256) dif_cols_kernel(DevView in, size_t entrie
             } else {
                 const uint2 w = tws[j + (size_t)(i & (half - 1)) * J];
                 x[i] = add_mod(a, b, p);
-                x[i + half] = shoup_mul(a + (p - b), w.x, w.y, p);
+                x[i + half] = mont_mul((int32_t)(a - b), w.x, w.y, p);
             }
         }
     }

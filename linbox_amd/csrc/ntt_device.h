@@ -129,11 +129,11 @@ __device__ __forceinline__ void dif_pass(uint32_t (&x)[Cfg<L>::R], int tp, const
                 } else {
                     const uint2 t = tws[off + ((q & (half - 1)) << HL)];
                     if (HALFZERO && s == 0) {  // right half of the input is zero
-                        B = shoup_mul(A, t.x, t.y, p);
+                        B = mont_mul((int32_t)A, t.x, t.y, p);
                     } else {
                         const uint32_t a = A, b = B;
                         A = add_mod(a, b, p);
-                        B = shoup_mul(a + (p - b), t.x, t.y, p);
+                        B = mont_mul((int32_t)(a - b), t.x, t.y, p);
                     }
                 }
             }
@@ -434,11 +434,11 @@ __device__ __forceinline__ void dif_head_persistent(const DevView &in, size_t en
                 if (q & half) continue;
                 const uint2 w = t0[first_tw_base<H::RP0>(st) + (q & (half - 1))];
                 if (HALFZERO && st == 0) {
-                    x[q + half] = shoup_mul(x[q], w.x, w.y, p);
+                    x[q + half] = mont_mul((int32_t)x[q], w.x, w.y, p);
                 } else {
                     const uint32_t a = x[q], b = x[q + half];
                     x[q] = add_mod(a, b, p);
-                    x[q + half] = shoup_mul(a + (p - b), w.x, w.y, p);
+                    x[q + half] = mont_mul((int32_t)(a - b), w.x, w.y, p);
                 }
             }
         }

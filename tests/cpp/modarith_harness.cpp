@@ -16,6 +16,12 @@ uint32_t lbx_test_shoup_lazy(uint32_t x, uint32_t w, uint32_t p) { return shoup_
 uint32_t lbx_test_shoup_mul(uint32_t x, uint32_t w, uint32_t p) { return shoup_mul(x, w, shoup_pre(w, p), p); }
 uint32_t lbx_test_add_mod(uint32_t a, uint32_t b, uint32_t p) { return add_mod(a, b, p); }
 uint32_t lbx_test_sub_mod(uint32_t a, uint32_t b, uint32_t p) { return sub_mod(a, b, p); }
+uint32_t lbx_test_mont_mul(int32_t x, uint32_t w, uint32_t p)
+{
+    const uint32_t wm = mont_wm(w, p);
+    return mont_mul(x, wm, mont_wq(wm, inv_mod32(p)), p);
+}
+uint32_t lbx_test_inv_mod32(uint32_t p) { return inv_mod32(p); }
 uint32_t lbx_test_to_limbs(uint32_t x, uint32_t p) { return to_limbs(x, p); }
 uint32_t lbx_test_powmod(uint32_t a, uint64_t e, uint32_t p) { return powmod_h(a, e, p); }
 uint32_t lbx_test_invmod(uint32_t a, uint32_t p) { return invmod_h(a, p); }

@@ -33,6 +33,7 @@ def lib():
     u32, u64 = ctypes.c_uint32, ctypes.c_uint64
     for name, args in [("csub", [u32, u32]), ("shoup_pre", [u32, u32]), ("shoup_lazy", [u32, u32, u32]),
                        ("shoup_mul", [u32, u32, u32]), ("add_mod", [u32, u32, u32]), ("sub_mod", [u32, u32, u32]),
+                       ("mont_mul", [ctypes.c_int32, u32, u32]), ("inv_mod32", [u32]),
                        ("to_limbs", [u32, u32]), ("powmod", [u32, u64, u32]), ("invmod", [u32, u32]),
                        ("combine", [ctypes.POINTER(ctypes.c_int32), u32, u32]),
                        ("plain_dot", [ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.c_int, u32, u32]),
@@ -82,6 +83,25 @@ def test_shoup_products(lib):
                 lazy = lib.lbx_test_shoup_lazy(x, w, p)
                 assert 0 <= lazy < 2 * p and lazy % p == x * w % p, (p, w, x)
                 assert lib.lbx_test_shoup_mul(x, w, p) == x * w % p, (p, w, x)
+
+
+def test_signed_montgomery(lib):
+    """The forward transforms' twiddle product: x*w mod p in [0,p) for signed
+    |x| < p (the difference a - b of two canonical words) and w < p, with the
+    twiddle in Montgomery form; p^-1 mod 2^32 by Newton's iteration.  Also for
+    2^31 - 1, the largest odd modulus below 2^31, where |x wm| and |m p|
+    come closest to 2^62."""
+    rng = random.Random(11)
+    for p in PRIMES + [(1 << 31) - 1]:
+        assert lib.lbx_test_inv_mod32(p) * p % (1 << 32) == 1
+        ws = edges(p) + [rng.randrange(p) for _ in range(8)]
+        xs = [0, 1, -1, p - 1, -(p - 1), p // 2, -(p // 2)] + [rng.randrange(-p + 1, p) for _ in range(40)]
+        for w in ws:
+            for x in xs:
+                assert lib.lbx_test_mont_mul(x, w, p) == x * w % p, (p, w, x)
+        for _ in range(3000):
+            a, b, w = rng.randrange(p), rng.randrange(p), rng.randrange(p)
+            assert lib.lbx_test_mont_mul(a - b, w, p) == (a - b) * w % p, (p, a, b, w)
 
 
 def test_balanced_digits(lib):
