@@ -281,8 +281,9 @@ __global__ void __launch_bounds__(Blk<BL>::T) dif_tail_limbs_kernel(const uint32
             for (int i = 0; i < 4; ++i) w4[i] = col[i * B::STRIDE];
         }
         const size_t t = (size_t)b * P + lam;
-        *reinterpret_cast<uint4 *>(Y + t * block_bytes + ((size_t)(row0 + r) * kchunks + c) * 256 + ch * 16) =
-            make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        store_stream4(reinterpret_cast<uint32_t *>(Y + t * block_bytes + ((size_t)(row0 + r) * kchunks + c) * 256 +
+                                                   ch * 16),
+                      w4[0], w4[1], w4[2], w4[3]);
     }
 }
 
