@@ -32,6 +32,27 @@ __host__ __device__ __forceinline__ size_t view_index(const DevView &v, size_t e
     return i * v.cols * v.stor + j + tt * v.cols;
 }
 
+// A pointer that a kernel loads from memory (a job descriptor's field), for
+// use as a global-memory pointer.  The compiler cannot tell where such a
+// pointer points and emits flat instructions for its accesses, and flat loads
+// count against the same wait counter as LDS operations, so every LDS wait
+// would also wait for them.  The cast to the global address space, held
+// behind an empty asm so it is not folded away, lets every access derived
+// from the pointer use global instructions.  p must point to global memory
+// and be the same across the wave.
+template <class T>
+__device__ __forceinline__ T *global_ptr(T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(1))) T global_t;
+    global_t *g = (global_t *)p;
+    asm("" : "+s"(g));
+    return (T *)g;
+#else
+    return p;
+#endif
+}
+
 constexpr int kMaxLdsLog = 15;  // LDS-resident transforms up to 2^15 points
 constexpr int kMaxLog = 27;     // longest transform: 2^27 points (the 2-adic valuation of 2013265921 - 1)
 

@@ -418,27 +418,29 @@ __global__ void __launch_bounds__(kBatchNttThreads, (batch_waves<L, INVERSE>()))
         if constexpr (!INVERSE) {
             const bool isa = w.op == 0;
             DevView in;
-            in.base = const_cast<uint32_t *>(isa ? J.a : J.b);
+            in.base = global_ptr(const_cast<uint32_t *>(isa ? J.a : J.b));
             in.layout = 0;
             in.rows = isa ? J.m : J.k;
             in.cols = isa ? J.k : J.n;
             in.stor = in.size = isa ? J.sa : J.sb;
             in.shift = 0;
             const EntryMap id{0, ~(size_t)0, 0};
-            uint32_t *out = isa ? J.ahat : J.bhat;
+            uint32_t *out = global_ptr(isa ? J.ahat : J.bhat);
+            const uint2 *tw = global_ptr(J.twf);
             if (in.size * 2 <= (size_t)C::N)
-                dif_body<L, L, true>(in, e_end, id, J.twf, J.p, out, e_base, tid, lds);
+                dif_body<L, L, true>(in, e_end, id, tw, J.p, out, e_base, tid, lds);
             else
-                dif_body<L, L, false>(in, e_end, id, J.twf, J.p, out, e_base, tid, lds);
+                dif_body<L, L, false>(in, e_end, id, tw, J.p, out, e_base, tid, lds);
         } else {
             DevView out;
-            out.base = J.c;
+            out.base = global_ptr(J.c);
             out.layout = 0;
             out.rows = J.m;
             out.cols = J.n;
             out.stor = out.size = J.sc;
             out.shift = 0;
-            dit_body<L, 0, false>(J.chat, e_end, J.twi, J.p, out, 0, J.sc, 1u, 0u, e_base, tid, lds);
+            dit_body<L, 0, false>(global_ptr(J.chat), e_end, global_ptr(J.twi), J.p, out, 0, J.sc, 1u, 0u, e_base, tid,
+                                  lds);
         }
         __syncthreads();  // LDS is reused by the next item
     }

@@ -100,13 +100,15 @@ __global__ void __launch_bounds__(256) batch_pointwise_kernel(const BatchJob *__
         const BatchJob &J = jobs[w.job];
         const size_t npts = (size_t)1 << J.L;
         const size_t t = (size_t)w.pt0 + threadIdx.x;
+        const uint32_t *A = global_ptr(J.ahat), *B = global_ptr(J.bhat);
+        uint32_t *Cp = global_ptr(J.chat);
         if (t < npts) {
             if (J.tile == 0)
-                pw_plain_body<2, 2>(J.ahat, J.bhat, J.chat, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 2, w.tj * 2);
+                pw_plain_body<2, 2>(A, B, Cp, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 2, w.tj * 2);
             else if (J.tile == 1)
-                pw_plain_body<4, 4>(J.ahat, J.bhat, J.chat, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 4, w.tj * 4);
+                pw_plain_body<4, 4>(A, B, Cp, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 4, w.tj * 4);
             else
-                pw_plain_body<8, 8>(J.ahat, J.bhat, J.chat, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 8, w.tj * 8);
+                pw_plain_body<8, 8>(A, B, Cp, J.m, J.k, J.n, npts, J.p, J.pc, t, w.ti * 8, w.tj * 8);
         }
     }
 }
@@ -590,7 +592,8 @@ __global__ void __launch_bounds__(256) batch_limbs_kernel(const BatchJob *__rest
         const int rows = isa ? J.m : J.n, kdim = J.k;
         const int kchunks = (kdim + 63) / 64;
         const size_t block_bytes = limb_block_bytes_dev(rows, kdim);
-        limbs_tile(isa ? J.ahat : J.bhat, isa ? J.alimb : J.blimb, rows, kdim, isa ? 1 : 0, (size_t)1 << J.L, J.p,
+        limbs_tile(global_ptr(isa ? J.ahat : J.bhat), global_ptr(isa ? J.alimb : J.blimb), rows, kdim, isa ? 1 : 0,
+                   (size_t)1 << J.L, J.p,
                    block_bytes, kchunks, w.t0, (int)(w.rc % kchunks), (int)(w.rc / kchunks), threadIdx.x, tile);
     }
 }
@@ -609,10 +612,12 @@ __global__ void __launch_bounds__(256) batch_mfma_kernel(const BatchJob *__restr
         const int mpad = (J.m + 31) & ~31, npad = (J.n + 31) & ~31, kchunks = (J.k + 63) / 64;
         const int tiles_n = npad / 32, tiles = (mpad / 32) * tiles_n;
         const size_t a_bytes = limb_block_bytes_dev(J.m, J.k), b_bytes = limb_block_bytes_dev(J.n, J.k);
+        const uint8_t *al = global_ptr(J.alimb), *bl = global_ptr(J.blimb);
+        uint32_t *cm = global_ptr(J.cmat);
         for (uint32_t u = wave; u < w.count * (uint32_t)tiles; u += 4) {
             const size_t t = w.t0 + u / tiles;
             const int tile = (int)(u % tiles);
-            mfma_tile_global(J.alimb + t * a_bytes, J.blimb + t * b_bytes, J.cmat + t * (size_t)J.m * J.n, J.m, J.n,
+            mfma_tile_global(al + t * a_bytes, bl + t * b_bytes, cm + t * (size_t)J.m * J.n, J.m, J.n,
                              kchunks, (tile / tiles_n) * 32, (tile % tiles_n) * 32, J.p, J.mc, lane & 15, lane >> 4);
         }
     }
@@ -629,7 +634,8 @@ __global__ void __launch_bounds__(256) batch_point_to_entry_kernel(const BatchJo
         if (it >= nitems) return;
         const TrItem w = items[it];
         const BatchJob &J = jobs[w.job];
-        point_to_entry_tile(J.cmat, J.chat, (size_t)J.m * J.n, (size_t)1 << J.L, w.e0, w.t0, threadIdx.x, tile);
+        point_to_entry_tile(global_ptr(J.cmat), global_ptr(J.chat), (size_t)J.m * J.n, (size_t)1 << J.L, w.e0, w.t0,
+                            threadIdx.x, tile);
     }
 }
 
