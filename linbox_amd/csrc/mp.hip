@@ -210,6 +210,23 @@ struct MpLayout {
     __host__ __device__ int words() const { return mq() + r * nq; }
 };
 
+// Register widths (RM primes, NLM words) of the CRT kernel mp_product
+// launches for r primes and an nl-word modulus, {0, 0} for the general
+// kernel; a plan's constants are laid out for that width (nq = NLM).  Narrow
+// moduli get narrow kernels: a word-size P (3 primes) takes 2-word
+// arithmetic and remainders instead of 8-word ones.
+struct CrtWidth {
+    int rm, nlm;
+};
+inline CrtWidth crt_width(int r, int nl)
+{
+    if (r <= 4 && nl <= 2) return {4, 2};
+    if (r <= 8 && nl <= 4) return {8, 4};
+    if (r <= 16 && nl <= 8) return {16, 8};
+    if (r <= 32 && nl <= 16) return {32, 16};
+    return {0, 0};
+}
+
 // Word index of element idx (entry idx / size, coefficient idx % size) of a
 // view.  A polfirst view without window (stride = size, no shift) maps idx to
 // itself, which spares the per-element 64-bit division.
@@ -332,6 +349,7 @@ __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__res
     uint32_t q[NLM];
 #pragma unroll
     for (int l = 0; l < NLM; ++l) q[l] = K[lay.Q() + l];
+    const uint32_t qinv = div21_inverse(q[NLM - 1]);  // Q is normalized: its top word has its top bit set
     for (size_t idx = (size_t)blockIdx.x * 128 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 128) {
         uint32_t v[RM], rv[RM];
         uint32_t acc[NLM + 2];
@@ -366,7 +384,7 @@ __global__ void __launch_bounds__(128) mp_crt_small_kernel(const uint32_t *__res
             }
         }
         // acc < r 2^31 Q <= 2^36 Q: acc mod Q = (X mod P) 2^(32 off + s) in acc[0..NLM)
-        knuth_rem2<NLM>(acc, q);
+        knuth_rem2<NLM>(acc, q, qinv);
         uint32_t *dst = out.base + elem_index(out, idx, size) * nl;
 #pragma unroll
         for (int l = 0; l < NLM; ++l)
@@ -493,7 +511,7 @@ lbx_status get_plan(lbx_ctx *ctx, const uint32_t *Pw, int nl, size_t k, size_t m
     if (!generate_primes((uint64_t)1 << 31, bound, lpts, primes)) return LBX_ERR_NOT_ENOUGH_PRIMES;
     const int r = (int)primes.size();
     // word count of the register kernel mp_product launches for (r, nl), 0 for the general one
-    const int nq = (r <= 16 && nl <= 8) ? 8 : ((r <= 32 && nl <= 16) ? 16 : 0);
+    const int nq = crt_width(r, nl).nlm;
     const MpLayout lay{r, nl, nq};
     std::vector<uint32_t> K((size_t)lay.words(), 0);
     // P moved to the top of nq words with its top bit set: 2^(32 off + s)
@@ -713,10 +731,17 @@ lbx_status mp_product(lbx_ctx *ctx, const uint32_t *Pw, int nl, const DevView &a
             LBX_TRY(fft_prime_product(ctx, plan->primes[i], lpts, va, vb, vc, m, k, n, sc, method, s));
     }
     const dim3 grid(grid_of(m * n * sc, 128));
-    if (r <= 16 && nl <= 8)
+    const int nlm = crt_width(r, nl).nlm;
+    if (nlm == 2)
+        hipLaunchKernelGGL((mp_crt_small_kernel<4, 2>), grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc,
+                           plan->d_const, r, nl, c);
+    else if (nlm == 4)
+        hipLaunchKernelGGL((mp_crt_small_kernel<8, 4>), grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc,
+                           plan->d_const, r, nl, c);
+    else if (nlm == 8)
         hipLaunchKernelGGL((mp_crt_small_kernel<16, 8>), grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc,
                            plan->d_const, r, nl, c);
-    else if (r <= 32 && nl <= 16)
+    else if (nlm == 16)
         hipLaunchKernelGGL((mp_crt_small_kernel<32, 16>), grid, dim3(128), 0, s, (const uint32_t *)rc, m * n, sc,
                            plan->d_const, r, nl, c);
     else

@@ -258,3 +258,36 @@ def test_random_wide_moduli(ctx):
         got = [sum(int(w[r, i]) << (32 * i) for i in range(nl)) for r in range(w.shape[0])]
         want = [x % P for x in naive_int(A, B, m, k, n, sa, sb)]
         assert got == want, case
+
+
+def test_narrow_moduli_crt_widths(ctx):
+    """lbx_polymat_mul_mp for moduli of one to four words, the sizes whose CRT
+    runs in narrow register arithmetic.  With the bound 2 P^2 k min(sa,sb) of
+    these shapes, P = 3, 2^32 - 5 and 2^32 + 15 need at most 4 primes (2-word
+    kernel), 2^64 - 59 and the three-word P 5 to 8 (4-word kernel), the
+    four-word P 9 (8-word kernel).  Word extremes of P, coefficients at P - 1
+    (the largest Garner sums) and random ones, both pointwise forms, against
+    Python integers."""
+    import random
+    rnd = random.Random(20261205)
+    moduli = [(3, 1), ((1 << 32) - 5, 1), ((1 << 32) + 15, 2), ((1 << 64) - 59, 2),
+              (rnd.randrange(1 << 80, 1 << 96), 3), (rnd.randrange(1 << 120, 1 << 128), 4)]
+    for P, nl in moduli:
+        for extreme in (False, True):
+            for method in (lbx.METHOD_PLAIN_U64, lbx.METHOD_INT8_MFMA):
+                m, k, n, sa, sb = 3, 17, 2, 40, 33
+                A = [P - 1 if extreme else rnd.randrange(P) for _ in range(m * k * sa)]
+                B = [P - 1 if extreme else rnd.randrange(P) for _ in range(k * n * sb)]
+                say("narrow mp", nl, P.bit_length(), extreme, method)
+
+                def words(vals):
+                    return np.array([(v >> (32 * i)) & 0xFFFFFFFF for v in vals for i in range(nl)],
+                                    dtype=np.uint32)
+                a = PolynomialMatrix(m, k, sa, data=words(A), nlimbs=nl)
+                b = PolynomialMatrix(k, n, sb, data=words(B), nlimbs=nl)
+                c = PolynomialMatrix(m, n, sa + sb - 1, nlimbs=nl)
+                lbx.mul_mp(P, c, a, b, ctx=ctx, method=method)
+                w = c.data.reshape(-1, nl)
+                got = [sum(int(w[r, i]) << (32 * i) for i in range(nl)) for r in range(w.shape[0])]
+                want = [x % P for x in naive_int(A, B, m, k, n, sa, sb)]
+                assert got == want, (P, nl, extreme, method)

@@ -32,9 +32,9 @@ def value(words):
     return sum(int(w) << (32 * i) for i, w in enumerate(words))
 
 
-@pytest.mark.parametrize("nl", [8, 16])
+@pytest.mark.parametrize("nl", [2, 4, 8, 16])
 def test_remainder_matches_big_integers(lib, nl):
-    fn = lib.lbx_test_rem8 if nl == 8 else lib.lbx_test_rem16
+    fn = {2: lib.lbx_test_rem2, 4: lib.lbx_test_rem4, 8: lib.lbx_test_rem8, 16: lib.lbx_test_rem16}[nl]
     rng = random.Random(nl)
     checked = 0
     for _ in range(6000):
@@ -67,3 +67,24 @@ def test_a_wrong_divisor_is_caught(lib):
     v = (ctypes.c_uint32 * nl)(*limbs(P << (sh - 1), nl))  # shifted one bit short
     lib.lbx_test_rem8(u, v)
     assert value(list(u)[:nl]) != (X % P) << sh
+
+
+def test_two_by_one_division(lib):
+    """The quotient digit of the remainder's estimate: (u1 2^32 + u0) / d for a
+    normalized d and u1 < d through the reciprocal floor((2^64-1)/d) - 2^32
+    (Moller and Granlund 2011, Algorithm 4), against Python integers at the
+    extremes of d, u1 and u0 and at random."""
+    lib.lbx_test_div21.restype = ctypes.c_uint32
+    lib.lbx_test_div21.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.POINTER(ctypes.c_uint32)]
+    rng = random.Random(77)
+    ds = [1 << 31, (1 << 31) + 1, 0xFFFFFFFF, 0xFFFFFFFE, 0xC0000000] + [rng.getrandbits(31) | (1 << 31)
+                                                                        for _ in range(200)]
+    r = ctypes.c_uint32()
+    for d in ds:
+        u1s = [0, 1, d - 1, d // 2] + [rng.randrange(d) for _ in range(20)]
+        for u1 in u1s:
+            for u0 in [0, 1, 0xFFFFFFFF, d - 1, d, rng.getrandbits(32)]:
+                num = (u1 << 32) | u0
+                q = lib.lbx_test_div21(u1, u0, d, ctypes.byref(r))
+                assert (q, r.value) == divmod(num, d), (hex(u1), hex(u0), hex(d))
