@@ -1,8 +1,11 @@
 // Host build of the word-size modular arithmetic the kernels use
 // (linbox_amd/csrc/modarith.h, linbox_amd/csrc/pw_math.h) for
 // tests/test_modarith_edges.py, which checks every result against Python
-// integers.  Each entry point computes one operation on plain integers.
+// integers.  Each entry point computes one operation on plain integers, or
+// (lbx_test_dif) one forward transform through a twiddle table.
+#include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../linbox_amd/csrc/pw_math.h"
 
@@ -70,3 +73,39 @@ uint32_t lbx_test_plain_s_dot(const uint32_t *a, const uint32_t *b, int n, uint3
 
 int32_t lbx_test_centre(uint32_t x, uint32_t p) { return centre(x, p); }
 }
+
+extern "C" {
+
+// A length-2^L DIF transform (natural order in, bit-reversed out) of x in
+// place, as the forward kernels compute it: pow_w twiddle table (the subarray
+// of span h at N - 2h, entry j = w^(j N/2h)), canonical butterflies, twiddle
+// products in the signed Montgomery form the forward tables hold when mont is
+// set, else in Shoup's form.  Returns 0, or 1 if L is out of range.
+int lbx_test_dif(uint32_t *x, unsigned L, uint32_t p, uint32_t w, int mont)
+{
+    if (L < 1 || L > 16) return 1;
+    const size_t n = (size_t)1 << L;
+    std::vector<uint32_t> pw(n - 1);
+    pw[0] = 1;
+    for (size_t i = 1; i < n / 2; ++i) pw[i] = (uint32_t)((uint64_t)pw[i - 1] * w % p);
+    size_t idx = n / 2;
+    for (size_t kk = 2; kk <= n / 2; kk <<= 1)
+        for (size_t i = 0; i < n / 2; i += kk, idx++) pw[idx] = pw[i];
+    const uint32_t pinv = inv_mod32(p);
+    for (size_t h = n / 2; h >= 1; h >>= 1) {
+        for (size_t blk = 0; blk < n; blk += 2 * h)
+            for (size_t j = 0; j < h; ++j) {
+                const uint32_t a = x[blk + j], b = x[blk + j + h], t = pw[n - 2 * h + j];
+                x[blk + j] = add_mod(a, b, p);
+                if (mont) {
+                    const uint32_t wm = mont_wm(t, p);
+                    x[blk + j + h] = mont_mul((int32_t)(a - b), wm, mont_wq(wm, pinv), p);
+                } else {
+                    x[blk + j + h] = shoup_mul(a + (p - b), t, shoup_pre(t, p), p);
+                }
+            }
+        if (h == 1) break;
+    }
+    return 0;
+}
+}  // extern "C"

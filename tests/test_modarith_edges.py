@@ -104,6 +104,47 @@ def test_signed_montgomery(lib):
             assert lib.lbx_test_mont_mul(a - b, w, p) == (a - b) * w % p, (p, a, b, w)
 
 
+def _root(p, L):
+    """compute_primitive_root (fft-utils.h:83-104): t^((p-1)/2^L) for the first t = 2, 3, ...
+    whose result has order exactly 2^L."""
+    t = 2
+    while True:
+        w = pow(t, (p - 1) >> L, p)
+        if pow(w, 1 << (L - 1), p) != 1:
+            return w
+        t += 1
+
+
+def test_forward_transform_with_montgomery_tables(lib):
+    """The forward transforms read twiddle tables of signed Montgomery pairs in the
+    pow_w layout: a DIF through such a table equals the DIF through the Shoup
+    table for every length to 2^14 and the four primes, and for lengths to 2^8
+    both equal the direct evaluation the reference's FFT test checks
+    (test-fft.C:157-176: output i, in bit-reversed order, is P(w^i))."""
+    lib.lbx_test_dif.restype = ctypes.c_int
+    lib.lbx_test_dif.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint, ctypes.c_uint32,
+                                 ctypes.c_uint32, ctypes.c_int]
+    rng = random.Random(13)
+    for p in PRIMES:
+        for L in range(1, 15):
+            n = 1 << L
+            w = _root(p, L)
+            x = [rng.randrange(p) for _ in range(n)]
+            xm = (ctypes.c_uint32 * n)(*x)
+            xs = (ctypes.c_uint32 * n)(*x)
+            assert lib.lbx_test_dif(xm, L, p, w, 1) == 0
+            assert lib.lbx_test_dif(xs, L, p, w, 0) == 0
+            assert list(xm) == list(xs), (p, L)
+            if L <= 8:
+                rev = [int(format(i, "0%db" % L)[::-1], 2) for i in range(n)]
+                for i in range(n):
+                    wi = pow(w, i, p)
+                    acc = 0
+                    for c in reversed(x):
+                        acc = (acc * wi + c) % p
+                    assert xm[rev[i]] == acc, (p, L, i)
+
+
 def test_balanced_digits(lib):
     for p in PRIMES:
         for x in edges(p) + [p // 3, 2 * p // 3]:
